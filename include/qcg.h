@@ -1,0 +1,122 @@
+/*
+ * qcg.h — C ABI of the MI355X (gfx950) Fock-build engine.
+ *
+ * Each entry point replaces one function of ExaPsi/IQCP qc-core's hot path (paths relative to
+ * crates/qc-core/src/).  Plain C types only: caller-owned host buffers (or device pointers for
+ * the *_device variants), every call returns a status code (0 = OK), qcg_last_error() gives the
+ * message.  One context per host thread; device memory is owned by the context.
+ *
+ * Matrix layout: nbf x nbf, row-major.  D, J, K, F, S, H are symmetric, so row- and
+ * column-major (qc-core's nalgebra column-major ScfOutput fields, scf/mod.rs:1567-1569) agree.
+ * Basis-function order: shells in the order given to qcg_set_basis, Cartesian components
+ * xx, xy, xz, yy, yz, zz (integrals/cartesian.rs:113-141).  Coordinates in bohr.
+ */
+#ifndef QCG_H
+#define QCG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (ScfError / RysError / BoysError mapping, scf/mod.rs:73-109, rys/mod.rs:82-114). */
+#define QCG_OK 0
+#define QCG_INVALID_ARG 1    /* DimensionMismatch / InvalidBasisSize / InvalidOrder / NegativeArgument */
+#define QCG_UNSUPPORTED_L 2  /* AngularMomentumTooHigh (l > 2) */
+#define QCG_NUMERICAL 3      /* NumericalInstability / SingularOverlap */
+#define QCG_DEVICE 4         /* HIP runtime error */
+#define QCG_NOT_CONVERGED 5  /* ScfError::NotConverged */
+#define QCG_NO_BASIS 6
+#define QCG_NO_GRID 7
+
+/* Precision of the two-electron kernels. */
+#define QCG_PREC_FP64 0   /* every surviving quartet in FP64 */
+#define QCG_PREC_MIXED 1  /* quartets with Q_ab*Q_cd < fp32_cutoff in FP32, the rest FP64; FP64 accumulation */
+
+typedef struct qcg_context qcg_context;
+
+/* Convergence/acceleration settings: ScfConfig + ConvergenceProfile (scf/mod.rs:123-262). */
+typedef struct qcg_scf_config {
+    double energy_tol;   /* Tight 1e-10, Medium 1e-8, Loose 1e-6 */
+    double density_tol;  /* Tight 1e-8,  Medium 1e-6, Loose 1e-4 */
+    int max_iterations;  /* default 100 */
+    int use_diis;        /* AO-basis DIIS */
+    int diis_size;       /* default 8 */
+    int diis_start;      /* default 1 */
+    double damp;         /* default 0 */
+    int damp_start;      /* default 5 */
+    double level_shift;  /* default 0 */
+} qcg_scf_config;
+
+/* Scalar part of ScfOutput (scf/mod.rs:629-650). */
+typedef struct qcg_scf_result {
+    int converged;
+    int iterations;
+    double energy_total;
+    double energy_electronic;
+    double energy_nuclear;
+    double last_delta_e;
+    double last_rms_d;
+    double energy_xc;    /* KS only */
+    double energy_j;     /* KS only: 1/2 Tr DJ */
+    double energy_k;     /* KS only: -1/4 a Tr DK */
+    double energy_1e;    /* KS only: Tr DH */
+} qcg_scf_result;
+
+/* ---- context ---------------------------------------------------------------------------- */
+int qcg_create(qcg_context** ctx, int device);
+void qcg_destroy(qcg_context* ctx);
+const char* qcg_last_error(const qcg_context* ctx);
+
+/* Basis set: replaces BasisSet::build (basis/mod.rs:164) output.  Shell s has angular momentum
+ * l[s] (0..2), primitives [prim_off[s], prim_off[s]+nprim[s]) of exps/coefs (raw contraction
+ * coefficients, basis/primitives.rs:232-237), centre centers[3s..3s+2], atom shell_atom[s].
+ * Z/xyz: nuclei (for V, E_nuc, grids, gradients).  Builds shell pairs, Schwarz bounds on the
+ * device (compute_schwarz_bounds, integrals/eri/mod.rs:1263) and the class-sorted pair lists. */
+int qcg_set_basis(qcg_context* ctx, int nshell, const int* l, const int* nprim, const int* prim_off,
+                  const double* exps, const double* coefs, const double* centers, const int* shell_atom,
+                  int natom, const int* Z, const double* xyz);
+
+/* precision: QCG_PREC_*; fp32_cutoff: Schwarz-product cutoff below which quartets run in FP32
+ * (default 1e-6); schwarz_threshold: SCHWARZ_THRESHOLD (integrals/eri/mod.rs:1258, default 1e-12). */
+int qcg_set_options(qcg_context* ctx, int precision, double fp32_cutoff, double schwarz_threshold);
+int qcg_nbf(const qcg_context* ctx, int* nbf);
+
+/* ---- special functions (batched, one argument per lane) -------------------------------- */
+/* boys_eval_all (boys/mod.rs:327): F[i*(m_max+1)+m] = F_m(T[i]). */
+int qcg_boys_eval_all(qcg_context* ctx, int m_max, int n, const double* T, double* F);
+/* rys_roots (rys/mod.rs:253): roots u = t^2 in [0,1) and weights, [i*nroots+k]. nroots 1..7. */
+int qcg_rys_roots(qcg_context* ctx, int nroots, int n, const double* T, double* roots, double* weights);
+
+/* ---- two-electron integrals and J/K ----------------------------------------------------- */
+/* compute_schwarz_bounds (integrals/eri/mod.rs:1263): Q[i*nshell+j] = sqrt(max |(ij|ij)|). */
+int qcg_schwarz_bounds(qcg_context* ctx, double* Q);
+/* Integer screening outputs: unique shell quartets (8-fold) with Q_ij*Q_kl >= threshold, and how
+ * many of those run in FP32 under QCG_PREC_MIXED. */
+int qcg_screen_counts(qcg_context* ctx, int64_t* n_kept, int64_t* n_fp32);
+/* eri_compressed (integrals/eri/mod.rs:992): out has npair(npair+1)/2 doubles, npair = nbf(nbf+1)/2,
+ * index eri_index (scf/mod.rs:608-613). */
+int qcg_eri_compressed(qcg_context* ctx, double* out);
+/* build_jk (dft/ks_scf.rs:1369) / build_coulomb (:1306) / build_exchange (:1342), integral-direct:
+ * J_mn = sum_ls D_ls (mn|ls), K_mn = sum_ls D_ls (ml|ns).  J or K may be NULL. */
+int qcg_build_jk(qcg_context* ctx, const double* D, double* J, double* K);
+/* build_fock (scf/mod.rs:825): F = H + J - K/2. */
+int qcg_build_fock(qcg_context* ctx, const double* H, const double* D, double* F);
+/* Device-pointer variant of qcg_build_jk on the given HIP stream (NULL = context stream);
+ * asynchronous with respect to the host. */
+int qcg_build_jk_device(qcg_context* ctx, const double* dD, double* dJ, double* dK, void* stream);
+
+/* ---- one-electron terms and SCF ---------------------------------------------------------- */
+/* overlap_matrix (integrals/overlap.rs:647), kinetic and nuclear attraction (hcore_matrix,
+ * integrals/hcore.rs:103 = T + V).  Any output may be NULL. */
+int qcg_one_electron(qcg_context* ctx, double* S, double* T, double* V);
+/* rhf_scf (scf/mod.rs:1363) with the integral-direct GPU Fock build.  D_init (optional) seeds the
+ * density (rhf_scf_with_guess, :1610).  Outputs (optional): D, C (columns = MOs), eps, F. */
+int qcg_rhf_scf(qcg_context* ctx, const qcg_scf_config* cfg, const double* D_init, qcg_scf_result* res,
+                double* D, double* C, double* eps, double* F);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QCG_H */

@@ -1,0 +1,280 @@
+// Direct J/K build and compressed-tensor kernels for all quartet classes whose bra pair class
+// is QCG_CB (this file is compiled once per bra class, -DQCG_CB=0..5, so the six units build
+// in parallel).  Pair class c <-> (la, lb): 0 (s,s) 1 (p,s) 2 (p,p) 3 (d,s) 4 (d,p) 5 (d,d).
+//
+// Work decomposition of the J/K build (one launch per quartet class, bra class >= ket class):
+//   * one workgroup per bra shell pair P (pairs of a class are sorted by Schwarz bound, desc);
+//   * its threads stride over the ket pairs Q of the ket class; for bra class == ket class only
+//     Q at or before P in the sorted list (each unordered pair-pair once = 8-fold symmetry);
+//   * the surviving kets are the prefix where Q_P*Q_Q >= threshold (sorted list, monotone
+//     product), found by binary search on exactly the product the reference tests
+//     (q_ij * q_kl < SCHWARZ_THRESHOLD, integrals/eri/mod.rs:1008-1023); in mixed precision the
+//     prefix splits into an FP64 range (Q_P*Q_Q >= cutoff) and an FP32 range;
+//   * digestion: J_P in registers + wave reduction, J_Q by FP64 atomics, K rows of the bra
+//     shells (all 4 K terms of a quartet land in rows of shell a or b) accumulated in LDS with
+//     ds_add_f64 and flushed once per workgroup.  J = Jacc + Jacc^T, K = Kacc + Kacc^T after.
+//   * symmetry factors: 1/2 for a == b shells, 1/2 for c == d shells, 1/2 for P == Q, folded
+//     together with the component normalisation into the density sub-blocks.
+// With the symmetric density this reproduces build_fock's G = J - K/2 (scf/mod.rs:825-881) and
+// build_jk (dft/ks_scf.rs:1369-1394).
+#include "eri_device.h"
+#include "eri_launch.h"
+
+#ifndef QCG_CB
+#error "compile with -DQCG_CB=<bra pair class>"
+#endif
+
+namespace qcg {
+
+// first index q in [0, qmax) with Qp * ket[q].Q < thr (ket sorted by Q descending)
+__device__ __forceinline__ int first_below(const PairRec* __restrict__ ket, int qmax, double Qp, double thr) {
+    int lo = 0, hi = qmax;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (Qp * ket[mid].Q >= thr) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int LA, int LB, int LC, int LD, typename R>
+__global__ __launch_bounds__(256) void jk_kernel(
+    const PairRec* __restrict__ bra, int nbra, const PairRec* __restrict__ ket, int nket, int diag,
+    const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_tab, const double* __restrict__ D, int nbf,
+    double* __restrict__ Jacc, double* __restrict__ Kacc, double thr_lo, double thr_hi, int do_j, int do_k,
+    int use_lds) {
+    using S = EriShape<LA, LB, LC, LD>;
+    constexpr bool U = S::kUnroll;
+    extern __shared__ double klds[];
+    __shared__ int s_q0, s_q1;
+    const int pidx = diag ? (nbra - 1 - (int)blockIdx.x) : (int)blockIdx.x;
+    const PairRec P = bra[pidx];
+    const int tid = threadIdx.x;
+    const int krows = S::NA + S::NB;
+    if (do_k && use_lds)
+        for (int i = tid; i < krows * nbf; i += blockDim.x) klds[i] = 0.0;
+    if (tid == 0) {
+        const int qmax = diag ? pidx + 1 : nket;
+        s_q0 = (thr_hi > 1e300) ? 0 : first_below(ket, qmax, P.Q, thr_hi);
+        s_q1 = first_below(ket, qmax, P.Q, thr_lo);
+    }
+    __syncthreads();
+    const int q0 = s_q0, q1 = s_q1;
+
+    // bra density block scaled by the bra component norms
+    double Dps[S::NAB], jp[S::NAB];
+    for_n<U, S::NAB>([&](int ab) {
+        const int a = ab / S::NB, b = ab % S::NB;
+        Dps[ab] = D[(size_t)(P.bfa + a) * nbf + P.bfb + b] * kCompNorm[LA][a] * kCompNorm[LB][b];
+        jp[ab] = 0.0;
+    });
+    const double deg_p = (P.sa == P.sb) ? 0.5 : 1.0;
+
+    for (int q = q0 + tid; q < q1; q += blockDim.x) {
+        const PairRec Q = ket[q];
+        const double qq = P.Q * Q.Q;
+        if (qq < thr_lo || qq >= thr_hi) continue;
+        R eri[S::NQ];
+        for_n<U, S::NQ>([&](int i) { eri[i] = R(0); });
+        contract_quartet<LA, LB, LC, LD, R>(P, Q, pp, rys_tab, eri);
+        const double deg = deg_p * ((Q.sa == Q.sb) ? 0.5 : 1.0) * ((diag && q == pidx) ? 0.5 : 1.0);
+        if (do_j) {
+            double Dqs[S::NCD];
+            for_n<U, S::NCD>([&](int cd) {
+                const int c = cd / S::ND, d = cd % S::ND;
+                Dqs[cd] = D[(size_t)(Q.bfa + c) * nbf + Q.bfb + d] * kCompNorm[LC][c] * kCompNorm[LD][d];
+            });
+            const double fj = 2.0 * deg;
+            for_n<U, S::NAB>([&](int ab) {
+                double s = 0.0;
+                for_n<U, S::NCD>([&](int cd) { s += Dqs[cd] * (double)eri[ab * S::NCD + cd]; });
+                const int a = ab / S::NB, b = ab % S::NB;
+                jp[ab] += fj * kCompNorm[LA][a] * kCompNorm[LB][b] * s;
+            });
+            for_n<U, S::NCD>([&](int cd) {
+                double s = 0.0;
+                for_n<U, S::NAB>([&](int ab) { s += Dps[ab] * (double)eri[ab * S::NCD + cd]; });
+                const int c = cd / S::ND, d = cd % S::ND;
+                atomicAdd(&Jacc[(size_t)(Q.bfa + c) * nbf + Q.bfb + d], fj * kCompNorm[LC][c] * kCompNorm[LD][d] * s);
+            });
+        }
+        if (do_k) {
+            // Kacc_ik += D_jl v, Kacc_jk += D_il v, Kacc_il += D_jk v, Kacc_jl += D_ik v
+            // (i in a, j in b, k in c, l in d); every D sub-block pre-scaled by the norms it sums over.
+            double Dbd[S::NB * S::ND], Dad[S::NA * S::ND], Dbc[S::NB * S::NC], Dac[S::NA * S::NC];
+            for_n<U, S::NB * S::ND>([&](int i) {
+                const int b = i / S::ND, d = i % S::ND;
+                Dbd[i] = D[(size_t)(P.bfb + b) * nbf + Q.bfb + d] * kCompNorm[LB][b] * kCompNorm[LD][d];
+            });
+            for_n<U, S::NA * S::ND>([&](int i) {
+                const int a = i / S::ND, d = i % S::ND;
+                Dad[i] = D[(size_t)(P.bfa + a) * nbf + Q.bfb + d] * kCompNorm[LA][a] * kCompNorm[LD][d];
+            });
+            for_n<U, S::NB * S::NC>([&](int i) {
+                const int b = i / S::NC, c = i % S::NC;
+                Dbc[i] = D[(size_t)(P.bfb + b) * nbf + Q.bfa + c] * kCompNorm[LB][b] * kCompNorm[LC][c];
+            });
+            for_n<U, S::NA * S::NC>([&](int i) {
+                const int a = i / S::NC, c = i % S::NC;
+                Dac[i] = D[(size_t)(P.bfa + a) * nbf + Q.bfa + c] * kCompNorm[LA][a] * kCompNorm[LC][c];
+            });
+            auto kadd = [&](int row_local, int row_global, int col, double val) {
+                if (use_lds) atomicAdd(&klds[(size_t)row_local * nbf + col], val);
+                else atomicAdd(&Kacc[(size_t)row_global * nbf + col], val);
+            };
+#define VIDX(a, b, c, d) ((((a) * S::NB + (b)) * S::NC + (c)) * S::ND + (d))
+            for_n<U, S::NA * S::NC>([&](int ac) {
+                const int a = ac / S::NC, c = ac % S::NC;
+                double s = 0.0;
+                for_n<U, S::NB * S::ND>([&](int bd) {
+                    const int b = bd / S::ND, d = bd % S::ND;
+                    s += Dbd[bd] * (double)eri[VIDX(a, b, c, d)];
+                });
+                kadd(a, P.bfa + a, Q.bfa + c, deg * kCompNorm[LA][a] * kCompNorm[LC][c] * s);
+            });
+            for_n<U, S::NB * S::NC>([&](int bc) {
+                const int b = bc / S::NC, c = bc % S::NC;
+                double s = 0.0;
+                for_n<U, S::NA * S::ND>([&](int ad) {
+                    const int a = ad / S::ND, d = ad % S::ND;
+                    s += Dad[ad] * (double)eri[VIDX(a, b, c, d)];
+                });
+                kadd(S::NA + b, P.bfb + b, Q.bfa + c, deg * kCompNorm[LB][b] * kCompNorm[LC][c] * s);
+            });
+            for_n<U, S::NA * S::ND>([&](int ad) {
+                const int a = ad / S::ND, d = ad % S::ND;
+                double s = 0.0;
+                for_n<U, S::NB * S::NC>([&](int bc) {
+                    const int b = bc / S::NC, c = bc % S::NC;
+                    s += Dbc[bc] * (double)eri[VIDX(a, b, c, d)];
+                });
+                kadd(a, P.bfa + a, Q.bfb + d, deg * kCompNorm[LA][a] * kCompNorm[LD][d] * s);
+            });
+            for_n<U, S::NB * S::ND>([&](int bd) {
+                const int b = bd / S::ND, d = bd % S::ND;
+                double s = 0.0;
+                for_n<U, S::NA * S::NC>([&](int ac) {
+                    const int a = ac / S::NC, c = ac % S::NC;
+                    s += Dac[ac] * (double)eri[VIDX(a, b, c, d)];
+                });
+                kadd(S::NA + b, P.bfb + b, Q.bfb + d, deg * kCompNorm[LB][b] * kCompNorm[LD][d] * s);
+            });
+#undef VIDX
+        }
+    }
+    // J_P: wave reduction, one atomic per component per wave
+    if (do_j) {
+        for_n<true, S::NAB>([&](int ab) {
+            double s = jp[ab];
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+            if ((tid & 63) == 0 && s != 0.0) {
+                const int a = ab / S::NB, b = ab % S::NB;
+                atomicAdd(&Jacc[(size_t)(P.bfa + a) * nbf + P.bfb + b], s);
+            }
+        });
+    }
+    if (do_k && use_lds) {
+        __syncthreads();
+        for (int i = tid; i < krows * nbf; i += blockDim.x) {
+            const double val = klds[i];
+            if (val != 0.0) {
+                const int r = i / nbf, col = i - r * nbf;
+                const int row = r < S::NA ? P.bfa + r : P.bfb + (r - S::NA);
+                atomicAdd(&Kacc[(size_t)row * nbf + col], val);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ int64_t pair_index64(int64_t i, int64_t j) {
+    return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
+}
+
+// The 8-fold compressed tensor (eri_compressed, integrals/eri/mod.rs:992-1070; index
+// pair_index/eri_index :946-960).  Every element of a surviving quartet is written once.
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void eri_tensor_kernel(
+    const PairRec* __restrict__ bra, int nbra, const PairRec* __restrict__ ket, int nket, int diag,
+    const PrimPair<double>* __restrict__ pp, const double* __restrict__ rys_tab, double* __restrict__ out,
+    double thr) {
+    using S = EriShape<LA, LB, LC, LD>;
+    constexpr bool U = S::kUnroll;
+    __shared__ int s_q1;
+    const int pidx = diag ? (nbra - 1 - (int)blockIdx.x) : (int)blockIdx.x;
+    const PairRec P = bra[pidx];
+    if (threadIdx.x == 0) s_q1 = first_below(ket, diag ? pidx + 1 : nket, P.Q, thr);
+    __syncthreads();
+    const int q1 = s_q1;
+    for (int q = threadIdx.x; q < q1; q += blockDim.x) {
+        const PairRec Q = ket[q];
+        if (P.Q * Q.Q < thr) continue;
+        double eri[S::NQ];
+        for_n<U, S::NQ>([&](int i) { eri[i] = 0.0; });
+        contract_quartet<LA, LB, LC, LD, double>(P, Q, pp, rys_tab, eri);
+#pragma unroll 1
+        for (int idx = 0; idx < S::NQ; ++idx) {
+            const int d = idx % S::ND, c = (idx / S::ND) % S::NC, b = (idx / S::NCD) % S::NB, a = idx / (S::NCD * S::NB);
+            const int i = P.bfa + a, j = P.bfb + b;
+            if (P.sa == P.sb && i < j) continue;
+            const int k = Q.bfa + c, l = Q.bfb + d;
+            if (Q.sa == Q.sb && k < l) continue;
+            const int64_t ij = pair_index64(i, j), kl = pair_index64(k, l);
+            if (diag && q == pidx && ij < kl) continue;
+            const int64_t hi = ij > kl ? ij : kl, lo = ij > kl ? kl : ij;
+            out[hi * (hi + 1) / 2 + lo] =
+                eri[idx] * kCompNorm[LA][a] * kCompNorm[LB][b] * kCompNorm[LC][c] * kCompNorm[LD][d];
+        }
+    }
+}
+
+#define QCG_LA(c) ((c) == 0 ? 0 : (c) <= 2 ? 1 : 2)
+#define QCG_LB(c) ((c) == 0 || (c) == 1 || (c) == 3 ? 0 : ((c) == 2 || (c) == 4) ? 1 : 2)
+
+template <int CB, int CK, typename R>
+static hipError_t launch_jk_t(const JkLaunch& a, hipStream_t st) {
+    constexpr int LA = QCG_LA(CB), LB = QCG_LB(CB), LC = QCG_LA(CK), LD = QCG_LB(CK);
+    using S = EriShape<LA, LB, LC, LD>;
+    size_t lds = a.do_k ? (size_t)(S::NA + S::NB) * a.nbf * sizeof(double) : 0;
+    int use_lds = (lds > 0 && lds <= 96 * 1024) ? 1 : 0;
+    if (!use_lds) lds = 0;
+    if (a.nbra <= 0 || a.nket <= 0) return hipSuccess;
+    hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, R>), dim3(a.nbra), dim3(256), lds, st, a.bra, a.nbra, a.ket,
+                       a.nket, a.diag, (const PrimPair<R>*)a.pp, (const R*)a.rys, a.D, a.nbf, a.Jacc, a.Kacc,
+                       a.thr_lo, a.thr_hi, a.do_j, a.do_k, use_lds);
+    return hipGetLastError();
+}
+
+template <int CB, int CK>
+static hipError_t launch_tensor_t(const JkLaunch& a, double* out, hipStream_t st) {
+    constexpr int LA = QCG_LA(CB), LB = QCG_LB(CB), LC = QCG_LA(CK), LD = QCG_LB(CK);
+    if (a.nbra <= 0 || a.nket <= 0) return hipSuccess;
+    hipLaunchKernelGGL((eri_tensor_kernel<LA, LB, LC, LD>), dim3(a.nbra), dim3(256), 0, st, a.bra, a.nbra, a.ket,
+                       a.nket, a.diag, (const PrimPair<double>*)a.pp, (const double*)a.rys, out, a.thr_lo);
+    return hipGetLastError();
+}
+
+#define QCG_CAT2(a, b) a##b
+#define QCG_CAT(a, b) QCG_CAT2(a, b)
+
+hipError_t QCG_CAT(launch_jk_cb, QCG_CB)(int ck, bool fp32, const JkLaunch& a, hipStream_t st) {
+    switch (ck) {
+#define QCG_K(CK) \
+    case CK: if constexpr (CK <= QCG_CB) return fp32 ? launch_jk_t<QCG_CB, CK, float>(a, st) : launch_jk_t<QCG_CB, CK, double>(a, st); break;
+        QCG_K(0) QCG_K(1) QCG_K(2) QCG_K(3) QCG_K(4) QCG_K(5)
+#undef QCG_K
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t QCG_CAT(launch_tensor_cb, QCG_CB)(int ck, const JkLaunch& a, double* out, hipStream_t st) {
+    switch (ck) {
+#define QCG_K(CK) \
+    case CK: if constexpr (CK <= QCG_CB) return launch_tensor_t<QCG_CB, CK>(a, out, st); break;
+        QCG_K(0) QCG_K(1) QCG_K(2) QCG_K(3) QCG_K(4) QCG_K(5)
+#undef QCG_K
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace qcg

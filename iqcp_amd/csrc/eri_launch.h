@@ -1,0 +1,73 @@
+// Host-side launch interface of the two-electron kernels.
+#pragma once
+#include "qcg_types.h"
+
+namespace qcg {
+
+struct JkLaunch {
+    const PairRec* bra;
+    int nbra;
+    const PairRec* ket;
+    int nket;
+    int diag;           // bra class == ket class: only ket positions <= bra position
+    const void* pp;     // PrimPair<double> or PrimPair<float>
+    const void* rys;    // Rys table (double or float)
+    const double* D;    // density, nbf x nbf, symmetric
+    int nbf;
+    double* Jacc;
+    double* Kacc;
+    double thr_lo;      // process quartets with thr_lo <= Q_P Q_Q < thr_hi
+    double thr_hi;
+    int do_j, do_k;
+};
+
+hipError_t launch_jk_cb0(int ck, bool fp32, const JkLaunch& a, hipStream_t st);
+hipError_t launch_jk_cb1(int ck, bool fp32, const JkLaunch& a, hipStream_t st);
+hipError_t launch_jk_cb2(int ck, bool fp32, const JkLaunch& a, hipStream_t st);
+hipError_t launch_jk_cb3(int ck, bool fp32, const JkLaunch& a, hipStream_t st);
+hipError_t launch_jk_cb4(int ck, bool fp32, const JkLaunch& a, hipStream_t st);
+hipError_t launch_jk_cb5(int ck, bool fp32, const JkLaunch& a, hipStream_t st);
+hipError_t launch_tensor_cb0(int ck, const JkLaunch& a, double* out, hipStream_t st);
+hipError_t launch_tensor_cb1(int ck, const JkLaunch& a, double* out, hipStream_t st);
+hipError_t launch_tensor_cb2(int ck, const JkLaunch& a, double* out, hipStream_t st);
+hipError_t launch_tensor_cb3(int ck, const JkLaunch& a, double* out, hipStream_t st);
+hipError_t launch_tensor_cb4(int ck, const JkLaunch& a, double* out, hipStream_t st);
+hipError_t launch_tensor_cb5(int ck, const JkLaunch& a, double* out, hipStream_t st);
+
+inline hipError_t launch_jk(int cb, int ck, bool fp32, const JkLaunch& a, hipStream_t st) {
+    switch (cb) {
+        case 0: return launch_jk_cb0(ck, fp32, a, st);
+        case 1: return launch_jk_cb1(ck, fp32, a, st);
+        case 2: return launch_jk_cb2(ck, fp32, a, st);
+        case 3: return launch_jk_cb3(ck, fp32, a, st);
+        case 4: return launch_jk_cb4(ck, fp32, a, st);
+        case 5: return launch_jk_cb5(ck, fp32, a, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+inline hipError_t launch_eri_tensor(int cb, int ck, const JkLaunch& a, double* out, hipStream_t st) {
+    switch (cb) {
+        case 0: return launch_tensor_cb0(ck, a, out, st);
+        case 1: return launch_tensor_cb1(ck, a, out, st);
+        case 2: return launch_tensor_cb2(ck, a, out, st);
+        case 3: return launch_tensor_cb3(ck, a, out, st);
+        case 4: return launch_tensor_cb4(ck, a, out, st);
+        case 5: return launch_tensor_cb5(ck, a, out, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_schwarz(int c, const PairRec* pairs, int n, const PrimPair<double>* pp, const double* rys,
+                          double* Q, hipStream_t st);
+hipError_t launch_fill(double* x, size_t n, double v, hipStream_t st);
+hipError_t launch_symmetrize(const double* Jacc, const double* Kacc, double* J, double* K, int nbf, hipStream_t st);
+hipError_t launch_fock_combine(const double* H, const double* Jacc, const double* Kacc, double* F, int nbf,
+                               double kfac, hipStream_t st);
+hipError_t launch_boys_batch(int m, int n, const double* T, double* F, int* status, hipStream_t st);
+hipError_t launch_rys_batch(int nroots, int n, const double* T, const double* tab, double* x, double* w,
+                            hipStream_t st);
+hipError_t launch_rys_batch_f32(int nroots, int n, const float* T, const float* tab, float* x, float* w,
+                                hipStream_t st);
+
+}  // namespace qcg

@@ -1,0 +1,137 @@
+// Small kernels around the J/K build: Schwarz bounds, Boys/Rys batches, fills, J/K symmetrisation.
+#include "boys.h"
+#include "eri_device.h"
+#include "eri_launch.h"
+
+namespace qcg {
+
+// Schwarz bound per pair: sqrt(max |(ab|ab)|) (compute_schwarz_bounds, integrals/eri/mod.rs:1263-1290)
+template <int LA, int LB>
+__global__ __launch_bounds__(128) void schwarz_kernel(const PairRec* __restrict__ pairs, int n,
+                                                      const PrimPair<double>* __restrict__ pp,
+                                                      const double* __restrict__ rys_tab, double* __restrict__ Q) {
+    using S = EriShape<LA, LB, LA, LB>;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const PairRec P = pairs[i];
+    double eri[S::NQ];
+    for_n<S::kUnroll, S::NQ>([&](int k) { eri[k] = 0.0; });
+    contract_quartet<LA, LB, LA, LB, double>(P, P, pp, rys_tab, eri);
+    double mx = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < S::NQ; ++k) {
+        const int d = k % S::NB, c = (k / S::NB) % S::NA, b = (k / S::NAB) % S::NB, a = k / (S::NAB * S::NB);
+        const double nrm = kCompNorm[LA][a] * kCompNorm[LB][b] * kCompNorm[LA][c] * kCompNorm[LB][d];
+        mx = fmax(mx, fabs(eri[k] * nrm));
+    }
+    Q[i] = sqrt(mx);
+}
+
+hipError_t launch_schwarz(int c, const PairRec* pairs, int n, const PrimPair<double>* pp, const double* rys,
+                          double* Q, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    dim3 g((n + 127) / 128), b(128);
+    switch (c) {
+        case 0: hipLaunchKernelGGL((schwarz_kernel<0, 0>), g, b, 0, st, pairs, n, pp, rys, Q); break;
+        case 1: hipLaunchKernelGGL((schwarz_kernel<1, 0>), g, b, 0, st, pairs, n, pp, rys, Q); break;
+        case 2: hipLaunchKernelGGL((schwarz_kernel<1, 1>), g, b, 0, st, pairs, n, pp, rys, Q); break;
+        case 3: hipLaunchKernelGGL((schwarz_kernel<2, 0>), g, b, 0, st, pairs, n, pp, rys, Q); break;
+        case 4: hipLaunchKernelGGL((schwarz_kernel<2, 1>), g, b, 0, st, pairs, n, pp, rys, Q); break;
+        case 5: hipLaunchKernelGGL((schwarz_kernel<2, 2>), g, b, 0, st, pairs, n, pp, rys, Q); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+__global__ void fill_kernel(double* __restrict__ x, size_t n, double v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) x[i] = v;
+}
+
+hipError_t launch_fill(double* x, size_t n, double v, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(256), 0, st, x, n, v);
+    return hipGetLastError();
+}
+
+// J = Jacc + Jacc^T, K = Kacc + Kacc^T
+__global__ void symmetrize_kernel(const double* __restrict__ Jacc, const double* __restrict__ Kacc,
+                                  double* __restrict__ J, double* __restrict__ K, int nbf) {
+    const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nbf) return;
+    if (J) J[(size_t)i * nbf + j] = Jacc[(size_t)i * nbf + j] + Jacc[(size_t)j * nbf + i];
+    if (K) K[(size_t)i * nbf + j] = Kacc[(size_t)i * nbf + j] + Kacc[(size_t)j * nbf + i];
+}
+
+hipError_t launch_symmetrize(const double* Jacc, const double* Kacc, double* J, double* K, int nbf, hipStream_t st) {
+    hipLaunchKernelGGL(symmetrize_kernel, dim3((nbf + 127) / 128, nbf), dim3(128), 0, st, Jacc, Kacc, J, K, nbf);
+    return hipGetLastError();
+}
+
+// F = H + J - kfac K  (build_fock, scf/mod.rs:825-881 with kfac = 1/2; hybrid form dft/ks_scf.rs:1960)
+__global__ void fock_combine_kernel(const double* __restrict__ H, const double* __restrict__ Jacc,
+                                    const double* __restrict__ Kacc, double* __restrict__ F, int nbf, double kfac) {
+    const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nbf) return;
+    const size_t ij = (size_t)i * nbf + j, ji = (size_t)j * nbf + i;
+    F[ij] = H[ij] + (Jacc[ij] + Jacc[ji]) - kfac * (Kacc[ij] + Kacc[ji]);
+}
+
+hipError_t launch_fock_combine(const double* H, const double* Jacc, const double* Kacc, double* F, int nbf,
+                               double kfac, hipStream_t st) {
+    hipLaunchKernelGGL(fock_combine_kernel, dim3((nbf + 127) / 128, nbf), dim3(128), 0, st, H, Jacc, Kacc, F, nbf,
+                       kfac);
+    return hipGetLastError();
+}
+
+// Batched Boys F_0..F_m (boys_eval_all, boys/mod.rs:327): one T per lane.
+__global__ void boys_batch_kernel(int m, int n, const double* __restrict__ T, double* __restrict__ F,
+                                  int* __restrict__ status) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double f[51];
+    const int rc = boys_all(m, T[i], f);
+    if (rc) {
+        atomicOr(status, 1);
+        for (int k = 0; k <= m; ++k) F[(size_t)i * (m + 1) + k] = 0.0;
+        return;
+    }
+    for (int k = 0; k <= m; ++k) F[(size_t)i * (m + 1) + k] = f[k];
+}
+
+hipError_t launch_boys_batch(int m, int n, const double* T, double* F, int* status, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(boys_batch_kernel, dim3((n + 255) / 256), dim3(256), 0, st, m, n, T, F, status);
+    return hipGetLastError();
+}
+
+// Batched Rys nodes/weights (rys_roots, rys/mod.rs:253): one T per lane.
+template <typename R>
+__global__ void rys_batch_kernel(int nroots, int n, const R* __restrict__ T, const R* __restrict__ tab,
+                                 R* __restrict__ x, R* __restrict__ w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    R xr[QCG_RYS_NMAX], wr[QCG_RYS_NMAX];
+    rys_eval_n<R>(nroots, T[i], tab, xr, wr);
+    for (int k = 0; k < nroots; ++k) {
+        x[(size_t)i * nroots + k] = xr[k];
+        w[(size_t)i * nroots + k] = wr[k];
+    }
+}
+
+hipError_t launch_rys_batch(int nroots, int n, const double* T, const double* tab, double* x, double* w,
+                            hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((rys_batch_kernel<double>), dim3((n + 255) / 256), dim3(256), 0, st, nroots, n, T, tab, x, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_rys_batch_f32(int nroots, int n, const float* T, const float* tab, float* x, float* w,
+                                hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((rys_batch_kernel<float>), dim3((n + 255) / 256), dim3(256), 0, st, nroots, n, T, tab, x, w);
+    return hipGetLastError();
+}
+
+}  // namespace qcg

@@ -1,0 +1,26 @@
+// Host-side basis description and one-electron integrals.
+#pragma once
+#include <vector>
+
+#include "qcg_common.h"
+
+namespace qcg {
+
+// Flat shell list (bohr), as passed through qcg_set_basis.
+struct HostBasis {
+    int nshell = 0, nbf = 0, natom = 0;
+    std::vector<int> l, nprim, prim_off, shell_atom, bf_off, Z;
+    std::vector<double> exps, coefs, centers, xyz;
+};
+
+// S, T, V (any may be null), full nbf x nbf row-major.
+void one_electron_matrices(const HostBasis& B, double* S, double* T, double* V);
+
+// Nuclear derivatives of the one-electron terms contracted with densities, accumulated into
+// grad[3*natom]: sum_mn D_mn dH_mn/dR - sum_mn W_mn dS_mn/dR.
+void one_electron_gradient(const HostBasis& B, const double* D, const double* W, double* grad);
+
+double nuclear_repulsion(const HostBasis& B);
+void nuclear_repulsion_gradient(const HostBasis& B, double* grad);
+
+}  // namespace qcg

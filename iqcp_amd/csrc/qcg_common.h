@@ -1,0 +1,52 @@
+// Shared definitions for the qcg library (host + device).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/qcg.h"  // status codes
+
+#define QCG_HD __host__ __device__ __forceinline__
+
+namespace qcg {
+
+constexpr int kMaxL = 2;             // s, p, d (all configured basis sets)
+constexpr int kMaxCart = 6;          // (kMaxL+1)(kMaxL+2)/2
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kSchwarzDefault = 1e-12;   // SCHWARZ_THRESHOLD (integrals/eri/mod.rs:1258)
+constexpr double kPrimScreen = 1e-15;       // k_ij, k_ij*k_kl screens (integrals/eri/mod.rs:603,611)
+
+QCG_HD constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+QCG_HD constexpr int pair_class(int la, int lb) { return la * (la + 1) / 2 + lb; }  // la >= lb
+constexpr int kNumPairClasses = pair_class(kMaxL, kMaxL) + 1;  // 6
+
+// Cartesian component powers in qc-core order (integrals/cartesian.rs:113-141):
+// for i = l..0, for j = l-i..0, k = l-i-j.
+QCG_HD constexpr int cart_x(int l, int c) {
+    int n = 0;
+    for (int i = l; i >= 0; --i)
+        for (int j = l - i; j >= 0; --j) {
+            if (n == c) return i;
+            ++n;
+        }
+    return 0;
+}
+QCG_HD constexpr int cart_y(int l, int c) {
+    int n = 0;
+    for (int i = l; i >= 0; --i)
+        for (int j = l - i; j >= 0; --j) {
+            if (n == c) return j;
+            ++n;
+        }
+    return 0;
+}
+QCG_HD constexpr int cart_z(int l, int c) { return l - cart_x(l, c) - cart_y(l, c); }
+
+QCG_HD constexpr double dfact_odd(int n) {  // (2n-1)!! with (-1)!! = 1
+    double r = 1.0;
+    for (int k = 2 * n - 1; k > 1; k -= 2) r *= k;
+    return r;
+}
+
+}  // namespace qcg

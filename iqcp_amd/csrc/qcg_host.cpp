@@ -1,0 +1,506 @@
+// Host side of the qcg library: context, basis/pair setup, C ABI (include/qcg.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/qcg.h"
+#include "eri_launch.h"
+#include "onee.h"
+#include "rys.h"
+#include "scf.h"
+
+using namespace qcg;
+
+struct qcg_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool has_basis = false;
+    HostBasis basis;
+    int precision = QCG_PREC_FP64;
+    double fp32_cutoff = 1e-6;
+    double schwarz_thr = kSchwarzDefault;
+    // device tables
+    double* d_rys = nullptr;
+    float* d_rys_f = nullptr;
+    // pairs
+    std::vector<PairRec> pairs;  // grouped by class, sorted by Q desc within a class
+    int class_off[kNumPairClasses + 1] = {0};
+    PairRec* d_pairs = nullptr;
+    PrimPair<double>* d_pp = nullptr;
+    PrimPair<float>* d_ppf = nullptr;
+    std::vector<double> Qshell;  // nshell x nshell
+    // work buffers
+    double *d_D = nullptr, *d_J = nullptr, *d_K = nullptr, *d_Jacc = nullptr, *d_Kacc = nullptr;
+    double *d_H = nullptr, *d_F = nullptr;
+    std::vector<void*> owned;
+
+    template <typename T>
+    hipError_t alloc(T** p, size_t n) {
+        hipError_t e = hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T));
+        if (e == hipSuccess) owned.push_back((void*)*p);
+        return e;
+    }
+    void release(void* p) {
+        if (!p) return;
+        auto it = std::find(owned.begin(), owned.end(), p);
+        if (it != owned.end()) owned.erase(it);
+        (void)hipFree(p);
+    }
+    void free_basis() {
+        for (void** p : {(void**)&d_pairs, (void**)&d_pp, (void**)&d_ppf, (void**)&d_D, (void**)&d_J, (void**)&d_K,
+                         (void**)&d_Jacc, (void**)&d_Kacc, (void**)&d_H, (void**)&d_F}) {
+            release(*p);
+            *p = nullptr;
+        }
+        has_basis = false;
+    }
+};
+
+namespace {
+
+int fail(qcg_context* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                        \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail(ctx, QCG_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+double radial_norm(double a, int l) {
+    const double t = 2.0 * a / kPi;
+    const double base = std::sqrt(std::sqrt(t * t * t));
+    const double fa = 4.0 * a;
+    return base * (l == 0 ? 1.0 : l == 1 ? std::sqrt(fa) : l == 2 ? fa : std::pow(fa, l / 2.0));
+}
+
+}  // namespace
+
+double qcg::nuclear_repulsion(const HostBasis& B) {
+    // compute_nuclear_repulsion (basis/mod.rs:375-391)
+    double e = 0.0;
+    for (int i = 0; i < B.natom; ++i)
+        for (int j = i + 1; j < B.natom; ++j) {
+            const double dx = B.xyz[3 * i] - B.xyz[3 * j], dy = B.xyz[3 * i + 1] - B.xyz[3 * j + 1],
+                         dz = B.xyz[3 * i + 2] - B.xyz[3 * j + 2];
+            const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (r > 1e-10) e += (double)B.Z[i] * B.Z[j] / r;
+        }
+    return e;
+}
+
+extern "C" {
+
+int qcg_create(qcg_context** out, int device) {
+    if (!out) return QCG_INVALID_ARG;
+    *out = nullptr;
+    auto c = std::make_unique<qcg_context>();
+    c->device = device;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return QCG_DEVICE;
+    if (device < 0 || device >= ndev) return QCG_INVALID_ARG;
+    if (hipSetDevice(device) != hipSuccess) return QCG_DEVICE;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return QCG_DEVICE;
+    const size_t nt = sizeof(qcg_rys_coef) / sizeof(double);
+    if (c->alloc(&c->d_rys, nt) != hipSuccess || c->alloc(&c->d_rys_f, nt) != hipSuccess) return QCG_DEVICE;
+    std::vector<float> tf(nt);
+    for (size_t i = 0; i < nt; ++i) tf[i] = (float)qcg_rys_coef[i];
+    if (hipMemcpy(c->d_rys, qcg_rys_coef, nt * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return QCG_DEVICE;
+    if (hipMemcpy(c->d_rys_f, tf.data(), nt * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return QCG_DEVICE;
+    *out = c.release();
+    return QCG_OK;
+}
+
+void qcg_destroy(qcg_context* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (void* p : c->owned) (void)hipFree(p);
+    c->owned.clear();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* qcg_last_error(const qcg_context* c) { return c ? c->err.c_str() : "null context"; }
+
+int qcg_set_options(qcg_context* c, int precision, double fp32_cutoff, double schwarz_threshold) {
+    if (!c) return QCG_INVALID_ARG;
+    if (precision != QCG_PREC_FP64 && precision != QCG_PREC_MIXED) return fail(c, QCG_INVALID_ARG, "precision");
+    if (!(fp32_cutoff > 0.0) || !(schwarz_threshold > 0.0)) return fail(c, QCG_INVALID_ARG, "thresholds must be > 0");
+    c->precision = precision;
+    c->fp32_cutoff = fp32_cutoff;
+    c->schwarz_thr = schwarz_threshold;
+    return QCG_OK;
+}
+
+int qcg_nbf(const qcg_context* c, int* nbf) {
+    if (!c || !nbf) return QCG_INVALID_ARG;
+    if (!c->has_basis) return QCG_NO_BASIS;
+    *nbf = c->basis.nbf;
+    return QCG_OK;
+}
+
+int qcg_set_basis(qcg_context* c, int nshell, const int* l, const int* nprim, const int* prim_off,
+                  const double* exps, const double* coefs, const double* centers, const int* shell_atom,
+                  int natom, const int* Z, const double* xyz) {
+    if (!c) return QCG_INVALID_ARG;
+    if (nshell <= 0 || !l || !nprim || !prim_off || !exps || !coefs || !centers || natom <= 0 || !Z || !xyz)
+        return fail(c, QCG_INVALID_ARG, "qcg_set_basis: empty or null input");
+    HIPCHK(c, hipSetDevice(c->device));
+    c->free_basis();
+    HostBasis& B = c->basis;
+    B = HostBasis();
+    B.nshell = nshell;
+    B.natom = natom;
+    int nprim_tot = 0;
+    for (int s = 0; s < nshell; ++s) {
+        if (l[s] < 0 || l[s] > kMaxL) return fail(c, QCG_UNSUPPORTED_L, "angular momentum > 2 not supported");
+        if (nprim[s] <= 0) return fail(c, QCG_INVALID_ARG, "shell without primitives");
+        nprim_tot = std::max(nprim_tot, prim_off[s] + nprim[s]);
+    }
+    B.l.assign(l, l + nshell);
+    B.nprim.assign(nprim, nprim + nshell);
+    B.prim_off.assign(prim_off, prim_off + nshell);
+    B.exps.assign(exps, exps + nprim_tot);
+    B.coefs.assign(coefs, coefs + nprim_tot);
+    B.centers.assign(centers, centers + 3 * nshell);
+    B.shell_atom.assign(shell_atom ? shell_atom : l, (shell_atom ? shell_atom : l) + nshell);
+    if (!shell_atom) std::fill(B.shell_atom.begin(), B.shell_atom.end(), 0);
+    B.Z.assign(Z, Z + natom);
+    B.xyz.assign(xyz, xyz + 3 * natom);
+    B.bf_off.resize(nshell);
+    int bf = 0;
+    for (int s = 0; s < nshell; ++s) {
+        B.bf_off[s] = bf;
+        bf += ncart(l[s]);
+    }
+    B.nbf = bf;
+
+    // shell pairs (si >= sj), relabelled so that l(sa) >= l(sb), grouped by class
+    std::vector<std::vector<PairRec>> by_class(kNumPairClasses);
+    std::vector<PrimPair<double>> pp;
+    for (int si = 0; si < nshell; ++si)
+        for (int sj = 0; sj <= si; ++sj) {
+            const int sa = l[si] >= l[sj] ? si : sj, sb = l[si] >= l[sj] ? sj : si;
+            PairRec r{};
+            const double* A = centers + 3 * sa;
+            const double* Bc = centers + 3 * sb;
+            r.ax = A[0]; r.ay = A[1]; r.az = A[2];
+            r.abx = A[0] - Bc[0]; r.aby = A[1] - Bc[1]; r.abz = A[2] - Bc[2];
+            r.sa = sa; r.sb = sb;
+            r.bfa = B.bf_off[sa]; r.bfb = B.bf_off[sb];
+            r.atom_a = B.shell_atom[sa]; r.atom_b = B.shell_atom[sb];
+            r.pp_off = (int)pp.size();
+            const double ab2 = r.abx * r.abx + r.aby * r.aby + r.abz * r.abz;
+            for (int i = 0; i < nprim[sa]; ++i)
+                for (int j = 0; j < nprim[sb]; ++j) {
+                    const double a = exps[prim_off[sa] + i], b = exps[prim_off[sb] + j];
+                    const double kab = std::exp(-(a * b / (a + b)) * ab2);
+                    if (kab < kPrimScreen) continue;  // integrals/eri/mod.rs:603
+                    PrimPair<double> q{};
+                    q.p = a + b;
+                    q.px = (a * A[0] + b * Bc[0]) / q.p;
+                    q.py = (a * A[1] + b * Bc[1]) / q.p;
+                    q.pz = (a * A[2] + b * Bc[2]) / q.p;
+                    q.kab = kab;
+                    q.ck = kab * coefs[prim_off[sa] + i] * coefs[prim_off[sb] + j] * radial_norm(a, l[sa]) *
+                           radial_norm(b, l[sb]);
+                    pp.push_back(q);
+                }
+            r.npp = (int)pp.size() - r.pp_off;
+            by_class[pair_class(l[sa], l[sb])].push_back(r);
+        }
+    std::vector<PairRec> all;
+    for (int k = 0; k < kNumPairClasses; ++k) {
+        c->class_off[k] = (int)all.size();
+        all.insert(all.end(), by_class[k].begin(), by_class[k].end());
+    }
+    c->class_off[kNumPairClasses] = (int)all.size();
+    const int npairs = (int)all.size();
+    std::vector<PrimPair<float>> ppf(pp.size());
+    for (size_t i = 0; i < pp.size(); ++i) {
+        ppf[i].p = (float)pp[i].p; ppf[i].px = (float)pp[i].px; ppf[i].py = (float)pp[i].py;
+        ppf[i].pz = (float)pp[i].pz; ppf[i].kab = (float)pp[i].kab; ppf[i].ck = (float)pp[i].ck;
+    }
+    HIPCHK(c, c->alloc(&c->d_pp, pp.size()));
+    HIPCHK(c, c->alloc(&c->d_ppf, pp.size()));
+    HIPCHK(c, c->alloc(&c->d_pairs, all.size()));
+    HIPCHK(c, hipMemcpy(c->d_pp, pp.data(), pp.size() * sizeof(pp[0]), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_ppf, ppf.data(), ppf.size() * sizeof(ppf[0]), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_pairs, all.data(), all.size() * sizeof(PairRec), hipMemcpyHostToDevice));
+    // Schwarz bounds on the device
+    double* dQ = nullptr;
+    HIPCHK(c, c->alloc(&dQ, npairs));
+    for (int k = 0; k < kNumPairClasses; ++k) {
+        const int n = c->class_off[k + 1] - c->class_off[k];
+        HIPCHK(c, launch_schwarz(k, c->d_pairs + c->class_off[k], n, c->d_pp, c->d_rys, dQ + c->class_off[k], c->stream));
+    }
+    std::vector<double> Q(npairs);
+    HIPCHK(c, hipMemcpyAsync(Q.data(), dQ, npairs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->release(dQ);
+    c->Qshell.assign((size_t)nshell * nshell, 0.0);
+    for (int i = 0; i < npairs; ++i) {
+        all[i].Q = Q[i];
+        c->Qshell[(size_t)all[i].sa * nshell + all[i].sb] = Q[i];
+        c->Qshell[(size_t)all[i].sb * nshell + all[i].sa] = Q[i];
+    }
+    for (int k = 0; k < kNumPairClasses; ++k)
+        std::stable_sort(all.begin() + c->class_off[k], all.begin() + c->class_off[k + 1],
+                         [](const PairRec& x, const PairRec& y) { return x.Q > y.Q; });
+    HIPCHK(c, hipMemcpy(c->d_pairs, all.data(), all.size() * sizeof(PairRec), hipMemcpyHostToDevice));
+    c->pairs = all;
+    const size_t n2 = (size_t)B.nbf * B.nbf;
+    HIPCHK(c, c->alloc(&c->d_D, n2));
+    HIPCHK(c, c->alloc(&c->d_J, n2));
+    HIPCHK(c, c->alloc(&c->d_K, n2));
+    HIPCHK(c, c->alloc(&c->d_Jacc, n2));
+    HIPCHK(c, c->alloc(&c->d_Kacc, n2));
+    HIPCHK(c, c->alloc(&c->d_H, n2));
+    HIPCHK(c, c->alloc(&c->d_F, n2));
+    c->has_basis = true;
+    return QCG_OK;
+}
+
+int qcg_schwarz_bounds(qcg_context* c, double* Q) {
+    if (!c || !Q) return QCG_INVALID_ARG;
+    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    std::memcpy(Q, c->Qshell.data(), c->Qshell.size() * sizeof(double));
+    return QCG_OK;
+}
+
+int qcg_screen_counts(qcg_context* c, int64_t* n_kept, int64_t* n_fp32) {
+    if (!c) return QCG_INVALID_ARG;
+    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    int64_t kept = 0, f32 = 0;
+    auto first_below = [&](int off, int qmax, double Qp, double thr) {
+        int lo = 0, hi = qmax;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (Qp * c->pairs[off + mid].Q >= thr) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    for (int cb = 0; cb < kNumPairClasses; ++cb)
+        for (int ck = 0; ck <= cb; ++ck) {
+            const int nb = c->class_off[cb + 1] - c->class_off[cb], nk = c->class_off[ck + 1] - c->class_off[ck];
+            for (int p = 0; p < nb; ++p) {
+                const double Qp = c->pairs[c->class_off[cb] + p].Q;
+                const int qmax = (cb == ck) ? p + 1 : nk;
+                const int e = first_below(c->class_off[ck], qmax, Qp, c->schwarz_thr);
+                kept += e;
+                if (c->fp32_cutoff > c->schwarz_thr)
+                    f32 += e - std::min(e, first_below(c->class_off[ck], qmax, Qp, c->fp32_cutoff));
+            }
+        }
+    if (n_kept) *n_kept = kept;
+    if (n_fp32) *n_fp32 = f32;
+    return QCG_OK;
+}
+
+static int run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStream_t st) {
+    const int nbf = c->basis.nbf;
+    const size_t n2 = (size_t)nbf * nbf;
+    const int do_j = dJ != nullptr, do_k = dK != nullptr;
+    if (do_j) HIPCHK(c, launch_fill(c->d_Jacc, n2, 0.0, st));
+    if (do_k) HIPCHK(c, launch_fill(c->d_Kacc, n2, 0.0, st));
+    const bool mixed = c->precision == QCG_PREC_MIXED && c->fp32_cutoff > c->schwarz_thr;
+    for (int cb = kNumPairClasses - 1; cb >= 0; --cb)
+        for (int ck = cb; ck >= 0; --ck) {
+            JkLaunch a{};
+            a.bra = c->d_pairs + c->class_off[cb];
+            a.nbra = c->class_off[cb + 1] - c->class_off[cb];
+            a.ket = c->d_pairs + c->class_off[ck];
+            a.nket = c->class_off[ck + 1] - c->class_off[ck];
+            a.diag = cb == ck;
+            a.D = dD;
+            a.nbf = nbf;
+            a.Jacc = c->d_Jacc;
+            a.Kacc = c->d_Kacc;
+            a.do_j = do_j;
+            a.do_k = do_k;
+            a.pp = c->d_pp;
+            a.rys = c->d_rys;
+            a.thr_lo = mixed ? c->fp32_cutoff : c->schwarz_thr;
+            a.thr_hi = 1e308 * 10.0;
+            HIPCHK(c, launch_jk(cb, ck, false, a, st));
+            if (mixed) {
+                a.pp = c->d_ppf;
+                a.rys = c->d_rys_f;
+                a.thr_lo = c->schwarz_thr;
+                a.thr_hi = c->fp32_cutoff;
+                HIPCHK(c, launch_jk(cb, ck, true, a, st));
+            }
+        }
+    HIPCHK(c, launch_symmetrize(c->d_Jacc, c->d_Kacc, dJ, dK, nbf, st));
+    return QCG_OK;
+}
+
+int qcg_build_jk_device(qcg_context* c, const double* dD, double* dJ, double* dK, void* stream) {
+    if (!c || !dD) return QCG_INVALID_ARG;
+    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    HIPCHK(c, hipSetDevice(c->device));
+    return run_jk(c, dD, dJ, dK, stream ? (hipStream_t)stream : c->stream);
+}
+
+int qcg_build_jk(qcg_context* c, const double* D, double* J, double* K) {
+    if (!c || !D) return QCG_INVALID_ARG;
+    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t n2 = (size_t)c->basis.nbf * c->basis.nbf;
+    HIPCHK(c, hipMemcpyAsync(c->d_D, D, n2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int rc = run_jk(c, c->d_D, J ? c->d_J : nullptr, K ? c->d_K : nullptr, c->stream);
+    if (rc) return rc;
+    if (J) HIPCHK(c, hipMemcpyAsync(J, c->d_J, n2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (K) HIPCHK(c, hipMemcpyAsync(K, c->d_K, n2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return QCG_OK;
+}
+
+int qcg_build_fock(qcg_context* c, const double* H, const double* D, double* F) {
+    if (!c || !H || !D || !F) return QCG_INVALID_ARG;
+    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nbf = c->basis.nbf;
+    const size_t n2 = (size_t)nbf * nbf;
+    HIPCHK(c, hipMemcpyAsync(c->d_D, D, n2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_H, H, n2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int rc = run_jk(c, c->d_D, c->d_J, c->d_K, c->stream);
+    if (rc) return rc;
+    HIPCHK(c, launch_fock_combine(c->d_H, c->d_Jacc, c->d_Kacc, c->d_F, nbf, 0.5, c->stream));
+    HIPCHK(c, hipMemcpyAsync(F, c->d_F, n2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return QCG_OK;
+}
+
+int qcg_eri_compressed(qcg_context* c, double* out) {
+    if (!c || !out) return QCG_INVALID_ARG;
+    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t nbf = c->basis.nbf;
+    const int64_t npair = nbf * (nbf + 1) / 2;
+    const size_t neri = (size_t)(npair * (npair + 1) / 2);
+    double* dout = nullptr;
+    HIPCHK(c, c->alloc(&dout, neri));
+    HIPCHK(c, launch_fill(dout, neri, 0.0, c->stream));
+    for (int cb = 0; cb < kNumPairClasses; ++cb)
+        for (int ck = 0; ck <= cb; ++ck) {
+            JkLaunch a{};
+            a.bra = c->d_pairs + c->class_off[cb];
+            a.nbra = c->class_off[cb + 1] - c->class_off[cb];
+            a.ket = c->d_pairs + c->class_off[ck];
+            a.nket = c->class_off[ck + 1] - c->class_off[ck];
+            a.diag = cb == ck;
+            a.pp = c->d_pp;
+            a.rys = c->d_rys;
+            a.thr_lo = c->schwarz_thr;
+            HIPCHK(c, launch_eri_tensor(cb, ck, a, dout, c->stream));
+        }
+    HIPCHK(c, hipMemcpyAsync(out, dout, neri * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->release(dout);
+    return QCG_OK;
+}
+
+int qcg_boys_eval_all(qcg_context* c, int m_max, int n, const double* T, double* F) {
+    if (!c || (n > 0 && (!T || !F)) || n < 0) return QCG_INVALID_ARG;
+    if (m_max < 0 || m_max > 50) return fail(c, QCG_INVALID_ARG, "BoysError::InvalidOrder");
+    for (int i = 0; i < n; ++i)
+        if (!(T[i] >= 0.0)) return fail(c, QCG_INVALID_ARG, "BoysError::NegativeArgument");
+    if (n == 0) return QCG_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    double *dT = nullptr, *dF = nullptr;
+    int* dst = nullptr;
+    HIPCHK(c, c->alloc(&dT, n));
+    HIPCHK(c, c->alloc(&dF, (size_t)n * (m_max + 1)));
+    HIPCHK(c, c->alloc(&dst, 1));
+    HIPCHK(c, hipMemcpyAsync(dT, T, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(dst, 0, sizeof(int), c->stream));
+    HIPCHK(c, launch_boys_batch(m_max, n, dT, dF, dst, c->stream));
+    int st = 0;
+    HIPCHK(c, hipMemcpyAsync(F, dF, (size_t)n * (m_max + 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&st, dst, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->release(dT);
+    c->release(dF);
+    c->release(dst);
+    return st ? fail(c, QCG_INVALID_ARG, "boys: invalid argument") : QCG_OK;
+}
+
+int qcg_rys_roots(qcg_context* c, int nroots, int n, const double* T, double* roots, double* weights) {
+    if (!c || n < 0 || (n > 0 && (!T || !roots || !weights))) return QCG_INVALID_ARG;
+    if (nroots < 1 || nroots > QCG_RYS_NMAX) return fail(c, QCG_INVALID_ARG, "RysError::InvalidOrder");
+    for (int i = 0; i < n; ++i)
+        if (!(T[i] >= 0.0)) return fail(c, QCG_INVALID_ARG, "RysError::InvalidArgument");
+    if (n == 0) return QCG_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    double *dT = nullptr, *dx = nullptr, *dw = nullptr;
+    HIPCHK(c, c->alloc(&dT, n));
+    HIPCHK(c, c->alloc(&dx, (size_t)n * nroots));
+    HIPCHK(c, c->alloc(&dw, (size_t)n * nroots));
+    HIPCHK(c, hipMemcpyAsync(dT, T, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_rys_batch(nroots, n, dT, c->d_rys, dx, dw, c->stream));
+    HIPCHK(c, hipMemcpyAsync(roots, dx, (size_t)n * nroots * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(weights, dw, (size_t)n * nroots * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->release(dT);
+    c->release(dx);
+    c->release(dw);
+    return QCG_OK;
+}
+
+int qcg_one_electron(qcg_context* c, double* S, double* T, double* V) {
+    if (!c) return QCG_INVALID_ARG;
+    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    one_electron_matrices(c->basis, S, T, V);
+    return QCG_OK;
+}
+
+namespace {
+struct RhfUser {
+    qcg_context* c;
+    const double* H;
+};
+int rhf_fock_cb(void* u, const double* D, double* F, double* e_elec) {
+    RhfUser* r = (RhfUser*)u;
+    int rc = qcg_build_fock(r->c, r->H, D, F);
+    if (rc) return rc;
+    const size_t n2 = (size_t)r->c->basis.nbf * r->c->basis.nbf;
+    double e = 0.0;  // compute_electronic_energy (scf/mod.rs:898): 1/2 sum D (H + F)
+    for (size_t i = 0; i < n2; ++i) e += D[i] * (r->H[i] + F[i]);
+    *e_elec = 0.5 * e;
+    return QCG_OK;
+}
+}  // namespace
+
+int qcg_rhf_scf(qcg_context* c, const qcg_scf_config* cfg, const double* D_init, qcg_scf_result* res, double* D,
+                double* C, double* eps, double* F) {
+    if (!c || !cfg) return QCG_INVALID_ARG;
+    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    const HostBasis& B = c->basis;
+    int nelec = 0;
+    for (int z : B.Z) nelec += z;
+    if (nelec % 2) return fail(c, QCG_INVALID_ARG, "RHF needs an even electron count");
+    const size_t n2 = (size_t)B.nbf * B.nbf;
+    std::vector<double> S(n2), T(n2), V(n2), H(n2);
+    one_electron_matrices(B, S.data(), T.data(), V.data());
+    for (size_t i = 0; i < n2; ++i) H[i] = T[i] + V[i];
+    RhfUser u{c, H.data()};
+    qcg_scf_result r{};
+    int rc = scf_driver(B.nbf, nelec / 2, S.data(), H.data(), nuclear_repulsion(B), *cfg, rhf_fock_cb, &u, D_init, &r,
+                        D, C, eps, F);
+    if (res) *res = r;
+    if (rc == QCG_NOT_CONVERGED) return fail(c, rc, "ScfError::NotConverged");
+    if (rc == QCG_NUMERICAL) return fail(c, rc, "ScfError::SingularOverlap");
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,60 @@
+// Rys quadrature nodes/weights from the piecewise fits in rys_table.inc.
+//
+// Same quantities as qc-core's rys_roots (crates/qc-core/src/rys/mod.rs:253): n nodes
+// u_i = t_i^2 in [0,1) and weights with sum_i w_i u_i^k = F_k(T) for k < 2n.  The reference
+// solves for them per call (Boys moments -> Schmidt -> Hessenberg QR, rys/mod.rs:468-583);
+// here they are read from fits of the exact rule (see gen_rys_table.py), so every lane
+// follows the same instruction stream.  Accuracy of the FP64 fits: ~1e-15 relative.
+#pragma once
+#include "qcg_common.h"
+#include "rys_table.inc"
+
+namespace qcg {
+
+constexpr int kRysStride = QCG_RYS_DEG + 1;
+
+QCG_HD constexpr int rys_tri(int n) { return n * (n - 1) / 2; }
+
+// tab: qcg_rys_coef (double) or its float copy, resident in device memory.
+template <typename R, int N>
+QCG_HD void rys_eval(R T, const R* __restrict__ tab, R* x, R* w) {
+    static_assert(N >= 1 && N <= QCG_RYS_NMAX, "Rys order out of table range");
+    if (T >= (R)qcg_rys_tmax[N]) {
+        const R it = R(1) / T;
+        const R rs = R(1) / sqrt(T);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            x[i] = (R)qcg_rys_hx[rys_tri(N) + i] * it;
+            w[i] = (R)qcg_rys_hw[rys_tri(N) + i] * rs;
+        }
+        return;
+    }
+    const int k = (int)(T * (R)(1.0 / QCG_RYS_H));
+    const R s = (T - (R)(k * QCG_RYS_H)) * (R)(2.0 / QCG_RYS_H) - R(1);
+    const R* c = tab + qcg_rys_off[N] + k * (2 * N * kRysStride);
+#pragma unroll
+    for (int i = 0; i < 2 * N; ++i) {
+        const R* ci = c + i * kRysStride;
+        R v = ci[QCG_RYS_DEG];
+#pragma unroll
+        for (int j = QCG_RYS_DEG - 1; j >= 0; --j) v = v * s + ci[j];
+        if (i < N) x[i] = v; else w[i - N] = v;
+    }
+}
+
+// Runtime-order variant for host code and batch kernels.
+template <typename R>
+QCG_HD int rys_eval_n(int n, R T, const R* __restrict__ tab, R* x, R* w) {
+    switch (n) {
+        case 1: rys_eval<R, 1>(T, tab, x, w); return 0;
+        case 2: rys_eval<R, 2>(T, tab, x, w); return 0;
+        case 3: rys_eval<R, 3>(T, tab, x, w); return 0;
+        case 4: rys_eval<R, 4>(T, tab, x, w); return 0;
+        case 5: rys_eval<R, 5>(T, tab, x, w); return 0;
+        case 6: rys_eval<R, 6>(T, tab, x, w); return 0;
+        case 7: rys_eval<R, 7>(T, tab, x, w); return 0;
+        default: return QCG_INVALID_ARG;
+    }
+}
+
+}  // namespace qcg
