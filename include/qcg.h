@@ -82,6 +82,8 @@ int qcg_set_basis(qcg_context* ctx, int nshell, const int* l, const int* nprim, 
  * (default 1e-6); schwarz_threshold: SCHWARZ_THRESHOLD (integrals/eri/mod.rs:1258, default 1e-12). */
 int qcg_set_options(qcg_context* ctx, int precision, double fp32_cutoff, double schwarz_threshold);
 int qcg_nbf(const qcg_context* ctx, int* nbf);
+/* Molecular charge (electron count = sum Z - charge), PresetSystem.nelec (scf/mod.rs:284-310). */
+int qcg_set_charge(qcg_context* ctx, int charge);
 
 /* ---- special functions (batched, one argument per lane) -------------------------------- */
 /* boys_eval_all (boys/mod.rs:327): F[i*(m_max+1)+m] = F_m(T[i]). */

@@ -23,6 +23,7 @@ struct qcg_context {
     int precision = QCG_PREC_FP64;
     double fp32_cutoff = 1e-6;
     double schwarz_thr = kSchwarzDefault;
+    int charge = 0;
     // device tables
     double* d_rys = nullptr;
     float* d_rys_f = nullptr;
@@ -137,6 +138,12 @@ int qcg_set_options(qcg_context* c, int precision, double fp32_cutoff, double sc
     c->precision = precision;
     c->fp32_cutoff = fp32_cutoff;
     c->schwarz_thr = schwarz_threshold;
+    return QCG_OK;
+}
+
+int qcg_set_charge(qcg_context* c, int charge) {
+    if (!c) return QCG_INVALID_ARG;
+    c->charge = charge;
     return QCG_OK;
 }
 
@@ -486,9 +493,9 @@ int qcg_rhf_scf(qcg_context* c, const qcg_scf_config* cfg, const double* D_init,
     if (!c || !cfg) return QCG_INVALID_ARG;
     if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
     const HostBasis& B = c->basis;
-    int nelec = 0;
+    int nelec = -c->charge;
     for (int z : B.Z) nelec += z;
-    if (nelec % 2) return fail(c, QCG_INVALID_ARG, "RHF needs an even electron count");
+    if (nelec <= 0 || nelec % 2) return fail(c, QCG_INVALID_ARG, "RHF needs an even electron count");
     const size_t n2 = (size_t)B.nbf * B.nbf;
     std::vector<double> S(n2), T(n2), V(n2), H(n2);
     one_electron_matrices(B, S.data(), T.data(), V.data());

@@ -1,0 +1,218 @@
+"""Python binding of libqcg.so (the C ABI in include/qcg.h).
+
+Mirrors the qc-core entry points the GPU path replaces (paths relative to crates/qc-core/src):
+  boys_eval_all (boys/mod.rs:327), rys_roots (rys/mod.rs:253), compute_schwarz_bounds /
+  eri_compressed (integrals/eri/mod.rs:1263 / :992), build_fock (scf/mod.rs:825),
+  build_jk (dft/ks_scf.rs:1369), rhf_scf (scf/mod.rs:1363), ks_scf (dft/ks_scf.rs:1567),
+  rhf_gradient (scf/gradient.rs:900).
+There is no CPU fallback: if the HIP library or a GPU is missing, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from .basis import BasisSet
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libqcg.so"
+_LIB = None
+
+QCG_PREC_FP64 = 0
+QCG_PREC_MIXED = 1
+STATUS = {0: "OK", 1: "INVALID_ARG", 2: "UNSUPPORTED_L", 3: "NUMERICAL", 4: "DEVICE", 5: "NOT_CONVERGED",
+          6: "NO_BASIS", 7: "NO_GRID"}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+# Every exported symbol of include/qcg.h (tests check that the library exports all of them).
+EXPORTS = [
+    "qcg_create", "qcg_destroy", "qcg_last_error", "qcg_set_basis", "qcg_set_options", "qcg_set_charge",
+    "qcg_nbf", "qcg_boys_eval_all", "qcg_rys_roots", "qcg_schwarz_bounds", "qcg_screen_counts",
+    "qcg_eri_compressed", "qcg_build_jk", "qcg_build_fock", "qcg_build_jk_device", "qcg_one_electron",
+    "qcg_rhf_scf",
+]
+
+
+class QcgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"qcg error {code} ({STATUS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class ScfConfig(ctypes.Structure):
+    """qcg_scf_config = ScfConfig + ConvergenceProfile (scf/mod.rs:123-262)."""
+    _fields_ = [("energy_tol", ctypes.c_double), ("density_tol", ctypes.c_double),
+                ("max_iterations", ctypes.c_int), ("use_diis", ctypes.c_int), ("diis_size", ctypes.c_int),
+                ("diis_start", ctypes.c_int), ("damp", ctypes.c_double), ("damp_start", ctypes.c_int),
+                ("level_shift", ctypes.c_double)]
+
+    @classmethod
+    def tight(cls, use_diis=True, **kw):
+        c = cls(1e-10, 1e-8, 100, int(use_diis), 8, 1, 0.0, 5, 0.0)
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+    @classmethod
+    def medium(cls, use_diis=True, **kw):
+        c = cls(1e-8, 1e-6, 100, int(use_diis), 8, 1, 0.0, 5, 0.0)
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+
+class ScfResult(ctypes.Structure):
+    _fields_ = [("converged", ctypes.c_int), ("iterations", ctypes.c_int), ("energy_total", ctypes.c_double),
+                ("energy_electronic", ctypes.c_double), ("energy_nuclear", ctypes.c_double),
+                ("last_delta_e", ctypes.c_double), ("last_rms_d", ctypes.c_double),
+                ("energy_xc", ctypes.c_double), ("energy_j", ctypes.c_double), ("energy_k", ctypes.c_double),
+                ("energy_1e", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def lib():
+    """Load libqcg.so; raise if it was not built (no fallback path exists)."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} not built: run __graft_entry__.build() (hipcc, gfx950)")
+        L = ctypes.CDLL(str(LIB_PATH))
+        L.qcg_last_error.restype = ctypes.c_char_p
+        L.qcg_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]
+        L.qcg_destroy.argtypes = [ctypes.c_void_p]
+        L.qcg_last_error.argtypes = [ctypes.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+@dataclass
+class JK:
+    J: np.ndarray
+    K: np.ndarray
+
+
+class Engine:
+    """One qcg context bound to a GPU and a basis set."""
+
+    def __init__(self, basis: BasisSet, device: int = 0, precision: str = "fp64",
+                 fp32_cutoff: float = 1e-6, schwarz_threshold: float = 1e-12, charge: int = 0):
+        self.L = lib()
+        self.ctx = ctypes.c_void_p()
+        rc = self.L.qcg_create(ctypes.byref(self.ctx), device)
+        if rc:
+            raise QcgError(rc, "qcg_create failed (no GPU?)")
+        self.basis = basis
+        self.nbf = basis.nbf
+        self.set_options(precision, fp32_cutoff, schwarz_threshold)
+        b = basis
+        self._check(self.L.qcg_set_basis(
+            self.ctx, b.nshell, _i(b.shell_l), _i(b.shell_nprim), _i(b.shell_prim_off), _d(b.exps), _d(b.coefs),
+            _d(np.ascontiguousarray(b.shell_center)), _i(b.shell_atom), b.natom, _i(b.Z),
+            _d(np.ascontiguousarray(b.xyz))))
+        self._check(self.L.qcg_set_charge(self.ctx, int(sum(b.Z)) - b.nelec))
+
+    def _check(self, rc):
+        if rc:
+            raise QcgError(rc, self.L.qcg_last_error(self.ctx).decode())
+
+    def close(self):
+        if self.ctx:
+            self.L.qcg_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_options(self, precision="fp64", fp32_cutoff=1e-6, schwarz_threshold=1e-12):
+        p = {"fp64": QCG_PREC_FP64, "mixed": QCG_PREC_MIXED}[precision]
+        self.precision = precision
+        self._check(self.L.qcg_set_options(self.ctx, p, ctypes.c_double(fp32_cutoff),
+                                           ctypes.c_double(schwarz_threshold)))
+
+    # ---- special functions
+    def boys_eval_all(self, m_max: int, T) -> np.ndarray:
+        T = np.ascontiguousarray(T, dtype=np.float64).ravel()
+        F = np.zeros((len(T), m_max + 1))
+        self._check(self.L.qcg_boys_eval_all(self.ctx, m_max, len(T), _d(T), _d(F)))
+        return F
+
+    def rys_roots(self, nroots: int, T):
+        T = np.ascontiguousarray(T, dtype=np.float64).ravel()
+        x = np.zeros((len(T), nroots))
+        w = np.zeros((len(T), nroots))
+        self._check(self.L.qcg_rys_roots(self.ctx, nroots, len(T), _d(T), _d(x), _d(w)))
+        return x, w
+
+    # ---- two-electron
+    def schwarz_bounds(self) -> np.ndarray:
+        Q = np.zeros((self.basis.nshell, self.basis.nshell))
+        self._check(self.L.qcg_schwarz_bounds(self.ctx, _d(Q)))
+        return Q
+
+    def screen_counts(self):
+        kept = ctypes.c_int64()
+        f32 = ctypes.c_int64()
+        self._check(self.L.qcg_screen_counts(self.ctx, ctypes.byref(kept), ctypes.byref(f32)))
+        return kept.value, f32.value
+
+    def eri_compressed(self) -> np.ndarray:
+        npair = self.nbf * (self.nbf + 1) // 2
+        out = np.zeros(npair * (npair + 1) // 2)
+        self._check(self.L.qcg_eri_compressed(self.ctx, _d(out)))
+        return out
+
+    def build_jk(self, D, want_j=True, want_k=True) -> JK:
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        J = np.zeros_like(D) if want_j else None
+        K = np.zeros_like(D) if want_k else None
+        self._check(self.L.qcg_build_jk(self.ctx, _d(D), _d(J) if want_j else None, _d(K) if want_k else None))
+        return JK(J, K)
+
+    def build_fock(self, H, D) -> np.ndarray:
+        H = np.ascontiguousarray(H, dtype=np.float64)
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        F = np.zeros_like(D)
+        self._check(self.L.qcg_build_fock(self.ctx, _d(H), _d(D), _d(F)))
+        return F
+
+    def build_jk_device(self, dD_ptr: int, dJ_ptr: int, dK_ptr: int, stream_ptr: int = 0):
+        """Device-pointer J/K build (e.g. torch tensor .data_ptr()) on a HIP stream."""
+        self._check(self.L.qcg_build_jk_device(self.ctx, ctypes.c_void_p(dD_ptr), ctypes.c_void_p(dJ_ptr),
+                                               ctypes.c_void_p(dK_ptr), ctypes.c_void_p(stream_ptr)))
+
+    # ---- one-electron + SCF
+    def one_electron(self):
+        n = self.nbf
+        S, T, V = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+        self._check(self.L.qcg_one_electron(self.ctx, _d(S), _d(T), _d(V)))
+        return S, T, V
+
+    def rhf_scf(self, config: ScfConfig | None = None, D_init=None):
+        cfg = config or ScfConfig.tight()
+        n = self.nbf
+        res = ScfResult()
+        D, C, eps, F = np.zeros((n, n)), np.zeros((n, n)), np.zeros(n), np.zeros((n, n))
+        Di = None if D_init is None else np.ascontiguousarray(D_init, dtype=np.float64)
+        self._check(self.L.qcg_rhf_scf(self.ctx, ctypes.byref(cfg), _d(Di) if Di is not None else None,
+                                       ctypes.byref(res), _d(D), _d(C), _d(eps), _d(F)))
+        out = res.as_dict()
+        out.update(D=D, C=C, eps=eps, F=F)
+        return out
