@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Fock-build throughput on one MI355X: RHF/6-31G* n-decane C10H22 (194 Cartesian bf).
+
+One step = one integral-direct Fock build F = H + J - K/2 from the converged SCF density
+(Schwarz threshold 1e-12; mixed precision: quartets with Q_ab Q_cd < 1e-6 in FP32, the rest FP64,
+FP64 accumulation), with H, D and F resident in HBM.  Reported `value` is mixed-precision Fock
+builds/s; the same GPU's all-FP64 rate, the speedup, and |dE| between the two energies at that
+density are reported beside it.  The CPU baseline is the oracle's restatement of qc-core's
+build_fock (scf/mod.rs:825) over the compressed ERI tensor of the same molecule.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK = {"f64": 78.6, "f32": 157.3}  # MI355X dense vector peaks, TFLOP/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--fp32-cutoff", type=float, default=1e-6)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build runs the Fock build on a single MI355X "
+              "(multi-GPU sharding of the quartet queue is described in DESIGN.md, not run).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from iqcp_amd import workloads
+    from iqcp_amd.engine import Engine, ScfConfig
+
+    b = workloads.decane_631gs()
+    log(f"workload: C10H22 6-31G* nbf={b.nbf} nshell={b.nshell}")
+    eng = Engine(b, device=0)
+    t0 = time.perf_counter()
+    scf = eng.rhf_scf(ScfConfig.tight())
+    t_scf = time.perf_counter() - t0
+    log(f"setup SCF (FP64, GPU Fock builds): E={scf['energy_total']:.10f} iters={scf['iterations']} {t_scf:.1f}s")
+    S, T, V = eng.one_electron()
+    H = T + V
+    D = scf["D"]
+    dev = torch.device("cuda:0")
+    dH = torch.tensor(H, device=dev)
+    dD = torch.tensor(D, device=dev)
+    dF = torch.empty_like(dD)
+    stream = torch.cuda.Stream(device=dev)
+
+    def timed(prec):
+        eng.set_options(prec, args.fp32_cutoff)
+        with torch.cuda.stream(stream):
+            for _ in range(args.warmup):
+                eng.build_fock_device(dH.data_ptr(), dD.data_ptr(), dF.data_ptr(), 0.5, stream.cuda_stream)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(args.steps):
+                eng.build_fock_device(dH.data_ptr(), dD.data_ptr(), dF.data_ptr(), 0.5, stream.cuda_stream)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t
+        F = dF.cpu().numpy()
+        E = 0.5 * float(np.sum(D * (H + F))) + b.e_nuc
+        return el, F, E
+
+    el64, F64, E64 = timed("fp64")
+    elmx, Fmx, Emx = timed("mixed")
+    log(f"fp64: {1e3 * el64 / args.steps:.3f} ms/build  mixed: {1e3 * elmx / args.steps:.3f} ms/build")
+
+    # Profiled pass: HIP events around every class kernel on the stream it runs on.
+    per_kernel = {}
+    for prec in ("fp64", "mixed"):
+        eng.set_options(prec, args.fp32_cutoff)
+        eng.set_profiling(True)
+        acc = None
+        with torch.cuda.stream(stream):
+            for _ in range(args.steps):
+                eng.build_fock_device(dH.data_ptr(), dD.data_ptr(), dF.data_ptr(), 0.5, stream.cuda_stream)
+                st = eng.class_stats()
+                if acc is None:
+                    acc = [dict(s) for s in st]
+                else:
+                    for a_, s_ in zip(acc, st):
+                        a_["ms_fp64"] += s_["ms_fp64"]
+                        a_["ms_fp32"] += s_["ms_fp32"]
+        eng.set_profiling(False)
+        for a_ in acc:
+            a_["ms_fp64"] /= args.steps
+            a_["ms_fp32"] /= args.steps
+        per_kernel[prec] = acc
+    # dominant kernel of the mixed build
+    best = None
+    for s in per_kernel["mixed"]:
+        for part, dt in (("fp64", "f64"), ("fp32", "f32")):
+            ms = s[f"ms_{part}"]
+            if ms > 0 and (best is None or ms > best[0]):
+                best = (ms, s, part, dt)
+    ms, s, part, dt = best
+    cls = f"({'spd'[s['la']]}{'spd'[s['lb']]}|{'spd'[s['lc']]}{'spd'[s['ld']]})"
+    achieved = s[f"flops_{part}"] / (ms * 1e-3) / 1e12
+    roofline = {"bound": "valu", "kernel": f"jk_kernel{cls} {dt}", "achieved": round(achieved, 3),
+                "peak": PEAK[dt], "unit": "TFLOP/s", "frac": round(achieved / PEAK[dt], 4), "traffic": None,
+                "kernel_ms": round(ms, 4), "kernel_flops_model": s[f"flops_{part}"]}
+    tot_fl = sum(x["flops_fp64"] + x["flops_fp32"] for x in per_kernel["mixed"])
+    tot_ms = sum(x["ms_fp64"] + x["ms_fp32"] for x in per_kernel["mixed"])
+    kept, n32 = eng.screen_counts()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        eng.set_options("fp64")
+        eri = eng.eri_compressed()  # the tensor qc-core's build_fock reads (eri_compressed, eri/mod.rs:992)
+        nthr = min(16, os.cpu_count() or 1)
+        n_done, t = 0, time.perf_counter()
+        Fc = None
+        while n_done < 5 and (n_done == 0 or time.perf_counter() - t < args.cpu_seconds):
+            Fc = oracle.build_fock_omp(H, D, eri, b.nbf, nthr)
+            n_done += 1
+        el = time.perf_counter() - t
+        del eri
+        cpu = {"value": round(n_done / el, 4), "unit": "Fock builds/s", "cores": nthr, "kind": "port",
+               "sample": f"{n_done} build_fock call(s) (scf/mod.rs:825 restated in oracle/) over the "
+                         f"{b.nbf * (b.nbf + 1) // 2 * (b.nbf * (b.nbf + 1) // 2 + 1) // 2}-element compressed "
+                         f"ERI tensor of the same decane geometry at the same density, {el:.1f}s",
+               "max_abs_F_vs_gpu_fp64": float(np.abs(Fc - F64).max())}
+        log(f"cpu baseline: {cpu['value']} builds/s on {nthr} threads")
+
+    out = {
+        "metric": "Fock builds/s, mixed vs FP64 on same GPU at |dE|<=1uHa; gradient evals/s",
+        "value": round(args.steps / elmx, 3),
+        "unit": "Fock builds/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elmx / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64 (FP32 quartets below Schwarz product 1e-6, FP64 rest and accumulation)",
+        "data": "synthetic geometry (ideal all-trans + seeded 0.02 A jitter), converged RHF density",
+        "config": {"workload": "RHF/6-31G* n-decane C10H22, integral-direct J+K Fock build at the converged density",
+                   "nbf": b.nbf, "nshell": b.nshell, "schwarz_threshold": 1e-12, "fp32_cutoff": args.fp32_cutoff,
+                   "unique_quartets_kept": kept, "quartets_fp32": n32},
+        "fp64_value": round(args.steps / el64, 3),
+        "fp64_ms_per_step": round(1e3 * el64 / args.steps, 4),
+        "speedup_mixed_vs_fp64": round(el64 / elmx, 3),
+        "dE_mixed_vs_fp64_Ha": float(Emx - E64),
+        "scf_energy_fp64": scf["energy_total"],
+        "scf_iterations": scf["iterations"],
+        "all_kernels_model_tflops": round(tot_fl / (tot_ms * 1e-3) / 1e12, 3) if tot_ms > 0 else None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

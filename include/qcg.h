@@ -108,6 +108,47 @@ int qcg_build_fock(qcg_context* ctx, const double* H, const double* D, double* F
 /* Device-pointer variant of qcg_build_jk on the given HIP stream (NULL = context stream);
  * asynchronous with respect to the host. */
 int qcg_build_jk_device(qcg_context* ctx, const double* dD, double* dJ, double* dK, void* stream);
+/* Device-pointer Fock build: dF = dH + J - k_factor K (RHF build_fock: k_factor = 1/2; hybrid
+ * KS, dft/ks_scf.rs:1960: k_factor = a/2). */
+int qcg_build_fock_device(qcg_context* ctx, const double* dH, const double* dD, double k_factor, double* dF,
+                          void* stream);
+
+/* Per quartet-class statistics of the integral-direct J/K build: integer counts (exact, same
+ * screens as the kernels), the algorithmic flop model, and (after qcg_set_profiling(ctx, 1) and
+ * one build) the measured kernel time of the last build.  Flop model per surviving primitive
+ * quartet of class (la lb|lc ld) with n = (la+lb+lc+ld)/2+1 Rys nodes and N_q Cartesian
+ * components:  n [80 + 15 (la+lb+1)(lc+ld+1) + N_q (4 + 6 (lb+ld))], plus 12 N_q per contracted
+ * quartet for the J/K digestion.  Classes are ordered bra class 5..0, ket class <= bra. */
+typedef struct qcg_class_stat {
+    int la, lb, lc, ld;
+    int64_t quartets_fp64, quartets_fp32;
+    int64_t prim_quartets_fp64, prim_quartets_fp32;
+    double flops_fp64, flops_fp32;
+    double ms_fp64, ms_fp32;
+} qcg_class_stat;
+int qcg_set_profiling(qcg_context* ctx, int on);
+/* out must hold 21 entries; *n receives the number written. */
+int qcg_class_stats(qcg_context* ctx, qcg_class_stat* out, int* n);
+
+/* ---- exchange-correlation (B3LYP) on a Becke/Lebedev grid ------------------------------ */
+/* build_becke_grid (dft/grid.rs:103) for the context's atoms: n_radial radial points (75),
+ * quality 0 = Standard (302) / 1 = Fine (590), pruning 1 = SG-1.  Also evaluates Phi and grad Phi
+ * on the device (evaluate_basis_and_gradients_on_grid, dft/ks_scf.rs:115).  *npts = grid size. */
+int qcg_build_grid(qcg_context* ctx, int n_radial, int quality, int pruning, int64_t* npts);
+/* A caller-supplied grid (xyz[3*npts] bohr, weights). */
+int qcg_set_grid(qcg_context* ctx, int64_t npts, const double* xyz, const double* w);
+/* Copy the current grid out (any pointer may be NULL). */
+int qcg_get_grid(qcg_context* ctx, double* xyz, double* w, int* atom_idx);
+/* B3LYP pointwise (B3lyp::eval_xc_gga, dft/functionals.rs:1625): exc per electron, vrho, vsigma. */
+int qcg_xc_eval(qcg_context* ctx, int64_t n, const double* rho, const double* sigma, double* exc, double* vrho,
+                double* vsigma);
+/* V_xc (symmetrised) and E_xc for density D (build_vxc_gga_matmul, dft/ks_scf.rs:1150). */
+int qcg_xc(qcg_context* ctx, const double* D, double* Vxc, double* Exc);
+/* Device-pointer variant; *dExc is written on the device. */
+int qcg_xc_device(qcg_context* ctx, const double* dD, double* dVxc, double* dExc, void* stream);
+/* ks_scf (dft/ks_scf.rs:1567) for B3LYP on the current grid: F = H + J - (a/2) K + V_xc. */
+int qcg_ks_scf(qcg_context* ctx, const qcg_scf_config* cfg, const double* D_init, qcg_scf_result* res, double* D,
+               double* C, double* eps, double* F);
 
 /* ---- one-electron terms and SCF ---------------------------------------------------------- */
 /* overlap_matrix (integrals/overlap.rs:647), kinetic and nuclear attraction (hcore_matrix,

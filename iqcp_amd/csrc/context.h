@@ -1,0 +1,111 @@
+// The qcg_context object shared by the host translation units.
+#pragma once
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include <rocblas/rocblas.h>
+
+#include "../../include/qcg.h"
+#include "eri_launch.h"
+#include "onee.h"
+#include "xc_launch.h"
+
+using namespace qcg;
+
+struct qcg_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool has_basis = false;
+    HostBasis basis;
+    int precision = QCG_PREC_FP64;
+    double fp32_cutoff = 1e-6;
+    double schwarz_thr = kSchwarzDefault;
+    int charge = 0;
+    // device tables
+    double* d_rys = nullptr;
+    float* d_rys_f = nullptr;
+    // pairs
+    std::vector<PairRec> pairs;  // grouped by class, sorted by Q desc within a class
+    int class_off[kNumPairClasses + 1] = {0};
+    PairRec* d_pairs = nullptr;
+    PrimPair<double>* d_pp = nullptr;
+    PrimPair<float>* d_ppf = nullptr;
+    std::vector<double> Qshell;  // nshell x nshell
+    // work buffers
+    double *d_D = nullptr, *d_J = nullptr, *d_K = nullptr, *d_Jacc = nullptr, *d_Kacc = nullptr;
+    double *d_H = nullptr, *d_F = nullptr;
+    std::vector<void*> owned;
+    std::vector<PrimPair<double>> h_pp;  // host copy of the primitive pairs (flop model counts)
+    // per-class profiling of the J/K build: events around each class launch
+    bool profiling = false;
+    hipEvent_t ev[2 * 21 * 2 + 2] = {};
+    bool ev_made = false;
+    bool ev_recorded[21][2] = {};
+    // cached class counts / flop model, valid for (precision, fp32_cutoff, schwarz_thr) of stats_key
+    std::vector<qcg_class_stat> stats;
+    // exchange-correlation grid and per-point buffers
+    rocblas_handle blas = nullptr;
+    int64_t G = 0;
+    std::vector<double> grid_xyz, grid_w;
+    std::vector<int> grid_atom;
+    double *d_pts = nullptr, *d_w = nullptr, *d_phi = nullptr, *d_dphx = nullptr, *d_dphy = nullptr,
+           *d_dphz = nullptr, *d_X = nullptr, *d_aow = nullptr, *d_V = nullptr, *d_exc = nullptr;
+    GridShell* d_gshell = nullptr;
+    double *d_gexp = nullptr, *d_gcn = nullptr;
+    void free_grid() {
+        for (void** p : {(void**)&d_pts, (void**)&d_w, (void**)&d_phi, (void**)&d_dphx, (void**)&d_dphy,
+                         (void**)&d_dphz, (void**)&d_X, (void**)&d_aow, (void**)&d_V, (void**)&d_exc,
+                         (void**)&d_gshell, (void**)&d_gexp, (void**)&d_gcn}) {
+            release(*p);
+            *p = nullptr;
+        }
+        G = 0;
+    }
+    double stats_key[3] = {-1.0, -1.0, -1.0};
+
+    template <typename T>
+    hipError_t alloc(T** p, size_t n) {
+        hipError_t e = hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T));
+        if (e == hipSuccess) owned.push_back((void*)*p);
+        return e;
+    }
+    void release(void* p) {
+        if (!p) return;
+        auto it = std::find(owned.begin(), owned.end(), p);
+        if (it != owned.end()) owned.erase(it);
+        (void)hipFree(p);
+    }
+    void free_basis() {
+        for (void** p : {(void**)&d_pairs, (void**)&d_pp, (void**)&d_ppf, (void**)&d_D, (void**)&d_J, (void**)&d_K,
+                         (void**)&d_Jacc, (void**)&d_Kacc, (void**)&d_H, (void**)&d_F}) {
+            release(*p);
+            *p = nullptr;
+        }
+        has_basis = false;
+        free_grid();
+    }
+};
+
+
+namespace qcg {
+
+inline int fail(qcg_context* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+// Integral-direct J/K into Jacc/Kacc then J = Jacc + Jacc^T, K = Kacc + Kacc^T (dJ/dK may be null).
+int run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStream_t st);
+double radial_norm(double a, int l);
+
+#define HIPCHK(ctx, expr)                                                                        \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail(ctx, QCG_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+
+}  // namespace qcg

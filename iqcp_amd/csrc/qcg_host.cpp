@@ -6,83 +6,16 @@
 #include <string>
 #include <vector>
 
-#include "../../include/qcg.h"
-#include "eri_launch.h"
-#include "onee.h"
+#include "context.h"
 #include "rys.h"
 #include "scf.h"
 
-using namespace qcg;
-
-struct qcg_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    bool has_basis = false;
-    HostBasis basis;
-    int precision = QCG_PREC_FP64;
-    double fp32_cutoff = 1e-6;
-    double schwarz_thr = kSchwarzDefault;
-    int charge = 0;
-    // device tables
-    double* d_rys = nullptr;
-    float* d_rys_f = nullptr;
-    // pairs
-    std::vector<PairRec> pairs;  // grouped by class, sorted by Q desc within a class
-    int class_off[kNumPairClasses + 1] = {0};
-    PairRec* d_pairs = nullptr;
-    PrimPair<double>* d_pp = nullptr;
-    PrimPair<float>* d_ppf = nullptr;
-    std::vector<double> Qshell;  // nshell x nshell
-    // work buffers
-    double *d_D = nullptr, *d_J = nullptr, *d_K = nullptr, *d_Jacc = nullptr, *d_Kacc = nullptr;
-    double *d_H = nullptr, *d_F = nullptr;
-    std::vector<void*> owned;
-
-    template <typename T>
-    hipError_t alloc(T** p, size_t n) {
-        hipError_t e = hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T));
-        if (e == hipSuccess) owned.push_back((void*)*p);
-        return e;
-    }
-    void release(void* p) {
-        if (!p) return;
-        auto it = std::find(owned.begin(), owned.end(), p);
-        if (it != owned.end()) owned.erase(it);
-        (void)hipFree(p);
-    }
-    void free_basis() {
-        for (void** p : {(void**)&d_pairs, (void**)&d_pp, (void**)&d_ppf, (void**)&d_D, (void**)&d_J, (void**)&d_K,
-                         (void**)&d_Jacc, (void**)&d_Kacc, (void**)&d_H, (void**)&d_F}) {
-            release(*p);
-            *p = nullptr;
-        }
-        has_basis = false;
-    }
-};
-
-namespace {
-
-int fail(qcg_context* c, int code, const std::string& msg) {
-    if (c) c->err = msg;
-    return code;
-}
-
-#define HIPCHK(ctx, expr)                                                                        \
-    do {                                                                                         \
-        hipError_t e_ = (expr);                                                                  \
-        if (e_ != hipSuccess)                                                                    \
-            return fail(ctx, QCG_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));     \
-    } while (0)
-
-double radial_norm(double a, int l) {
+double qcg::radial_norm(double a, int l) {
     const double t = 2.0 * a / kPi;
     const double base = std::sqrt(std::sqrt(t * t * t));
     const double fa = 4.0 * a;
     return base * (l == 0 ? 1.0 : l == 1 ? std::sqrt(fa) : l == 2 ? fa : std::pow(fa, l / 2.0));
 }
-
-}  // namespace
 
 double qcg::nuclear_repulsion(const HostBasis& B) {
     // compute_nuclear_repulsion (basis/mod.rs:375-391)
@@ -125,6 +58,9 @@ void qcg_destroy(qcg_context* c) {
     (void)hipStreamSynchronize(c->stream);
     for (void* p : c->owned) (void)hipFree(p);
     c->owned.clear();
+    if (c->ev_made)
+        for (auto& e : c->ev) (void)hipEventDestroy(e);
+    if (c->blas) (void)rocblas_destroy_handle(c->blas);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -162,6 +98,7 @@ int qcg_set_basis(qcg_context* c, int nshell, const int* l, const int* nprim, co
         return fail(c, QCG_INVALID_ARG, "qcg_set_basis: empty or null input");
     HIPCHK(c, hipSetDevice(c->device));
     c->free_basis();
+    c->stats.clear();
     HostBasis& B = c->basis;
     B = HostBasis();
     B.nshell = nshell;
@@ -236,6 +173,7 @@ int qcg_set_basis(qcg_context* c, int nshell, const int* l, const int* nprim, co
         ppf[i].p = (float)pp[i].p; ppf[i].px = (float)pp[i].px; ppf[i].py = (float)pp[i].py;
         ppf[i].pz = (float)pp[i].pz; ppf[i].kab = (float)pp[i].kab; ppf[i].ck = (float)pp[i].ck;
     }
+    c->h_pp = pp;
     HIPCHK(c, c->alloc(&c->d_pp, pp.size()));
     HIPCHK(c, c->alloc(&c->d_ppf, pp.size()));
     HIPCHK(c, c->alloc(&c->d_pairs, all.size()));
@@ -313,15 +251,23 @@ int qcg_screen_counts(qcg_context* c, int64_t* n_kept, int64_t* n_fp32) {
     return QCG_OK;
 }
 
-static int run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStream_t st) {
+}  // extern "C"
+
+int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStream_t st) {
     const int nbf = c->basis.nbf;
     const size_t n2 = (size_t)nbf * nbf;
     const int do_j = dJ != nullptr, do_k = dK != nullptr;
     if (do_j) HIPCHK(c, launch_fill(c->d_Jacc, n2, 0.0, st));
     if (do_k) HIPCHK(c, launch_fill(c->d_Kacc, n2, 0.0, st));
     const bool mixed = c->precision == QCG_PREC_MIXED && c->fp32_cutoff > c->schwarz_thr;
+    if (c->profiling && !c->ev_made) {
+        for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
+        c->ev_made = true;
+    }
+    std::memset(c->ev_recorded, 0, sizeof(c->ev_recorded));
+    int cls = 0;
     for (int cb = kNumPairClasses - 1; cb >= 0; --cb)
-        for (int ck = cb; ck >= 0; --ck) {
+        for (int ck = cb; ck >= 0; --ck, ++cls) {
             JkLaunch a{};
             a.bra = c->d_pairs + c->class_off[cb];
             a.nbra = c->class_off[cb + 1] - c->class_off[cb];
@@ -338,16 +284,100 @@ static int run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipS
             a.rys = c->d_rys;
             a.thr_lo = mixed ? c->fp32_cutoff : c->schwarz_thr;
             a.thr_hi = 1e308 * 10.0;
+            if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4 * cls + 0], st));
             HIPCHK(c, launch_jk(cb, ck, false, a, st));
+            if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4 * cls + 1], st));
+            c->ev_recorded[cls][0] = c->profiling;
             if (mixed) {
                 a.pp = c->d_ppf;
                 a.rys = c->d_rys_f;
                 a.thr_lo = c->schwarz_thr;
                 a.thr_hi = c->fp32_cutoff;
+                if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4 * cls + 2], st));
                 HIPCHK(c, launch_jk(cb, ck, true, a, st));
+                if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4 * cls + 3], st));
+                c->ev_recorded[cls][1] = c->profiling;
             }
         }
     HIPCHK(c, launch_symmetrize(c->d_Jacc, c->d_Kacc, dJ, dK, nbf, st));
+    return QCG_OK;
+}
+
+extern "C" {
+
+int qcg_set_profiling(qcg_context* c, int on) {
+    if (!c) return QCG_INVALID_ARG;
+    c->profiling = on != 0;
+    return QCG_OK;
+}
+
+int qcg_class_stats(qcg_context* c, qcg_class_stat* out, int* n) {
+    if (!c || !out) return QCG_INVALID_ARG;
+    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    const bool mixed = c->precision == QCG_PREC_MIXED && c->fp32_cutoff > c->schwarz_thr;
+    const double thr = c->schwarz_thr, cut = mixed ? c->fp32_cutoff : thr;
+    const bool cached = c->stats.size() == 21 && c->stats_key[0] == (double)c->precision &&
+                        c->stats_key[1] == c->fp32_cutoff && c->stats_key[2] == c->schwarz_thr;
+    if (cached) {
+        for (int i = 0; i < 21; ++i) out[i] = c->stats[i];
+    }
+    auto first_below = [&](int off, int qmax, double Qp, double t) {
+        int lo = 0, hi = qmax;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (Qp * c->pairs[off + mid].Q >= t) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    int cls = 0;
+    for (int cb = kNumPairClasses - 1; cb >= 0; --cb)
+        for (int ck = cb; ck >= 0; --ck, ++cls) {
+            qcg_class_stat& s = out[cls];
+            if (cached) {
+                s.ms_fp64 = s.ms_fp32 = 0.0;
+            } else {
+            std::memset(&s, 0, sizeof(s));
+            const int la = cb == 0 ? 0 : cb <= 2 ? 1 : 2, lb = (cb == 0 || cb == 1 || cb == 3) ? 0 : (cb == 2 || cb == 4) ? 1 : 2;
+            const int lc = ck == 0 ? 0 : ck <= 2 ? 1 : 2, ld = (ck == 0 || ck == 1 || ck == 3) ? 0 : (ck == 2 || ck == 4) ? 1 : 2;
+            s.la = la; s.lb = lb; s.lc = lc; s.ld = ld;
+            const int nb = c->class_off[cb + 1] - c->class_off[cb], nk = c->class_off[ck + 1] - c->class_off[ck];
+            int64_t q64 = 0, q32 = 0, p64 = 0, p32 = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : q64, q32, p64, p32)
+            for (int p = 0; p < nb; ++p) {
+                const PairRec& P = c->pairs[c->class_off[cb] + p];
+                const int qmax = (cb == ck) ? p + 1 : nk;
+                const int e_all = first_below(c->class_off[ck], qmax, P.Q, thr);
+                const int e_64 = mixed ? std::min(e_all, first_below(c->class_off[ck], qmax, P.Q, cut)) : e_all;
+                for (int q = 0; q < e_all; ++q) {
+                    const PairRec& Q = c->pairs[c->class_off[ck] + q];
+                    int64_t np = 0;
+                    for (int i = 0; i < P.npp; ++i) {
+                        const double ka = c->h_pp[P.pp_off + i].kab;
+                        for (int j = 0; j < Q.npp; ++j) np += (ka * c->h_pp[Q.pp_off + j].kab >= kPrimScreen) ? 1 : 0;
+                    }
+                    if (q < e_64) { ++q64; p64 += np; } else { ++q32; p32 += np; }
+                }
+            }
+            const int nr = (la + lb + lc + ld) / 2 + 1;
+            const double nq = (double)ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+            const double fpq = nr * (80.0 + 15.0 * (la + lb + 1) * (lc + ld + 1) + nq * (4.0 + 6.0 * (lb + ld)));
+            s.quartets_fp64 = q64; s.quartets_fp32 = q32;
+            s.prim_quartets_fp64 = p64; s.prim_quartets_fp32 = p32;
+            s.flops_fp64 = fpq * (double)p64 + 12.0 * nq * (double)q64;
+            s.flops_fp32 = fpq * (double)p32 + 12.0 * nq * (double)q32;
+            }
+            if (c->ev_made) {
+                float ms = 0.f;
+                if (c->ev_recorded[cls][0] && hipEventSynchronize(c->ev[4 * cls + 1]) == hipSuccess &&
+                    hipEventElapsedTime(&ms, c->ev[4 * cls + 0], c->ev[4 * cls + 1]) == hipSuccess)
+                    s.ms_fp64 = ms;
+                if (c->ev_recorded[cls][1] && hipEventSynchronize(c->ev[4 * cls + 3]) == hipSuccess &&
+                    hipEventElapsedTime(&ms, c->ev[4 * cls + 2], c->ev[4 * cls + 3]) == hipSuccess)
+                    s.ms_fp32 = ms;
+            }
+        }
+    if (n) *n = cls;
     return QCG_OK;
 }
 
@@ -356,6 +386,18 @@ int qcg_build_jk_device(qcg_context* c, const double* dD, double* dJ, double* dK
     if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
     HIPCHK(c, hipSetDevice(c->device));
     return run_jk(c, dD, dJ, dK, stream ? (hipStream_t)stream : c->stream);
+}
+
+int qcg_build_fock_device(qcg_context* c, const double* dH, const double* dD, double k_factor, double* dF,
+                          void* stream) {
+    if (!c || !dH || !dD || !dF) return QCG_INVALID_ARG;
+    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    int rc = run_jk(c, dD, c->d_J, c->d_K, st);
+    if (rc) return rc;
+    HIPCHK(c, launch_fock_combine(dH, c->d_Jacc, c->d_Kacc, dF, c->basis.nbf, k_factor, st));
+    return QCG_OK;
 }
 
 int qcg_build_jk(qcg_context* c, const double* D, double* J, double* K) {

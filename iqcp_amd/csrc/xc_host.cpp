@@ -1,0 +1,251 @@
+// Host side of the exchange-correlation path: grid set-up, V_xc / E_xc builds, KS-SCF.
+//
+// Per build (build_vxc_gga_matmul, dft/ks_scf.rs:1150-1296, with compute_chi_d :936 and
+// compute_density_and_gradient_matmul :1028):
+//   X = Phi D                       (GEMM, G x nbf x nbf)
+//   point_kernel: rho, grad rho, sigma, B3LYP, aow row, E_xc
+//   V = Phi^T aow, V_xc = (V + V^T)/2   (GEMM, nbf x nbf x G)
+// Phi and grad Phi are evaluated once per grid and kept resident in HBM.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "context.h"
+#include "grid.h"
+#include "scf.h"
+
+using namespace qcg;
+
+namespace {
+
+int ensure_blas(qcg_context* c) {
+    if (!c->blas && rocblas_create_handle(&c->blas) != rocblas_status_success)
+        return fail(c, QCG_DEVICE, "rocblas_create_handle failed");
+    return QCG_OK;
+}
+
+int upload_grid(qcg_context* c) {
+    const HostBasis& B = c->basis;
+    c->free_grid();
+    const int64_t G = (int64_t)c->grid_w.size();
+    c->G = G;
+    if (G == 0) return QCG_OK;
+    const size_t gn = (size_t)G * B.nbf;
+    HIPCHK(c, c->alloc(&c->d_pts, 3 * (size_t)G));
+    HIPCHK(c, c->alloc(&c->d_w, (size_t)G));
+    HIPCHK(c, c->alloc(&c->d_phi, gn));
+    HIPCHK(c, c->alloc(&c->d_dphx, gn));
+    HIPCHK(c, c->alloc(&c->d_dphy, gn));
+    HIPCHK(c, c->alloc(&c->d_dphz, gn));
+    HIPCHK(c, c->alloc(&c->d_X, gn));
+    HIPCHK(c, c->alloc(&c->d_aow, gn));
+    HIPCHK(c, c->alloc(&c->d_V, (size_t)B.nbf * B.nbf));
+    HIPCHK(c, c->alloc(&c->d_exc, 1));
+    HIPCHK(c, hipMemcpy(c->d_pts, c->grid_xyz.data(), 3 * (size_t)G * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_w, c->grid_w.data(), (size_t)G * sizeof(double), hipMemcpyHostToDevice));
+    // shells for the basis evaluation: coef * (2a/pi)^(3/4) (4a)^(l/2) per primitive
+    std::vector<GridShell> gs(B.nshell);
+    std::vector<double> ex, cn;
+    for (int s = 0; s < B.nshell; ++s) {
+        GridShell& g = gs[s];
+        g.cx = B.centers[3 * s];
+        g.cy = B.centers[3 * s + 1];
+        g.cz = B.centers[3 * s + 2];
+        g.l = B.l[s];
+        g.nprim = B.nprim[s];
+        g.poff = (int)ex.size();
+        g.bf = B.bf_off[s];
+        double amin = 1e300;
+        for (int p = 0; p < B.nprim[s]; ++p) {
+            const double a = B.exps[B.prim_off[s] + p];
+            amin = std::min(amin, a);
+            ex.push_back(a);
+            cn.push_back(B.coefs[B.prim_off[s] + p] * radial_norm(a, B.l[s]));
+        }
+        g.amin = amin;
+    }
+    HIPCHK(c, c->alloc(&c->d_gshell, gs.size()));
+    HIPCHK(c, c->alloc(&c->d_gexp, ex.size()));
+    HIPCHK(c, c->alloc(&c->d_gcn, cn.size()));
+    HIPCHK(c, hipMemcpy(c->d_gshell, gs.data(), gs.size() * sizeof(GridShell), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_gexp, ex.data(), ex.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_gcn, cn.data(), cn.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, launch_basis_on_grid((int)G, B.nshell, c->d_gshell, c->d_gexp, c->d_gcn, c->d_pts, B.nbf, c->d_phi,
+                                   c->d_dphx, c->d_dphy, c->d_dphz, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return QCG_OK;
+}
+
+int xc_build(qcg_context* c, const double* dD, double* dV, double* dExc, hipStream_t st) {
+    if (c->G <= 0) return fail(c, QCG_NO_GRID, "no grid: call qcg_build_grid or qcg_set_grid");
+    int rc = ensure_blas(c);
+    if (rc) return rc;
+    const int nbf = c->basis.nbf;
+    const rocblas_int G = (rocblas_int)c->G;
+    if (rocblas_set_stream(c->blas, st) != rocblas_status_success) return fail(c, QCG_DEVICE, "rocblas_set_stream");
+    const double one = 1.0, zero = 0.0;
+    // X^T (nbf x G, column-major) = D (nbf x nbf) * Phi^T (nbf x G)
+    if (rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, nbf, G, nbf, &one, dD, nbf, c->d_phi,
+                      nbf, &zero, c->d_X, nbf) != rocblas_status_success)
+        return fail(c, QCG_DEVICE, "rocblas_dgemm (Phi D)");
+    HIPCHK(c, launch_fill(dExc, 1, 0.0, st));
+    HIPCHK(c, launch_xc_points((int)G, nbf, c->d_w, c->d_phi, c->d_dphx, c->d_dphy, c->d_dphz, c->d_X, c->d_aow, dExc,
+                               nullptr, st));
+    // V (column-major nbf x nbf) = Phi^T (nbf x G) * aow^T^T
+    if (rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose, nbf, nbf, G, &one, c->d_phi, nbf,
+                      c->d_aow, nbf, &zero, c->d_V, nbf) != rocblas_status_success)
+        return fail(c, QCG_DEVICE, "rocblas_dgemm (Phi^T aow)");
+    HIPCHK(c, launch_sym_half(c->d_V, dV, nbf, st));
+    return QCG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qcg_build_grid(qcg_context* c, int n_radial, int quality, int pruning, int64_t* npts) {
+    if (!c) return QCG_INVALID_ARG;
+    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    if (n_radial <= 0 || (quality != 0 && quality != 1)) return fail(c, QCG_INVALID_ARG, "grid config");
+    HIPCHK(c, hipSetDevice(c->device));
+    GridConfig cfg;
+    cfg.n_radial = n_radial;
+    cfg.quality = quality;
+    cfg.pruning = pruning ? 1 : 0;
+    int rc = build_becke_grid(c->basis, cfg, c->grid_xyz, c->grid_w, c->grid_atom);
+    if (rc) return fail(c, rc, "grid construction failed");
+    rc = upload_grid(c);
+    if (rc) return rc;
+    if (npts) *npts = c->G;
+    return QCG_OK;
+}
+
+int qcg_set_grid(qcg_context* c, int64_t npts, const double* xyz, const double* w) {
+    if (!c || npts < 0 || (npts > 0 && (!xyz || !w))) return QCG_INVALID_ARG;
+    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    HIPCHK(c, hipSetDevice(c->device));
+    c->grid_xyz.assign(xyz, xyz + 3 * npts);
+    c->grid_w.assign(w, w + npts);
+    c->grid_atom.assign(npts, -1);
+    return upload_grid(c);
+}
+
+int qcg_get_grid(qcg_context* c, double* xyz, double* w, int* atom_idx) {
+    if (!c) return QCG_INVALID_ARG;
+    if (xyz) std::memcpy(xyz, c->grid_xyz.data(), c->grid_xyz.size() * sizeof(double));
+    if (w) std::memcpy(w, c->grid_w.data(), c->grid_w.size() * sizeof(double));
+    if (atom_idx) std::memcpy(atom_idx, c->grid_atom.data(), c->grid_atom.size() * sizeof(int));
+    return QCG_OK;
+}
+
+int qcg_xc_eval(qcg_context* c, int64_t n, const double* rho, const double* sigma, double* exc, double* vrho,
+                double* vsigma) {
+    if (!c || n < 0 || (n > 0 && (!rho || !sigma || !exc || !vrho || !vsigma))) return QCG_INVALID_ARG;
+    if (n == 0) return QCG_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    double* d = nullptr;
+    HIPCHK(c, c->alloc(&d, 5 * (size_t)n));
+    HIPCHK(c, hipMemcpyAsync(d, rho, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d + n, sigma, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_xc_eval((int)n, d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, c->stream));
+    HIPCHK(c, hipMemcpyAsync(exc, d + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(vrho, d + 3 * n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(vsigma, d + 4 * n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->release(d);
+    return QCG_OK;
+}
+
+int qcg_xc_device(qcg_context* c, const double* dD, double* dV, double* dExc, void* stream) {
+    if (!c || !dD || !dV || !dExc) return QCG_INVALID_ARG;
+    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    HIPCHK(c, hipSetDevice(c->device));
+    return xc_build(c, dD, dV, dExc, stream ? (hipStream_t)stream : c->stream);
+}
+
+int qcg_xc(qcg_context* c, const double* D, double* Vxc, double* Exc) {
+    if (!c || !D || !Vxc || !Exc) return QCG_INVALID_ARG;
+    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t n2 = (size_t)c->basis.nbf * c->basis.nbf;
+    HIPCHK(c, hipMemcpyAsync(c->d_D, D, n2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int rc = xc_build(c, c->d_D, c->d_F, c->d_exc, c->stream);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(Vxc, c->d_F, n2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(Exc, c->d_exc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return QCG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+struct KsUser {
+    qcg_context* c;
+    const double* H;
+    double a;  // exact-exchange fraction (B3lyp::A = 0.20)
+    double e_xc, e_j, e_k, e_1e;
+};
+
+// F = H + J - (a/2) K + V_xc;  E_elec = Tr DH + 1/2 Tr DJ - 1/4 a Tr DK + E_xc  (dft/ks_scf.rs:1960-1981)
+int ks_fock_cb(void* u, const double* D, double* F, double* e_elec) {
+    KsUser* k = (KsUser*)u;
+    qcg_context* c = k->c;
+    const int nbf = c->basis.nbf;
+    const size_t n2 = (size_t)nbf * nbf;
+    HIPCHK(c, hipMemcpyAsync(c->d_D, D, n2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int rc = run_jk(c, c->d_D, c->d_J, c->d_K, c->stream);
+    if (rc) return rc;
+    rc = xc_build(c, c->d_D, c->d_F, c->d_exc, c->stream);
+    if (rc) return rc;
+    std::vector<double> J(n2), K(n2), V(n2);
+    double exc = 0.0;
+    HIPCHK(c, hipMemcpyAsync(J.data(), c->d_J, n2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(K.data(), c->d_K, n2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(V.data(), c->d_F, n2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&exc, c->d_exc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double tdh = 0.0, tdj = 0.0, tdk = 0.0;
+    for (size_t i = 0; i < n2; ++i) {
+        F[i] = k->H[i] + J[i] - 0.5 * k->a * K[i] + V[i];
+        tdh += D[i] * k->H[i];
+        tdj += D[i] * J[i];
+        tdk += D[i] * K[i];
+    }
+    k->e_1e = tdh;
+    k->e_j = 0.5 * tdj;
+    k->e_k = -0.5 * k->a * 0.5 * tdk;
+    k->e_xc = exc;
+    *e_elec = tdh + 0.5 * tdj - 0.25 * k->a * tdk + exc;
+    return QCG_OK;
+}
+
+}  // namespace
+
+extern "C" int qcg_ks_scf(qcg_context* c, const qcg_scf_config* cfg, const double* D_init, qcg_scf_result* res,
+                          double* D, double* C, double* eps, double* F) {
+    if (!c || !cfg) return QCG_INVALID_ARG;
+    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    if (c->G <= 0) return fail(c, QCG_NO_GRID, "no grid: call qcg_build_grid or qcg_set_grid");
+    const HostBasis& B = c->basis;
+    int nelec = -c->charge;
+    for (int z : B.Z) nelec += z;
+    if (nelec <= 0 || nelec % 2) return fail(c, QCG_INVALID_ARG, "closed-shell KS needs an even electron count");
+    const size_t n2 = (size_t)B.nbf * B.nbf;
+    std::vector<double> S(n2), T(n2), V(n2), H(n2);
+    one_electron_matrices(B, S.data(), T.data(), V.data());
+    for (size_t i = 0; i < n2; ++i) H[i] = T[i] + V[i];
+    KsUser u{c, H.data(), 0.20, 0, 0, 0, 0};
+    qcg_scf_result r{};
+    int rc = scf_driver(B.nbf, nelec / 2, S.data(), H.data(), nuclear_repulsion(B), *cfg, ks_fock_cb, &u, D_init, &r,
+                        D, C, eps, F);
+    r.energy_xc = u.e_xc;
+    r.energy_j = u.e_j;
+    r.energy_k = u.e_k;
+    r.energy_1e = u.e_1e;
+    if (res) *res = r;
+    if (rc == QCG_NOT_CONVERGED) return fail(c, rc, "ScfError::NotConverged");
+    if (rc == QCG_NUMERICAL) return fail(c, rc, "ScfError::SingularOverlap");
+    return rc;
+}

@@ -1,0 +1,22 @@
+// Host-side launch interface of the XC kernels.
+#pragma once
+#include "qcg_common.h"
+
+namespace qcg {
+
+struct GridShell {
+    double cx, cy, cz, amin;  // centre, smallest exponent (screening)
+    int l, nprim, poff, bf;
+};
+
+hipError_t launch_basis_on_grid(int G, int nshell, const GridShell* shells, const double* exps, const double* cn,
+                                const double* pts, int nbf, double* phi, double* dphx, double* dphy, double* dphz,
+                                hipStream_t st);
+hipError_t launch_xc_points(int G, int nbf, const double* w, const double* phi, const double* dphx,
+                            const double* dphy, const double* dphz, const double* X, double* aow, double* exc_sum,
+                            double* rho_out, hipStream_t st);
+hipError_t launch_xc_eval(int n, const double* rho, const double* sigma, double* exc, double* vrho, double* vsigma,
+                          hipStream_t st);
+hipError_t launch_sym_half(const double* V, double* out, int n, hipStream_t st);
+
+}  // namespace qcg

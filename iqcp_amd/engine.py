@@ -34,7 +34,8 @@ EXPORTS = [
     "qcg_create", "qcg_destroy", "qcg_last_error", "qcg_set_basis", "qcg_set_options", "qcg_set_charge",
     "qcg_nbf", "qcg_boys_eval_all", "qcg_rys_roots", "qcg_schwarz_bounds", "qcg_screen_counts",
     "qcg_eri_compressed", "qcg_build_jk", "qcg_build_fock", "qcg_build_jk_device", "qcg_one_electron",
-    "qcg_rhf_scf",
+    "qcg_rhf_scf", "qcg_build_fock_device", "qcg_set_profiling", "qcg_class_stats",
+    "qcg_build_grid", "qcg_set_grid", "qcg_get_grid", "qcg_xc_eval", "qcg_xc", "qcg_xc_device", "qcg_ks_scf",
 ]
 
 
@@ -64,6 +65,18 @@ class ScfConfig(ctypes.Structure):
         for k, v in kw.items():
             setattr(c, k, v)
         return c
+
+
+class ClassStat(ctypes.Structure):
+    """qcg_class_stat: per quartet-class counts, flop model and last profiled kernel times."""
+    _fields_ = [("la", ctypes.c_int), ("lb", ctypes.c_int), ("lc", ctypes.c_int), ("ld", ctypes.c_int),
+                ("quartets_fp64", ctypes.c_int64), ("quartets_fp32", ctypes.c_int64),
+                ("prim_quartets_fp64", ctypes.c_int64), ("prim_quartets_fp32", ctypes.c_int64),
+                ("flops_fp64", ctypes.c_double), ("flops_fp32", ctypes.c_double),
+                ("ms_fp64", ctypes.c_double), ("ms_fp32", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class ScfResult(ctypes.Structure):
@@ -197,6 +210,75 @@ class Engine:
         """Device-pointer J/K build (e.g. torch tensor .data_ptr()) on a HIP stream."""
         self._check(self.L.qcg_build_jk_device(self.ctx, ctypes.c_void_p(dD_ptr), ctypes.c_void_p(dJ_ptr),
                                                ctypes.c_void_p(dK_ptr), ctypes.c_void_p(stream_ptr)))
+
+    def build_fock_device(self, dH_ptr: int, dD_ptr: int, dF_ptr: int, k_factor: float = 0.5,
+                          stream_ptr: int = 0):
+        self._check(self.L.qcg_build_fock_device(self.ctx, ctypes.c_void_p(dH_ptr), ctypes.c_void_p(dD_ptr),
+                                                 ctypes.c_double(k_factor), ctypes.c_void_p(dF_ptr),
+                                                 ctypes.c_void_p(stream_ptr)))
+
+    def set_profiling(self, on: bool):
+        self._check(self.L.qcg_set_profiling(self.ctx, int(bool(on))))
+
+    def class_stats(self):
+        arr = (ClassStat * 21)()
+        n = ctypes.c_int()
+        self._check(self.L.qcg_class_stats(self.ctx, arr, ctypes.byref(n)))
+        return [arr[i].as_dict() for i in range(n.value)]
+
+    # ---- exchange-correlation
+    def build_grid(self, n_radial=75, quality="standard", pruning=True) -> int:
+        """build_becke_grid (dft/grid.rs:103) + Phi / grad Phi on the device; returns the point count."""
+        q = {"standard": 0, "fine": 1}[quality]
+        n = ctypes.c_int64()
+        self._check(self.L.qcg_build_grid(self.ctx, int(n_radial), q, int(bool(pruning)), ctypes.byref(n)))
+        self.npts = n.value
+        return n.value
+
+    def set_grid(self, xyz, w):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        self._check(self.L.qcg_set_grid(self.ctx, ctypes.c_int64(len(w)), _d(xyz), _d(w)))
+        self.npts = len(w)
+
+    def get_grid(self):
+        xyz = np.zeros((self.npts, 3))
+        w = np.zeros(self.npts)
+        at = np.zeros(self.npts, dtype=np.int32)
+        self._check(self.L.qcg_get_grid(self.ctx, _d(xyz), _d(w), _i(at)))
+        return xyz, w, at
+
+    def xc_eval(self, rho, sigma):
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+        exc, vr, vs = np.zeros_like(rho), np.zeros_like(rho), np.zeros_like(rho)
+        self._check(self.L.qcg_xc_eval(self.ctx, ctypes.c_int64(rho.size), _d(rho), _d(sigma), _d(exc), _d(vr),
+                                       _d(vs)))
+        return exc, vr, vs
+
+    def xc(self, D):
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        V = np.zeros_like(D)
+        exc = ctypes.c_double()
+        self._check(self.L.qcg_xc(self.ctx, _d(D), _d(V), ctypes.byref(exc)))
+        return V, exc.value
+
+    def xc_device(self, dD_ptr, dV_ptr, dExc_ptr, stream_ptr=0):
+        self._check(self.L.qcg_xc_device(self.ctx, ctypes.c_void_p(dD_ptr), ctypes.c_void_p(dV_ptr),
+                                         ctypes.c_void_p(dExc_ptr), ctypes.c_void_p(stream_ptr)))
+
+    def ks_scf(self, config: ScfConfig | None = None, D_init=None):
+        """B3LYP KS-SCF on the current grid (ks_scf, dft/ks_scf.rs:1567)."""
+        cfg = config or ScfConfig.tight()
+        n = self.nbf
+        res = ScfResult()
+        D, C, eps, F = np.zeros((n, n)), np.zeros((n, n)), np.zeros(n), np.zeros((n, n))
+        Di = None if D_init is None else np.ascontiguousarray(D_init, dtype=np.float64)
+        self._check(self.L.qcg_ks_scf(self.ctx, ctypes.byref(cfg), _d(Di) if Di is not None else None,
+                                      ctypes.byref(res), _d(D), _d(C), _d(eps), _d(F)))
+        out = res.as_dict()
+        out.update(D=D, C=C, eps=eps, F=F)
+        return out
 
     # ---- one-electron + SCF
     def one_electron(self):

@@ -99,6 +99,13 @@ def build_fock(h, d, eri, nbf):
     return f
 
 
+def build_fock_omp(h, d, eri, nbf, nthreads):
+    f = np.zeros((nbf, nbf))
+    lib().or_build_fock_omp(_d(np.ascontiguousarray(h)), _d(np.ascontiguousarray(d)), _d(eri), nbf, _d(f),
+                            int(nthreads))
+    return f
+
+
 def build_jk(d, eri, nbf):
     j = np.zeros((nbf, nbf))
     k = np.zeros((nbf, nbf))
@@ -126,3 +133,60 @@ def expand_eri(eri, nbf) -> np.ndarray:
     a = np.maximum(P, Q)
     bq = np.minimum(P, Q)
     return eri[a * (a + 1) // 2 + bq]
+
+
+# ------------------------------------------------------------------ DFT (oracle_dft.cpp)
+def becke_grid(Z, xyz, n_radial=75, quality=0, pruning=1):
+    """build_becke_grid (dft/grid.rs:103): (points (G,3), weights (G,), atom index (G,))."""
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    cap = len(Z) * n_radial * (302 if quality == 0 else 590)
+    pts = np.zeros((cap, 3))
+    w = np.zeros(cap)
+    at = np.zeros(cap, dtype=np.int32)
+    L = lib()
+    L.or_becke_grid.restype = ctypes.c_int64
+    n = L.or_becke_grid(len(Z), _i(Z), _d(xyz), n_radial, quality, pruning, ctypes.c_int64(cap), _d(pts), _d(w),
+                        _i(at))
+    if n < 0:
+        raise ValueError("grid construction failed")
+    return pts[:n].copy(), w[:n].copy(), at[:n].copy()
+
+
+def b3lyp(rho, sigma):
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+    exc, vr, vs = np.zeros_like(rho), np.zeros_like(rho), np.zeros_like(rho)
+    lib().or_b3lyp(ctypes.c_int64(rho.size), _d(rho), _d(sigma), _d(exc), _d(vr), _d(vs))
+    return exc, vr, vs
+
+
+def basis_on_grid(b, pts):
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    G = len(pts)
+    chi = np.zeros((G, b.nbf))
+    grad = np.zeros((G, b.nbf, 3))
+    lib().or_basis_on_grid(*_basis_args(b), ctypes.c_int64(G), _d(pts), _d(chi), _d(grad))
+    return chi, grad
+
+
+def vxc(chi, grad, w, D):
+    """build_vxc_gga_matmul (dft/ks_scf.rs:1150) incl. the 512-point block screening: (V, Exc, rho)."""
+    G, nbf = chi.shape
+    V = np.zeros((nbf, nbf))
+    rho = np.zeros(G)
+    L = lib()
+    L.or_vxc.restype = ctypes.c_double
+    exc = L.or_vxc(nbf, ctypes.c_int64(G), _d(np.ascontiguousarray(chi)), _d(np.ascontiguousarray(grad)),
+                   _d(np.ascontiguousarray(w)), _d(np.ascontiguousarray(D)), _d(V), _d(rho))
+    return V, exc, rho
+
+
+def xc_component(which, rho, sigma):
+    """which: 'slater' | 'vwn5' | 'b88' | 'lyp'."""
+    k = {"slater": 0, "vwn5": 1, "b88": 2, "lyp": 3}[which]
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+    exc, vr, vs = np.zeros_like(rho), np.zeros_like(rho), np.zeros_like(rho)
+    lib().or_xc_component(k, ctypes.c_int64(rho.size), _d(rho), _d(sigma), _d(exc), _d(vr), _d(vs))
+    return exc, vr, vs

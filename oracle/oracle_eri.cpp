@@ -629,3 +629,31 @@ int or_build_jk(const double* D, const double* eri, int nbf, double* J, double* 
 }
 
 }  // extern "C"
+
+extern "C" {
+// build_fock (scf/mod.rs:825-881) with the mu loop spread over nthreads OpenMP threads (the
+// (mu, nu) elements are independent, so the result is identical to or_build_fock).  Used as
+// the timed CPU baseline.
+int or_build_fock_omp(const double* H, const double* D, const double* eri, int nbf, double* F, int nthreads) {
+    for (int i = 0; i < nbf * nbf; ++i) F[i] = H[i];
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+    for (int mu = nbf - 1; mu >= 0; --mu)
+        for (int nu = 0; nu <= mu; ++nu) {
+            double g = 0.0;
+            for (int la = 0; la < nbf; ++la) {
+                double dll = D[la * nbf + la];
+                g += dll * (eri[eidx(mu, nu, la, la)] - 0.5 * eri[eidx(mu, la, nu, la)]);
+                for (int si = 0; si < la; ++si) {
+                    double dls = D[la * nbf + si];
+                    double j = eri[eidx(mu, nu, la, si)];
+                    double k1 = eri[eidx(mu, la, nu, si)];
+                    double k2 = eri[eidx(mu, si, nu, la)];
+                    g += dls * (2.0 * j - 0.5 * (k1 + k2));
+                }
+            }
+            F[mu * nbf + nu] += g;
+            if (mu != nu) F[nu * nbf + mu] += g;
+        }
+    return 0;
+}
+}  // extern "C"

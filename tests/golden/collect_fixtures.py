@@ -84,6 +84,8 @@ def main(ref: str):
         "tolerance": 1e-10,
     }
     (OUT / "h2_fock.json").write_text(json.dumps(h2, indent=1))
+    # libxc pointwise vectors quoted in functionals.rs tests are written by hand into
+    # tests/golden/xc_libxc.json (numbers from functionals.rs:1753-1800, 2299-2320, 2427-2450, 2510-2533).
     print("fixtures written to", OUT)
 
 
