@@ -159,6 +159,18 @@ int qcg_one_electron(qcg_context* ctx, double* S, double* T, double* V);
 int qcg_rhf_scf(qcg_context* ctx, const qcg_scf_config* cfg, const double* D_init, qcg_scf_result* res,
                 double* D, double* C, double* eps, double* F);
 
+/* ---- analytic nuclear gradient ---------------------------------------------------------- */
+/* Two-electron part (eri_gradient_fused, scf/gradient.rs:1430): grad[3*natom] += sum over unique
+ * Schwarz-surviving quartets of [1/2 D_pq D_rs - (hf_frac/4) D_pr D_qs] d(pq|rs)/dR.  grad is
+ * overwritten (not accumulated). */
+int qcg_eri_gradient(qcg_context* ctx, const double* D, double hf_frac, double* grad);
+/* One-electron + nuclear part: dVnn/dR + Tr(D dH/dR) - Tr(W dS/dR) (rhf_gradient :915-950).
+ * grad overwritten. */
+int qcg_one_electron_gradient(qcg_context* ctx, const double* D, const double* W, double* grad);
+/* rhf_gradient (scf/gradient.rs:900): W = 2 sum_i<n_occ eps_i C_i C_i^T (:101), then both parts.
+ * C: nbf x nbf, columns = MOs (row-major storage, C[mu*nbf + i]). */
+int qcg_rhf_gradient(qcg_context* ctx, const double* D, const double* C, const double* eps, int n_occ, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

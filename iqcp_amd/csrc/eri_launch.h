@@ -58,6 +58,41 @@ inline hipError_t launch_eri_tensor(int cb, int ck, const JkLaunch& a, double* o
     return hipErrorInvalidValue;
 }
 
+struct GradLaunch {
+    const PairRec* bra;
+    int nbra;
+    const PairRec* ket;
+    int nket;
+    int diag;
+    const PrimPair<double>* pp;  // pad0/pad1 hold the two primitive exponents
+    const double* rys;
+    const double* D;
+    int nbf;
+    double xfac;  // exchange weight c = hf_frac / 4 (eri_gradient_fused, scf/gradient.rs:1438)
+    double thr;
+    int natom;
+    double* grad;  // 3 * natom, accumulated
+};
+
+hipError_t launch_grad_cb0(int ck, const GradLaunch& a, hipStream_t st);
+hipError_t launch_grad_cb1(int ck, const GradLaunch& a, hipStream_t st);
+hipError_t launch_grad_cb2(int ck, const GradLaunch& a, hipStream_t st);
+hipError_t launch_grad_cb3(int ck, const GradLaunch& a, hipStream_t st);
+hipError_t launch_grad_cb4(int ck, const GradLaunch& a, hipStream_t st);
+hipError_t launch_grad_cb5(int ck, const GradLaunch& a, hipStream_t st);
+
+inline hipError_t launch_grad(int cb, int ck, const GradLaunch& a, hipStream_t st) {
+    switch (cb) {
+        case 0: return launch_grad_cb0(ck, a, st);
+        case 1: return launch_grad_cb1(ck, a, st);
+        case 2: return launch_grad_cb2(ck, a, st);
+        case 3: return launch_grad_cb3(ck, a, st);
+        case 4: return launch_grad_cb4(ck, a, st);
+        case 5: return launch_grad_cb5(ck, a, st);
+    }
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_schwarz(int c, const PairRec* pairs, int n, const PrimPair<double>* pp, const double* rys,
                           double* Q, hipStream_t st);
 hipError_t launch_fill(double* x, size_t n, double v, hipStream_t st);

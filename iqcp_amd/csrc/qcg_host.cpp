@@ -154,6 +154,8 @@ int qcg_set_basis(qcg_context* c, int nshell, const int* l, const int* nprim, co
                     q.py = (a * A[1] + b * Bc[1]) / q.p;
                     q.pz = (a * A[2] + b * Bc[2]) / q.p;
                     q.kab = kab;
+                    q.pad0 = a;  // exponents, used by the gradient kernel
+                    q.pad1 = b;
                     q.ck = kab * coefs[prim_off[sa] + i] * coefs[prim_off[sb] + j] * radial_norm(a, l[sa]) *
                            radial_norm(b, l[sb]);
                     pp.push_back(q);
@@ -172,6 +174,7 @@ int qcg_set_basis(qcg_context* c, int nshell, const int* l, const int* nprim, co
     for (size_t i = 0; i < pp.size(); ++i) {
         ppf[i].p = (float)pp[i].p; ppf[i].px = (float)pp[i].px; ppf[i].py = (float)pp[i].py;
         ppf[i].pz = (float)pp[i].pz; ppf[i].kab = (float)pp[i].kab; ppf[i].ck = (float)pp[i].ck;
+        ppf[i].pad0 = (float)pp[i].pad0; ppf[i].pad1 = (float)pp[i].pad1;
     }
     c->h_pp = pp;
     HIPCHK(c, c->alloc(&c->d_pp, pp.size()));

@@ -36,6 +36,7 @@ EXPORTS = [
     "qcg_eri_compressed", "qcg_build_jk", "qcg_build_fock", "qcg_build_jk_device", "qcg_one_electron",
     "qcg_rhf_scf", "qcg_build_fock_device", "qcg_set_profiling", "qcg_class_stats",
     "qcg_build_grid", "qcg_set_grid", "qcg_get_grid", "qcg_xc_eval", "qcg_xc", "qcg_xc_device", "qcg_ks_scf",
+    "qcg_eri_gradient", "qcg_one_electron_gradient", "qcg_rhf_gradient",
 ]
 
 
@@ -279,6 +280,29 @@ class Engine:
         out = res.as_dict()
         out.update(D=D, C=C, eps=eps, F=F)
         return out
+
+    # ---- gradients
+    def eri_gradient(self, D, hf_frac=1.0):
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        g = np.zeros((self.basis.natom, 3))
+        self._check(self.L.qcg_eri_gradient(self.ctx, _d(D), ctypes.c_double(hf_frac), _d(g)))
+        return g
+
+    def one_electron_gradient(self, D, W):
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        g = np.zeros((self.basis.natom, 3))
+        self._check(self.L.qcg_one_electron_gradient(self.ctx, _d(D), _d(W), _d(g)))
+        return g
+
+    def rhf_gradient(self, D, C, eps, n_occ):
+        """rhf_gradient (scf/gradient.rs:900): (natom, 3) in Ha/bohr."""
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        C = np.ascontiguousarray(C, dtype=np.float64)
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        g = np.zeros((self.basis.natom, 3))
+        self._check(self.L.qcg_rhf_gradient(self.ctx, _d(D), _d(C), _d(eps), int(n_occ), _d(g)))
+        return g
 
     # ---- one-electron + SCF
     def one_electron(self):
