@@ -78,6 +78,14 @@ int qcg_set_basis(qcg_context* ctx, int nshell, const int* l, const int* nprim, 
                   const double* exps, const double* coefs, const double* centers, const int* shell_atom,
                   int natom, const int* Z, const double* xyz);
 
+/* A batch of independent molecules (replaces nmol BasisSet::build calls): molecule m owns the next
+ * mol_nshell[m] shells and mol_natom[m] atoms of the concatenated arrays; shell_atom is the atom
+ * index inside the molecule.  Quartets never mix molecules.  Matrices of the batch entry points are
+ * packed block-diagonally: molecule m's nbf_m x nbf_m block follows those of molecules < m. */
+int qcg_set_basis_batch(qcg_context* ctx, int nmol, const int* mol_nshell, const int* mol_natom, const int* l,
+                        const int* nprim, const int* prim_off, const double* exps, const double* coefs,
+                        const double* centers, const int* shell_atom, const int* Z, const double* xyz);
+
 /* precision: QCG_PREC_*; fp32_cutoff: Schwarz-product cutoff below which quartets run in FP32
  * (default 1e-6); schwarz_threshold: SCHWARZ_THRESHOLD (integrals/eri/mod.rs:1258, default 1e-12). */
 int qcg_set_options(qcg_context* ctx, int precision, double fp32_cutoff, double schwarz_threshold);
@@ -158,6 +166,15 @@ int qcg_one_electron(qcg_context* ctx, double* S, double* T, double* V);
  * density (rhf_scf_with_guess, :1610).  Outputs (optional): D, C (columns = MOs), eps, F. */
 int qcg_rhf_scf(qcg_context* ctx, const qcg_scf_config* cfg, const double* D_init, qcg_scf_result* res,
                 double* D, double* C, double* eps, double* F);
+
+/* ---- batches of molecules (after qcg_set_basis_batch) ------------------------------------ */
+/* J, K for every molecule of the batch in one set of launches; D, J, K packed block-diagonally. */
+int qcg_batch_build_jk(qcg_context* ctx, const double* D, double* J, double* K);
+/* rhf_scf for every molecule, iterating together: one batched J/K build per iteration (a captured
+ * HIP graph replayed each iteration when use_graph != 0).  res[nmol]; D (optional, packed);
+ * *ms_jk (optional) = total device time of the J/K builds. */
+int qcg_batch_rhf_scf(qcg_context* ctx, const qcg_scf_config* cfg, int use_graph, qcg_scf_result* res, double* D,
+                      double* ms_jk);
 
 /* ---- analytic nuclear gradient ---------------------------------------------------------- */
 /* Two-electron part (eri_gradient_fused, scf/gradient.rs:1430): grad[3*natom] += sum over unique

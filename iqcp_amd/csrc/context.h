@@ -32,7 +32,18 @@ struct qcg_context {
     PairRec* d_pairs = nullptr;
     PrimPair<double>* d_pp = nullptr;
     PrimPair<float>* d_ppf = nullptr;
-    std::vector<double> Qshell;  // nshell x nshell
+    std::vector<double> Qshell;  // nshell x nshell (single molecule)
+    // batch of molecules: nmol = 1 for qcg_set_basis
+    int nmol = 1, max_nbf = 0;
+    std::vector<HostBasis> mols;             // per-molecule basis (local atom / bf indices)
+    std::vector<int> mol_nbf_h;
+    std::vector<int64_t> mat_off_h;          // offset of each molecule's nbf^2 block
+    std::vector<int> ket_off_h, ket_cnt_h;   // [class][mol], relative to the class start
+    int* d_mol_nbf = nullptr;
+    int64_t* d_mat_off = nullptr;
+    int* d_ket_off = nullptr;
+    int* d_ket_cnt = nullptr;
+    int64_t mat_total = 0;
     // work buffers
     double *d_D = nullptr, *d_J = nullptr, *d_K = nullptr, *d_Jacc = nullptr, *d_Kacc = nullptr;
     double *d_H = nullptr, *d_F = nullptr;
@@ -79,7 +90,8 @@ struct qcg_context {
     }
     void free_basis() {
         for (void** p : {(void**)&d_pairs, (void**)&d_pp, (void**)&d_ppf, (void**)&d_D, (void**)&d_J, (void**)&d_K,
-                         (void**)&d_Jacc, (void**)&d_Kacc, (void**)&d_H, (void**)&d_F}) {
+                         (void**)&d_Jacc, (void**)&d_Kacc, (void**)&d_H, (void**)&d_F, (void**)&d_mol_nbf,
+                         (void**)&d_mat_off, (void**)&d_ket_off, (void**)&d_ket_cnt}) {
             release(*p);
             *p = nullptr;
         }

@@ -42,13 +42,22 @@ __global__ __launch_bounds__(256) void jk_kernel(
     const PairRec* __restrict__ bra, int nbra, const PairRec* __restrict__ ket, int nket, int diag,
     const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_tab, const double* __restrict__ D, int nbf,
     double* __restrict__ Jacc, double* __restrict__ Kacc, double thr_lo, double thr_hi, int do_j, int do_k,
-    int use_lds) {
+    int use_lds, const int* __restrict__ ket_off, const int* __restrict__ ket_cnt,
+    const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf) {
     using S = EriShape<LA, LB, LC, LD>;
     constexpr bool U = S::kUnroll;
     extern __shared__ double klds[];
     __shared__ int s_q0, s_q1;
-    const int pidx = diag ? (nbra - 1 - (int)blockIdx.x) : (int)blockIdx.x;
-    const PairRec P = bra[pidx];
+    const int bidx = diag ? (nbra - 1 - (int)blockIdx.x) : (int)blockIdx.x;
+    const PairRec P = bra[bidx];
+    const int pidx = P.pos;
+    // the molecule's own matrices and ket list
+    ket += ket_off[P.mol];
+    nket = ket_cnt[P.mol];
+    D += mat_off[P.mol];
+    Jacc += mat_off[P.mol];
+    Kacc += mat_off[P.mol];
+    nbf = mol_nbf[P.mol];
     const int tid = threadIdx.x;
     const int krows = S::NA + S::NB;
     if (do_k && use_lds)
@@ -241,7 +250,7 @@ static hipError_t launch_jk_t(const JkLaunch& a, hipStream_t st) {
     if (a.nbra <= 0 || a.nket <= 0) return hipSuccess;
     hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, R>), dim3(a.nbra), dim3(256), lds, st, a.bra, a.nbra, a.ket,
                        a.nket, a.diag, (const PrimPair<R>*)a.pp, (const R*)a.rys, a.D, a.nbf, a.Jacc, a.Kacc,
-                       a.thr_lo, a.thr_hi, a.do_j, a.do_k, use_lds);
+                       a.thr_lo, a.thr_hi, a.do_j, a.do_k, use_lds, a.ket_off, a.ket_cnt, a.mat_off, a.mol_nbf);
     return hipGetLastError();
 }
 

@@ -19,6 +19,13 @@ struct JkLaunch {
     double thr_lo;      // process quartets with thr_lo <= Q_P Q_Q < thr_hi
     double thr_hi;
     int do_j, do_k;
+    // batch of molecules (nmol = 1 for a single molecule): kets of bra pair P are
+    // ket[ket_off[P.mol] .. + ket_cnt[P.mol]); matrices of molecule m start at mat_off[m],
+    // dimension mol_nbf[m]; nbf above is the largest mol_nbf.
+    const int* ket_off;
+    const int* ket_cnt;
+    const int64_t* mat_off;
+    const int* mol_nbf;
 };
 
 hipError_t launch_jk_cb0(int ck, bool fp32, const JkLaunch& a, hipStream_t st);
@@ -97,6 +104,9 @@ hipError_t launch_schwarz(int c, const PairRec* pairs, int n, const PrimPair<dou
                           double* Q, hipStream_t st);
 hipError_t launch_fill(double* x, size_t n, double v, hipStream_t st);
 hipError_t launch_symmetrize(const double* Jacc, const double* Kacc, double* J, double* K, int nbf, hipStream_t st);
+// batched J = Jacc + Jacc^T, K = Kacc + Kacc^T over block-diagonal packed matrices
+hipError_t launch_symmetrize_batch(const double* Jacc, const double* Kacc, double* J, double* K, int nmol,
+                                   const int64_t* mat_off, const int* mol_nbf, int max_nbf, hipStream_t st);
 hipError_t launch_fock_combine(const double* H, const double* Jacc, const double* Kacc, double* F, int nbf,
                                double kfac, hipStream_t st);
 hipError_t launch_boys_batch(int m, int n, const double* T, double* F, int* status, hipStream_t st);

@@ -64,6 +64,24 @@ __global__ void symmetrize_kernel(const double* __restrict__ Jacc, const double*
     if (K) K[(size_t)i * nbf + j] = Kacc[(size_t)i * nbf + j] + Kacc[(size_t)j * nbf + i];
 }
 
+__global__ void symmetrize_batch_kernel(const double* __restrict__ Jacc, const double* __restrict__ Kacc,
+                                        double* __restrict__ J, double* __restrict__ K,
+                                        const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf) {
+    const int m = blockIdx.z, n = mol_nbf[m];
+    const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || j >= n) return;
+    const size_t o = (size_t)mat_off[m];
+    if (J) J[o + (size_t)i * n + j] = Jacc[o + (size_t)i * n + j] + Jacc[o + (size_t)j * n + i];
+    if (K) K[o + (size_t)i * n + j] = Kacc[o + (size_t)i * n + j] + Kacc[o + (size_t)j * n + i];
+}
+
+hipError_t launch_symmetrize_batch(const double* Jacc, const double* Kacc, double* J, double* K, int nmol,
+                                   const int64_t* mat_off, const int* mol_nbf, int max_nbf, hipStream_t st) {
+    hipLaunchKernelGGL(symmetrize_batch_kernel, dim3((max_nbf + 127) / 128, max_nbf, nmol), dim3(128), 0, st, Jacc,
+                       Kacc, J, K, mat_off, mol_nbf);
+    return hipGetLastError();
+}
+
 hipError_t launch_symmetrize(const double* Jacc, const double* Kacc, double* J, double* K, int nbf, hipStream_t st) {
     hipLaunchKernelGGL(symmetrize_kernel, dim3((nbf + 127) / 128, nbf), dim3(128), 0, st, Jacc, Kacc, J, K, nbf);
     return hipGetLastError();

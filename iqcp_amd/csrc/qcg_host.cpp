@@ -94,79 +94,118 @@ int qcg_set_basis(qcg_context* c, int nshell, const int* l, const int* nprim, co
                   const double* exps, const double* coefs, const double* centers, const int* shell_atom,
                   int natom, const int* Z, const double* xyz) {
     if (!c) return QCG_INVALID_ARG;
-    if (nshell <= 0 || !l || !nprim || !prim_off || !exps || !coefs || !centers || natom <= 0 || !Z || !xyz)
+    std::vector<int> sa_local(nshell > 0 ? nshell : 0, 0);
+    if (shell_atom) sa_local.assign(shell_atom, shell_atom + nshell);
+    return qcg_set_basis_batch(c, 1, &nshell, &natom, l, nprim, prim_off, exps, coefs, centers, sa_local.data(), Z,
+                               xyz);
+}
+
+int qcg_set_basis_batch(qcg_context* c, int nmol, const int* mol_nshell, const int* mol_natom, const int* l,
+                        const int* nprim, const int* prim_off, const double* exps, const double* coefs,
+                        const double* centers, const int* shell_atom, const int* Z, const double* xyz) {
+    if (!c) return QCG_INVALID_ARG;
+    if (nmol <= 0 || !mol_nshell || !mol_natom || !l || !nprim || !prim_off || !exps || !coefs || !centers ||
+        !shell_atom || !Z || !xyz)
         return fail(c, QCG_INVALID_ARG, "qcg_set_basis: empty or null input");
     HIPCHK(c, hipSetDevice(c->device));
     c->free_basis();
     c->stats.clear();
-    HostBasis& B = c->basis;
-    B = HostBasis();
-    B.nshell = nshell;
-    B.natom = natom;
-    int nprim_tot = 0;
-    for (int s = 0; s < nshell; ++s) {
-        if (l[s] < 0 || l[s] > kMaxL) return fail(c, QCG_UNSUPPORTED_L, "angular momentum > 2 not supported");
-        if (nprim[s] <= 0) return fail(c, QCG_INVALID_ARG, "shell without primitives");
-        nprim_tot = std::max(nprim_tot, prim_off[s] + nprim[s]);
-    }
-    B.l.assign(l, l + nshell);
-    B.nprim.assign(nprim, nprim + nshell);
-    B.prim_off.assign(prim_off, prim_off + nshell);
-    B.exps.assign(exps, exps + nprim_tot);
-    B.coefs.assign(coefs, coefs + nprim_tot);
-    B.centers.assign(centers, centers + 3 * nshell);
-    B.shell_atom.assign(shell_atom ? shell_atom : l, (shell_atom ? shell_atom : l) + nshell);
-    if (!shell_atom) std::fill(B.shell_atom.begin(), B.shell_atom.end(), 0);
-    B.Z.assign(Z, Z + natom);
-    B.xyz.assign(xyz, xyz + 3 * natom);
-    B.bf_off.resize(nshell);
-    int bf = 0;
-    for (int s = 0; s < nshell; ++s) {
-        B.bf_off[s] = bf;
-        bf += ncart(l[s]);
-    }
-    B.nbf = bf;
-
-    // shell pairs (si >= sj), relabelled so that l(sa) >= l(sb), grouped by class
-    std::vector<std::vector<PairRec>> by_class(kNumPairClasses);
-    std::vector<PrimPair<double>> pp;
-    for (int si = 0; si < nshell; ++si)
-        for (int sj = 0; sj <= si; ++sj) {
-            const int sa = l[si] >= l[sj] ? si : sj, sb = l[si] >= l[sj] ? sj : si;
-            PairRec r{};
-            const double* A = centers + 3 * sa;
-            const double* Bc = centers + 3 * sb;
-            r.ax = A[0]; r.ay = A[1]; r.az = A[2];
-            r.abx = A[0] - Bc[0]; r.aby = A[1] - Bc[1]; r.abz = A[2] - Bc[2];
-            r.sa = sa; r.sb = sb;
-            r.bfa = B.bf_off[sa]; r.bfb = B.bf_off[sb];
-            r.atom_a = B.shell_atom[sa]; r.atom_b = B.shell_atom[sb];
-            r.pp_off = (int)pp.size();
-            const double ab2 = r.abx * r.abx + r.aby * r.aby + r.abz * r.abz;
-            for (int i = 0; i < nprim[sa]; ++i)
-                for (int j = 0; j < nprim[sb]; ++j) {
-                    const double a = exps[prim_off[sa] + i], b = exps[prim_off[sb] + j];
-                    const double kab = std::exp(-(a * b / (a + b)) * ab2);
-                    if (kab < kPrimScreen) continue;  // integrals/eri/mod.rs:603
-                    PrimPair<double> q{};
-                    q.p = a + b;
-                    q.px = (a * A[0] + b * Bc[0]) / q.p;
-                    q.py = (a * A[1] + b * Bc[1]) / q.p;
-                    q.pz = (a * A[2] + b * Bc[2]) / q.p;
-                    q.kab = kab;
-                    q.pad0 = a;  // exponents, used by the gradient kernel
-                    q.pad1 = b;
-                    q.ck = kab * coefs[prim_off[sa] + i] * coefs[prim_off[sb] + j] * radial_norm(a, l[sa]) *
-                           radial_norm(b, l[sb]);
-                    pp.push_back(q);
-                }
-            r.npp = (int)pp.size() - r.pp_off;
-            by_class[pair_class(l[sa], l[sb])].push_back(r);
+    c->nmol = nmol;
+    c->mols.assign(nmol, HostBasis());
+    c->mol_nbf_h.assign(nmol, 0);
+    c->mat_off_h.assign(nmol, 0);
+    int s0 = 0, a0 = 0;
+    int64_t moff = 0;
+    c->max_nbf = 0;
+    for (int m = 0; m < nmol; ++m) {
+        const int ns = mol_nshell[m], na = mol_natom[m];
+        if (ns <= 0 || na <= 0) return fail(c, QCG_INVALID_ARG, "molecule without shells or atoms");
+        HostBasis& B = c->mols[m];
+        B.nshell = ns;
+        B.natom = na;
+        int bf = 0;
+        for (int s = s0; s < s0 + ns; ++s) {
+            if (l[s] < 0 || l[s] > kMaxL) return fail(c, QCG_UNSUPPORTED_L, "angular momentum > 2 not supported");
+            if (nprim[s] <= 0) return fail(c, QCG_INVALID_ARG, "shell without primitives");
+            if (shell_atom[s] < 0 || shell_atom[s] >= na) return fail(c, QCG_INVALID_ARG, "shell atom out of range");
+            B.l.push_back(l[s]);
+            B.nprim.push_back(nprim[s]);
+            B.prim_off.push_back((int)B.exps.size());
+            for (int p = 0; p < nprim[s]; ++p) {
+                B.exps.push_back(exps[prim_off[s] + p]);
+                B.coefs.push_back(coefs[prim_off[s] + p]);
+            }
+            for (int x = 0; x < 3; ++x) B.centers.push_back(centers[3 * s + x]);
+            B.shell_atom.push_back(shell_atom[s]);
+            B.bf_off.push_back(bf);
+            bf += ncart(l[s]);
         }
+        B.nbf = bf;
+        B.Z.assign(Z + a0, Z + a0 + na);
+        B.xyz.assign(xyz + 3 * a0, xyz + 3 * (a0 + na));
+        c->mol_nbf_h[m] = bf;
+        c->mat_off_h[m] = moff;
+        moff += (int64_t)bf * bf;
+        c->max_nbf = std::max(c->max_nbf, bf);
+        s0 += ns;
+        a0 += na;
+    }
+    c->mat_total = moff;
+    c->basis = c->mols[0];
+
+    // shell pairs (si >= sj) of each molecule, relabelled so that l(sa) >= l(sb), grouped by
+    // class and, inside a class, by molecule
+    std::vector<std::vector<std::vector<PairRec>>> by(kNumPairClasses, std::vector<std::vector<PairRec>>(nmol));
+    std::vector<PrimPair<double>> pp;
+    int gs0 = 0;
+    for (int m = 0; m < nmol; ++m) {
+        const HostBasis& B = c->mols[m];
+        for (int si = 0; si < B.nshell; ++si)
+            for (int sj = 0; sj <= si; ++sj) {
+                const int sa = B.l[si] >= B.l[sj] ? si : sj, sb = B.l[si] >= B.l[sj] ? sj : si;
+                PairRec r{};
+                const double* A = &B.centers[3 * sa];
+                const double* Bc = &B.centers[3 * sb];
+                r.ax = A[0]; r.ay = A[1]; r.az = A[2];
+                r.abx = A[0] - Bc[0]; r.aby = A[1] - Bc[1]; r.abz = A[2] - Bc[2];
+                r.sa = gs0 + sa; r.sb = gs0 + sb;
+                r.bfa = B.bf_off[sa]; r.bfb = B.bf_off[sb];
+                r.atom_a = B.shell_atom[sa]; r.atom_b = B.shell_atom[sb];
+                r.mol = m;
+                r.pp_off = (int)pp.size();
+                const double ab2 = r.abx * r.abx + r.aby * r.aby + r.abz * r.abz;
+                for (int i = 0; i < B.nprim[sa]; ++i)
+                    for (int j = 0; j < B.nprim[sb]; ++j) {
+                        const double a = B.exps[B.prim_off[sa] + i], b = B.exps[B.prim_off[sb] + j];
+                        const double kab = std::exp(-(a * b / (a + b)) * ab2);
+                        if (kab < kPrimScreen) continue;  // integrals/eri/mod.rs:603
+                        PrimPair<double> q{};
+                        q.p = a + b;
+                        q.px = (a * A[0] + b * Bc[0]) / q.p;
+                        q.py = (a * A[1] + b * Bc[1]) / q.p;
+                        q.pz = (a * A[2] + b * Bc[2]) / q.p;
+                        q.kab = kab;
+                        q.pad0 = a;  // exponents, used by the gradient kernel
+                        q.pad1 = b;
+                        q.ck = kab * B.coefs[B.prim_off[sa] + i] * B.coefs[B.prim_off[sb] + j] *
+                               radial_norm(a, B.l[sa]) * radial_norm(b, B.l[sb]);
+                        pp.push_back(q);
+                    }
+                r.npp = (int)pp.size() - r.pp_off;
+                by[pair_class(B.l[sa], B.l[sb])][m].push_back(r);
+            }
+        gs0 += B.nshell;
+    }
     std::vector<PairRec> all;
+    c->ket_off_h.assign(kNumPairClasses * nmol, 0);
+    c->ket_cnt_h.assign(kNumPairClasses * nmol, 0);
     for (int k = 0; k < kNumPairClasses; ++k) {
         c->class_off[k] = (int)all.size();
-        all.insert(all.end(), by_class[k].begin(), by_class[k].end());
+        for (int m = 0; m < nmol; ++m) {
+            c->ket_off_h[k * nmol + m] = (int)all.size() - c->class_off[k];
+            c->ket_cnt_h[k * nmol + m] = (int)by[k][m].size();
+            all.insert(all.end(), by[k][m].begin(), by[k][m].end());
+        }
     }
     c->class_off[kNumPairClasses] = (int)all.size();
     const int npairs = (int)all.size();
@@ -194,18 +233,34 @@ int qcg_set_basis(qcg_context* c, int nshell, const int* l, const int* nprim, co
     HIPCHK(c, hipMemcpyAsync(Q.data(), dQ, npairs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->release(dQ);
-    c->Qshell.assign((size_t)nshell * nshell, 0.0);
-    for (int i = 0; i < npairs; ++i) {
-        all[i].Q = Q[i];
-        c->Qshell[(size_t)all[i].sa * nshell + all[i].sb] = Q[i];
-        c->Qshell[(size_t)all[i].sb * nshell + all[i].sa] = Q[i];
+    for (int i = 0; i < npairs; ++i) all[i].Q = Q[i];
+    if (nmol == 1) {
+        const int ns = c->basis.nshell;
+        c->Qshell.assign((size_t)ns * ns, 0.0);
+        for (int i = 0; i < npairs; ++i) {
+            c->Qshell[(size_t)all[i].sa * ns + all[i].sb] = Q[i];
+            c->Qshell[(size_t)all[i].sb * ns + all[i].sa] = Q[i];
+        }
     }
     for (int k = 0; k < kNumPairClasses; ++k)
-        std::stable_sort(all.begin() + c->class_off[k], all.begin() + c->class_off[k + 1],
-                         [](const PairRec& x, const PairRec& y) { return x.Q > y.Q; });
+        for (int m = 0; m < nmol; ++m) {
+            auto first = all.begin() + c->class_off[k] + c->ket_off_h[k * nmol + m];
+            auto last = first + c->ket_cnt_h[k * nmol + m];
+            std::stable_sort(first, last, [](const PairRec& x, const PairRec& y) { return x.Q > y.Q; });
+            int pos = 0;
+            for (auto it = first; it != last; ++it) it->pos = pos++;
+        }
     HIPCHK(c, hipMemcpy(c->d_pairs, all.data(), all.size() * sizeof(PairRec), hipMemcpyHostToDevice));
     c->pairs = all;
-    const size_t n2 = (size_t)B.nbf * B.nbf;
+    HIPCHK(c, c->alloc(&c->d_mol_nbf, nmol));
+    HIPCHK(c, c->alloc(&c->d_mat_off, nmol));
+    HIPCHK(c, c->alloc(&c->d_ket_off, kNumPairClasses * nmol));
+    HIPCHK(c, c->alloc(&c->d_ket_cnt, kNumPairClasses * nmol));
+    HIPCHK(c, hipMemcpy(c->d_mol_nbf, c->mol_nbf_h.data(), nmol * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_mat_off, c->mat_off_h.data(), nmol * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_ket_off, c->ket_off_h.data(), c->ket_off_h.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_ket_cnt, c->ket_cnt_h.data(), c->ket_cnt_h.size() * sizeof(int), hipMemcpyHostToDevice));
+    const size_t n2 = (size_t)c->mat_total;
     HIPCHK(c, c->alloc(&c->d_D, n2));
     HIPCHK(c, c->alloc(&c->d_J, n2));
     HIPCHK(c, c->alloc(&c->d_K, n2));
@@ -257,8 +312,8 @@ int qcg_screen_counts(qcg_context* c, int64_t* n_kept, int64_t* n_fp32) {
 }  // extern "C"
 
 int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStream_t st) {
-    const int nbf = c->basis.nbf;
-    const size_t n2 = (size_t)nbf * nbf;
+    const int nbf = c->max_nbf;
+    const size_t n2 = (size_t)c->mat_total;
     const int do_j = dJ != nullptr, do_k = dK != nullptr;
     if (do_j) HIPCHK(c, launch_fill(c->d_Jacc, n2, 0.0, st));
     if (do_k) HIPCHK(c, launch_fill(c->d_Kacc, n2, 0.0, st));
@@ -283,6 +338,10 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
             a.Kacc = c->d_Kacc;
             a.do_j = do_j;
             a.do_k = do_k;
+            a.ket_off = c->d_ket_off + ck * c->nmol;
+            a.ket_cnt = c->d_ket_cnt + ck * c->nmol;
+            a.mat_off = c->d_mat_off;
+            a.mol_nbf = c->d_mol_nbf;
             a.pp = c->d_pp;
             a.rys = c->d_rys;
             a.thr_lo = mixed ? c->fp32_cutoff : c->schwarz_thr;
@@ -302,7 +361,8 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
                 c->ev_recorded[cls][1] = c->profiling;
             }
         }
-    HIPCHK(c, launch_symmetrize(c->d_Jacc, c->d_Kacc, dJ, dK, nbf, st));
+    if (c->nmol == 1) HIPCHK(c, launch_symmetrize(c->d_Jacc, c->d_Kacc, dJ, dK, nbf, st));
+    else HIPCHK(c, launch_symmetrize_batch(c->d_Jacc, c->d_Kacc, dJ, dK, c->nmol, c->d_mat_off, c->d_mol_nbf, nbf, st));
     return QCG_OK;
 }
 

@@ -28,6 +28,8 @@ struct alignas(16) PairRec {
     int bfa, bfb;       // first basis function of each shell
     int pp_off, npp;    // primitive pairs [pp_off, pp_off + npp) surviving k_ab >= 1e-15
     int atom_a, atom_b;
+    int mol;            // molecule of a batch (0 for a single molecule)
+    int pos;            // position in its molecule's sorted class list
 };
 
 }  // namespace qcg

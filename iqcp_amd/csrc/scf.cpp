@@ -6,181 +6,203 @@
 // optional damping (:1430-1440), AO-basis DIIS on FDS - SDF (compute_diis_error :990,
 // DiisState::extrapolate :1189 with the equal-weight fallback :1248-1286), level shift (:1015),
 // convergence on |dE|, RMS dD and the orbital-gradient norm (:1514-1520).
-// The Fock matrix comes from a callback so the same driver runs the RHF and KS builds.
+// ScfRun holds one molecule's state; scf_driver runs it against a Fock callback, and the batch
+// driver (qcg_batch_rhf_scf) steps many of them in lockstep around one batched Fock build.
 #include "scf.h"
 
 #include <cmath>
 #include <cstring>
-#include <deque>
 #include <vector>
 
 #include "linalg.h"
 
 namespace qcg {
 
-namespace {
-
-struct Diis {
-    int max_size, n2;
-    std::deque<std::vector<double>> f, e;
-    void push(const std::vector<double>& F, const std::vector<double>& E) {
-        if ((int)f.size() >= max_size) {
-            f.pop_front();
-            e.pop_front();
-        }
-        f.push_back(F);
-        e.push_back(E);
-    }
-    static double dot(const std::vector<double>& a, const std::vector<double>& b) {
+std::vector<double> ScfRun::Diis::extrapolate() const {
+    const int n = (int)f.size();
+    if (n == 1) return f[0];
+    auto dot = [](const std::vector<double>& a, const std::vector<double>& b) {
         double s = 0.0;
         for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
         return s;
-    }
-    std::vector<double> average() const {
-        std::vector<double> out(n2, 0.0);
-        const double c = 1.0 / (double)f.size();
+    };
+    auto average = [&]() {
+        std::vector<double> out(f[0].size(), 0.0);
+        const double c = 1.0 / (double)n;
         for (auto& fi : f)
-            for (int j = 0; j < n2; ++j) out[j] += c * fi[j];
+            for (size_t j = 0; j < out.size(); ++j) out[j] += c * fi[j];
         return out;
-    }
-    std::vector<double> extrapolate() const {
-        const int n = (int)f.size();
-        if (n == 1) return f[0];
-        std::vector<double> B((size_t)(n + 1) * (n + 1), 0.0), rhs(n + 1, 0.0), c;
-        for (int i = 0; i < n; ++i)
-            for (int j = 0; j <= i; ++j) B[(size_t)i * (n + 1) + j] = B[(size_t)j * (n + 1) + i] = dot(e[i], e[j]);
-        for (int i = 0; i < n; ++i) B[(size_t)i * (n + 1) + n] = B[(size_t)n * (n + 1) + i] = -1.0;
-        rhs[n] = -1.0;
-        if (!lu_solve(B, rhs, n + 1, c)) return average();
-        double cs = 0.0;
-        for (int i = 0; i < n; ++i) cs += c[i];
-        if (std::fabs(cs - 1.0) > 1e-6) return average();
-        std::vector<double> out(n2, 0.0);
-        for (int i = 0; i < n; ++i)
-            for (int j = 0; j < n2; ++j) out[j] += c[i] * f[i][j];
-        return out;
-    }
-};
+    };
+    std::vector<double> B((size_t)(n + 1) * (n + 1), 0.0), rhs(n + 1, 0.0), c;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) B[(size_t)i * (n + 1) + j] = B[(size_t)j * (n + 1) + i] = dot(e[i], e[j]);
+    for (int i = 0; i < n; ++i) B[(size_t)i * (n + 1) + n] = B[(size_t)n * (n + 1) + i] = -1.0;
+    rhs[n] = -1.0;
+    if (!lu_solve(B, rhs, n + 1, c)) return average();
+    double cs = 0.0;
+    for (int i = 0; i < n; ++i) cs += c[i];
+    if (std::fabs(cs - 1.0) > 1e-6) return average();
+    std::vector<double> out(f[0].size(), 0.0);
+    for (int i = 0; i < n; ++i)
+        for (size_t j = 0; j < out.size(); ++j) out[j] += c[i] * f[i][j];
+    return out;
+}
 
-}  // namespace
+void ScfRun::diagonalize(const std::vector<double>& Fm) {
+    const int n = nbf;
+    std::vector<double> tmp((size_t)n * n), Fp((size_t)n * n), Cp;
+    matmul(Fm.data(), X.data(), tmp.data(), n);         // F X
+    matmul(X.data(), tmp.data(), Fp.data(), n, true);   // X^T F X
+    sym_eigen(Fp, n, eps, Cp);
+    C.assign((size_t)n * n, 0.0);
+    matmul(X.data(), Cp.data(), C.data(), n);           // C = X C'
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < nocc; ++k) s += C[(size_t)i * n + k] * C[(size_t)j * n + k];
+            D[(size_t)i * n + j] = 2.0 * s;
+        }
+}
+
+int ScfRun::init(int nbf_, int nocc_, const double* S_, const double* H_, double e_nuc_, const ScfConfig& cfg_,
+                 const double* D_init) {
+    nbf = nbf_;
+    nocc = nocc_;
+    cfg = cfg_;
+    e_nuc = e_nuc_;
+    const size_t n2 = (size_t)nbf * nbf;
+    if (nbf <= 0 || nocc < 0 || nocc > nbf) return QCG_INVALID_ARG;
+    S.assign(S_, S_ + n2);
+    H.assign(H_, H_ + n2);
+    std::vector<double> sw, sU;
+    sym_eigen(S, nbf, sw, sU);
+    if (sw[0] <= 1e-10) return QCG_NUMERICAL;  // ScfError::SingularOverlap
+    X.assign(n2, 0.0);
+    for (int k = 0; k < nbf; ++k) {
+        const double f = 1.0 / std::sqrt(sw[k]);
+        for (int i = 0; i < nbf; ++i) {
+            const double a = sU[(size_t)i * nbf + k] * f;
+            for (int j = 0; j < nbf; ++j) X[(size_t)i * nbf + j] += a * sU[(size_t)j * nbf + k];
+        }
+    }
+    D.assign(n2, 0.0);
+    if (D_init) std::memcpy(D.data(), D_init, n2 * sizeof(double));
+    else diagonalize(H);
+    diis.max_size = cfg.diis_size > 0 ? cfg.diis_size : 1;
+    iter = 0;
+    done = converged = false;
+    have_last = false;
+    return QCG_OK;
+}
+
+// F and E_elec for the current D.  Returns true when this molecule is finished.
+bool ScfRun::accept_fock(const double* F_, double e_elec) {
+    const size_t n2 = (size_t)nbf * nbf;
+    F.assign(F_, F_ + n2);
+    const double e_new = e_elec + e_nuc;
+    if (iter > 0) {
+        F_last = F;
+        have_last = true;
+        last_dE = std::fabs(e_new - e_tot);
+        double ss = 0.0;
+        for (size_t i = 0; i < n2; ++i) ss += (D[i] - Dold[i]) * (D[i] - Dold[i]);
+        last_rms = std::sqrt(ss / (double)n2);
+        std::vector<double> tmp(n2);
+        matmul(F.data(), C.data(), tmp.data(), nbf);
+        double g2 = 0.0;
+        for (int v = nocc; v < nbf; ++v)
+            for (int o = 0; o < nocc; ++o) {
+                double s = 0.0;
+                for (int k = 0; k < nbf; ++k) s += C[(size_t)k * nbf + v] * tmp[(size_t)k * nbf + o];
+                g2 += s * s;
+            }
+        const int nvir = nbf - nocc;
+        const double gnorm = nvir > 0 ? std::sqrt(4.0 * g2) / std::sqrt((double)nvir * nocc) : 0.0;
+        e_tot = e_new;
+        if (last_dE < cfg.energy_tol && last_rms < cfg.density_tol && gnorm < std::sqrt(cfg.energy_tol)) {
+            converged = done = true;
+            if (std::fabs(cfg.level_shift) > 1e-10) diagonalize(F);
+            return true;
+        }
+        if (iter >= cfg.max_iterations) {
+            done = true;
+            return true;
+        }
+    } else {
+        e_tot = e_new;
+    }
+    return false;
+}
+
+// Next density from the current Fock matrix (damping, DIIS, level shift, diagonalisation).
+// Damping mixes with the Fock matrix of the latest build, as rhf_scf does (fock_last is set right
+// after every build, scf/mod.rs:1497-1500), so it only changes rounding.
+void ScfRun::step() {
+    const int n = nbf;
+    const size_t n2 = (size_t)n * n;
+    ++iter;
+    Dold = D;
+    std::vector<double> Fd = F;
+    const bool damp = std::fabs(cfg.damp) > 1e-4 && have_last && (cfg.damp_start == 0 || iter < cfg.damp_start);
+    if (damp)
+        for (size_t i = 0; i < n2; ++i) Fd[i] = F_last[i] * cfg.damp + F[i] * (1.0 - cfg.damp);
+    std::vector<double> Fdiag = Fd;
+    if (cfg.use_diis && iter >= cfg.diis_start) {
+        std::vector<double> FD(n2), FDS(n2), SD(n2), SDF(n2), Err(n2);
+        matmul(Fd.data(), D.data(), FD.data(), n);
+        matmul(FD.data(), S.data(), FDS.data(), n);
+        matmul(S.data(), D.data(), SD.data(), n);
+        matmul(SD.data(), Fd.data(), SDF.data(), n);
+        for (size_t i = 0; i < n2; ++i) Err[i] = FDS[i] - SDF[i];
+        if ((int)diis.f.size() >= diis.max_size) {
+            diis.f.pop_front();
+            diis.e.pop_front();
+        }
+        diis.f.push_back(Fd);
+        diis.e.push_back(Err);
+        if (diis.f.size() >= 2) Fdiag = diis.extrapolate();
+    }
+    if (std::fabs(cfg.level_shift) >= 1e-10) {
+        std::vector<double> SD(n2), SDS(n2);
+        matmul(S.data(), D.data(), SD.data(), n);
+        matmul(SD.data(), S.data(), SDS.data(), n);
+        for (size_t i = 0; i < n2; ++i) Fdiag[i] += cfg.level_shift * (S[i] - 0.5 * SDS[i]);
+    }
+    diagonalize(Fdiag);
+}
+
+void ScfRun::fill_result(ScfResult* res) const {
+    if (!res) return;
+    res->converged = converged ? 1 : 0;
+    res->iterations = converged ? iter : cfg.max_iterations;
+    res->energy_total = e_tot;
+    res->energy_electronic = e_tot - e_nuc;
+    res->energy_nuclear = e_nuc;
+    res->last_delta_e = last_dE;
+    res->last_rms_d = last_rms;
+}
 
 int scf_driver(int nbf, int nocc, const double* S, const double* H, double e_nuc, const ScfConfig& cfg,
                FockFn fock, void* user, const double* D_init, ScfResult* res, double* D_out, double* C_out,
                double* eps_out, double* F_out) {
-    const int n = nbf;
-    const size_t n2 = (size_t)n * n;
-    if (n <= 0 || nocc < 0 || nocc > n) return QCG_INVALID_ARG;
-    std::vector<double> Sv(S, S + n2), Hv(H, H + n2);
-    // X = U diag(1/sqrt(l)) U^T
-    std::vector<double> sw, sU;
-    sym_eigen(Sv, n, sw, sU);
-    if (sw[0] <= 1e-10) return QCG_NUMERICAL;  // ScfError::SingularOverlap
-    std::vector<double> X(n2, 0.0);
-    for (int k = 0; k < n; ++k) {
-        const double f = 1.0 / std::sqrt(sw[k]);
-        for (int i = 0; i < n; ++i) {
-            const double a = sU[(size_t)i * n + k] * f;
-            for (int j = 0; j < n; ++j) X[(size_t)i * n + j] += a * sU[(size_t)j * n + k];
-        }
-    }
-    std::vector<double> tmp(n2), Fp(n2), C(n2), eps, Cp;
-    auto diagonalize = [&](const std::vector<double>& F) {
-        matmul(F.data(), X.data(), tmp.data(), n);          // F X
-        matmul(X.data(), tmp.data(), Fp.data(), n, true);   // X^T F X
-        sym_eigen(Fp, n, eps, Cp);
-        matmul(X.data(), Cp.data(), C.data(), n);           // C = X C'
-    };
-    auto density = [&](std::vector<double>& D) {
-        for (int i = 0; i < n; ++i)
-            for (int j = 0; j < n; ++j) {
-                double s = 0.0;
-                for (int k = 0; k < nocc; ++k) s += C[(size_t)i * n + k] * C[(size_t)j * n + k];
-                D[(size_t)i * n + j] = 2.0 * s;
-            }
-    };
-    std::vector<double> D(n2), F(n2);
-    if (D_init) {
-        std::memcpy(D.data(), D_init, n2 * sizeof(double));
-    } else {
-        diagonalize(Hv);
-        density(D);
-    }
-    double e_elec = 0.0;
-    int rc = fock(user, D.data(), F.data(), &e_elec);
+    ScfRun run;
+    int rc = run.init(nbf, nocc, S, H, e_nuc, cfg, D_init);
     if (rc) return rc;
-    double e_tot = e_elec + e_nuc;
-    Diis diis{cfg.diis_size > 0 ? cfg.diis_size : 1, (int)n2, {}, {}};
-    std::vector<double> F_last;
-    bool have_last = false, converged = false;
-    int iter = 0;
-    std::vector<double> FD(n2), FDS(n2), SD(n2), SDF(n2), Err(n2), Dold(n2);
-    for (iter = 1; iter <= cfg.max_iterations; ++iter) {
-        const double e_old = e_tot;
-        Dold = D;
-        std::vector<double> Fd = F;
-        const bool damp = std::fabs(cfg.damp) > 1e-4 && have_last && (cfg.damp_start == 0 || iter < cfg.damp_start);
-        if (damp)
-            for (size_t i = 0; i < n2; ++i) Fd[i] = F_last[i] * cfg.damp + F[i] * (1.0 - cfg.damp);
-        std::vector<double> Fdiag = Fd;
-        if (cfg.use_diis && iter >= cfg.diis_start) {
-            matmul(Fd.data(), D.data(), FD.data(), n);
-            matmul(FD.data(), Sv.data(), FDS.data(), n);
-            matmul(Sv.data(), D.data(), SD.data(), n);
-            matmul(SD.data(), Fd.data(), SDF.data(), n);
-            for (size_t i = 0; i < n2; ++i) Err[i] = FDS[i] - SDF[i];
-            diis.push(Fd, Err);
-            if (diis.f.size() >= 2) Fdiag = diis.extrapolate();
-        }
-        if (std::fabs(cfg.level_shift) >= 1e-10) {
-            // F + shift (S - S (D/2) S)
-            matmul(Sv.data(), D.data(), SD.data(), n);
-            matmul(SD.data(), Sv.data(), tmp.data(), n);
-            for (size_t i = 0; i < n2; ++i) Fdiag[i] += cfg.level_shift * (Sv[i] - 0.5 * tmp[i]);
-        }
-        diagonalize(Fdiag);
-        density(D);
-        rc = fock(user, D.data(), F.data(), &e_elec);
+    std::vector<double> F((size_t)nbf * nbf);
+    for (;;) {
+        double e_elec = 0.0;
+        rc = fock(user, run.D.data(), F.data(), &e_elec);
         if (rc) return rc;
-        F_last = F;
-        have_last = true;
-        e_tot = e_elec + e_nuc;
-        const double dE = std::fabs(e_tot - e_old);
-        double ss = 0.0;
-        for (size_t i = 0; i < n2; ++i) ss += (D[i] - Dold[i]) * (D[i] - Dold[i]);
-        const double rms = std::sqrt(ss / (double)n2);
-        // orbital gradient: C^T F C, virtual-occupied block
-        matmul(F.data(), C.data(), tmp.data(), n);
-        double g2 = 0.0;
-        const int nvir = n - nocc;
-        for (int v = nocc; v < n; ++v)
-            for (int o = 0; o < nocc; ++o) {
-                double s = 0.0;
-                for (int k = 0; k < n; ++k) s += C[(size_t)k * n + v] * tmp[(size_t)k * n + o];
-                g2 += s * s;
-            }
-        const double gnorm = nvir > 0 ? std::sqrt(4.0 * g2) / std::sqrt((double)nvir * nocc) : 0.0;
-        if (res) {
-            res->last_delta_e = dE;
-            res->last_rms_d = rms;
-        }
-        if (dE < cfg.energy_tol && rms < cfg.density_tol && gnorm < std::sqrt(cfg.energy_tol)) {
-            converged = true;
-            break;
-        }
+        if (run.accept_fock(F.data(), e_elec)) break;
+        run.step();
     }
-    if (converged && std::fabs(cfg.level_shift) > 1e-10) diagonalize(F);
-    if (res) {
-        res->converged = converged ? 1 : 0;
-        res->iterations = converged ? iter : cfg.max_iterations;
-        res->energy_total = e_tot;
-        res->energy_electronic = e_tot - e_nuc;
-        res->energy_nuclear = e_nuc;
-    }
-    if (D_out) std::memcpy(D_out, D.data(), n2 * sizeof(double));
-    if (C_out) std::memcpy(C_out, C.data(), n2 * sizeof(double));
-    if (eps_out) std::memcpy(eps_out, eps.data(), n * sizeof(double));
-    if (F_out) std::memcpy(F_out, F.data(), n2 * sizeof(double));
-    return converged ? QCG_OK : QCG_NOT_CONVERGED;
+    run.fill_result(res);
+    const size_t n2 = (size_t)nbf * nbf;
+    if (D_out) std::memcpy(D_out, run.D.data(), n2 * sizeof(double));
+    if (C_out && !run.C.empty()) std::memcpy(C_out, run.C.data(), n2 * sizeof(double));
+    if (eps_out && !run.eps.empty()) std::memcpy(eps_out, run.eps.data(), nbf * sizeof(double));
+    if (F_out) std::memcpy(F_out, run.F.data(), n2 * sizeof(double));
+    return run.converged ? QCG_OK : QCG_NOT_CONVERGED;
 }
 
 }  // namespace qcg

@@ -37,6 +37,7 @@ EXPORTS = [
     "qcg_rhf_scf", "qcg_build_fock_device", "qcg_set_profiling", "qcg_class_stats",
     "qcg_build_grid", "qcg_set_grid", "qcg_get_grid", "qcg_xc_eval", "qcg_xc", "qcg_xc_device", "qcg_ks_scf",
     "qcg_eri_gradient", "qcg_one_electron_gradient", "qcg_rhf_gradient",
+    "qcg_set_basis_batch", "qcg_batch_build_jk", "qcg_batch_rhf_scf",
 ]
 
 
@@ -322,3 +323,72 @@ class Engine:
         out = res.as_dict()
         out.update(D=D, C=C, eps=eps, F=F)
         return out
+
+
+class BatchEngine:
+    """Many molecules in one context (qcg_set_basis_batch): batched J/K and lockstep SCF."""
+
+    def __init__(self, bases, device: int = 0, precision: str = "fp64", fp32_cutoff: float = 1e-6):
+        self.L = lib()
+        self.ctx = ctypes.c_void_p()
+        rc = self.L.qcg_create(ctypes.byref(self.ctx), device)
+        if rc:
+            raise QcgError(rc, "qcg_create failed (no GPU?)")
+        self.bases = bases
+        p = {"fp64": QCG_PREC_FP64, "mixed": QCG_PREC_MIXED}[precision]
+        self._check(self.L.qcg_set_options(self.ctx, p, ctypes.c_double(fp32_cutoff), ctypes.c_double(1e-12)))
+        ns = np.array([b.nshell for b in bases], np.int32)
+        na = np.array([b.natom for b in bases], np.int32)
+        l = np.concatenate([b.shell_l for b in bases]).astype(np.int32)
+        npr = np.concatenate([b.shell_nprim for b in bases]).astype(np.int32)
+        po, off = [], 0
+        for b in bases:
+            po.append(b.shell_prim_off + off)
+            off += len(b.exps)
+        po = np.concatenate(po).astype(np.int32)
+        ex = np.concatenate([b.exps for b in bases])
+        co = np.concatenate([b.coefs for b in bases])
+        ce = np.ascontiguousarray(np.concatenate([b.shell_center for b in bases]))
+        sa = np.concatenate([b.shell_atom for b in bases]).astype(np.int32)
+        Z = np.concatenate([b.Z for b in bases]).astype(np.int32)
+        xyz = np.ascontiguousarray(np.concatenate([b.xyz for b in bases]))
+        self._check(self.L.qcg_set_basis_batch(self.ctx, len(bases), _i(ns), _i(na), _i(l), _i(npr), _i(po),
+                                               _d(ex), _d(co), _d(ce), _i(sa), _i(Z), _d(xyz)))
+        self.nbf = [b.nbf for b in bases]
+        self.offsets = np.cumsum([0] + [n * n for n in self.nbf])
+
+    def _check(self, rc):
+        if rc:
+            raise QcgError(rc, self.L.qcg_last_error(self.ctx).decode())
+
+    def close(self):
+        if self.ctx:
+            self.L.qcg_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pack(self, mats):
+        return np.concatenate([np.ascontiguousarray(m, dtype=np.float64).ravel() for m in mats])
+
+    def unpack(self, flat):
+        return [flat[self.offsets[i]:self.offsets[i + 1]].reshape(n, n) for i, n in enumerate(self.nbf)]
+
+    def build_jk(self, Ds):
+        D = self.pack(Ds)
+        J, K = np.zeros_like(D), np.zeros_like(D)
+        self._check(self.L.qcg_batch_build_jk(self.ctx, _d(D), _d(J), _d(K)))
+        return self.unpack(J), self.unpack(K)
+
+    def rhf_scf(self, config: ScfConfig | None = None, use_graph: bool = True):
+        cfg = config or ScfConfig.tight()
+        res = (ScfResult * len(self.bases))()
+        D = np.zeros(int(self.offsets[-1]))
+        ms = ctypes.c_double()
+        self._check(self.L.qcg_batch_rhf_scf(self.ctx, ctypes.byref(cfg), int(bool(use_graph)), res, _d(D),
+                                             ctypes.byref(ms)))
+        return [res[i].as_dict() for i in range(len(self.bases))], self.unpack(D), ms.value

@@ -1,0 +1,41 @@
+"""Batched molecules (64 n-alkanes, RHF/6-31G): one launch per quartet class for all molecules."""
+import numpy as np
+import pytest
+
+from helpers import bench_molecule, random_density
+from iqcp_amd import workloads
+from iqcp_amd.engine import BatchEngine, Engine, ScfConfig
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batch_jk_equals_single():
+    bases = [bench_molecule(m, b) for m, b in (("H2O", "6-31g*"), ("CH4", "6-31g"), ("NH3", "6-31+g*"), ("HF", "sto-3g"))]
+    be = BatchEngine(bases)
+    Ds = [random_density(b.nbf, seed=i, nocc=b.n_occ) for i, b in enumerate(bases)]
+    Js, Ks = be.build_jk(Ds)
+    for b, D, J, K in zip(bases, Ds, Js, Ks):
+        jk = Engine(b).build_jk(D)
+        np.testing.assert_allclose(J, jk.J, atol=1e-12, rtol=0)
+        np.testing.assert_allclose(K, jk.K, atol=1e-12, rtol=0)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_scf_equals_single(use_graph):
+    bases = [bench_molecule(m, b) for m, b in (("H2O", "6-31g*"), ("CH4", "6-31g"), ("HF", "6-31g"))]
+    res, Ds, _ = BatchEngine(bases).rhf_scf(ScfConfig.tight(), use_graph=use_graph)
+    for b, r in zip(bases, res):
+        single = Engine(b).rhf_scf(ScfConfig.tight())
+        assert r["converged"] == 1
+        assert abs(r["energy_total"] - single["energy_total"]) < 1e-9
+
+
+def test_alkane_batch_64():
+    """The 64-molecule workload: every molecule converges; three checked against single runs."""
+    bases = workloads.alkane_batch_631g()
+    assert len(bases) == 64 and min(b.nbf for b in bases) == 56 and max(b.nbf for b in bases) == 160
+    res, Ds, ms = BatchEngine(bases).rhf_scf(ScfConfig.tight())
+    assert all(r["converged"] == 1 for r in res)
+    for i in (0, 17, 63):
+        single = Engine(bases[i]).rhf_scf(ScfConfig.tight())
+        assert abs(res[i]["energy_total"] - single["energy_total"]) < 1e-9
