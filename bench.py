@@ -63,8 +63,11 @@ def main():
     dF = torch.empty_like(dD)
     stream = torch.cuda.Stream(device=dev)
 
+    tiles = {}
+
     def timed(prec):
         eng.set_options(prec, args.fp32_cutoff)
+        tiles[prec] = eng.autotune(dD.data_ptr(), 2)  # per-class tile choice for this precision mode
         with torch.cuda.stream(stream):
             for _ in range(args.warmup):
                 eng.build_fock_device(dH.data_ptr(), dD.data_ptr(), dF.data_ptr(), 0.5, stream.cuda_stream)
@@ -86,6 +89,7 @@ def main():
     per_kernel = {}
     for prec in ("fp64", "mixed"):
         eng.set_options(prec, args.fp32_cutoff)
+        eng.autotune(dD.data_ptr(), 2)
         eng.set_profiling(True)
         acc = None
         with torch.cuda.stream(stream):
@@ -103,6 +107,14 @@ def main():
             a_["ms_fp64"] /= args.steps
             a_["ms_fp32"] /= args.steps
         per_kernel[prec] = acc
+    for prec in ("fp64", "mixed"):
+        log(f"per-class kernels ({prec}): class  quartets64 quartets32  ms64  ms32  model-TFLOP/s")
+        for x in per_kernel[prec]:
+            c_ = f"({'spd'[x['la']]}{'spd'[x['lb']]}|{'spd'[x['lc']]}{'spd'[x['ld']]})"
+            t = (x["ms_fp64"] + x["ms_fp32"]) * 1e-3
+            fl = x["flops_fp64"] + x["flops_fp32"]
+            log(f"  {c_} {x['quartets_fp64']:10d} {x['quartets_fp32']:10d} {x['ms_fp64']:7.3f} {x['ms_fp32']:7.3f}"
+                f" {fl / t / 1e12 if t > 0 else 0:7.3f}")
     # dominant kernel of the mixed build
     best = None
     for s in per_kernel["mixed"]:
@@ -163,6 +175,7 @@ def main():
         "scf_energy_fp64": scf["energy_total"],
         "scf_iterations": scf["iterations"],
         "all_kernels_model_tflops": round(tot_fl / (tot_ms * 1e-3) / 1e12, 3) if tot_ms > 0 else None,
+        "tiles_mixed": tiles.get("mixed"),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

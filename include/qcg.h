@@ -135,6 +135,10 @@ typedef struct qcg_class_stat {
     double ms_fp64, ms_fp32;
 } qcg_class_stat;
 int qcg_set_profiling(qcg_context* ctx, int on);
+/* Autotuner for the J/K kernels: for every quartet class, time each candidate tile (threads per
+ * block x kets per thread) on the device density dD (iters timed builds per candidate) and keep
+ * the fastest.  chosen (optional, 42 ints) receives (block, kets/thread) per class. */
+int qcg_autotune(qcg_context* ctx, const double* dD, int iters, int* chosen);
 /* out must hold 21 entries; *n receives the number written. */
 int qcg_class_stats(qcg_context* ctx, qcg_class_stat* out, int* n);
 

@@ -54,6 +54,17 @@ struct qcg_context {
     hipEvent_t ev[2 * 21 * 2 + 2] = {};
     bool ev_made = false;
     bool ev_recorded[21][2] = {};
+    // J/K work plan: items per (class pair, precision part), rebuilt when options or tiles change
+    struct Tile { int block = 64, reps = 1; };
+    Tile tile[21];
+    std::vector<int4> items;
+    int item_off[21][2] = {}, item_cnt[21][2] = {};
+    int4* d_items = nullptr;
+    double plan_key[4] = {-1.0, -1.0, -1.0, -1.0};
+    int plan_sig = -1, tile_sig = 0;
+    static constexpr int kStreams = 4;
+    hipStream_t aux[kStreams] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[kStreams] = {};
     // cached class counts / flop model, valid for (precision, fp32_cutoff, schwarz_thr) of stats_key
     std::vector<qcg_class_stat> stats;
     // exchange-correlation grid and per-point buffers
@@ -91,11 +102,12 @@ struct qcg_context {
     void free_basis() {
         for (void** p : {(void**)&d_pairs, (void**)&d_pp, (void**)&d_ppf, (void**)&d_D, (void**)&d_J, (void**)&d_K,
                          (void**)&d_Jacc, (void**)&d_Kacc, (void**)&d_H, (void**)&d_F, (void**)&d_mol_nbf,
-                         (void**)&d_mat_off, (void**)&d_ket_off, (void**)&d_ket_cnt}) {
+                         (void**)&d_mat_off, (void**)&d_ket_off, (void**)&d_ket_cnt, (void**)&d_items}) {
             release(*p);
             *p = nullptr;
         }
         has_basis = false;
+        plan_sig = -1;
         free_grid();
     }
 };

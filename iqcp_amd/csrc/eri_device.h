@@ -39,7 +39,7 @@ QCG_HD constexpr double comp_norm(int l, int c) {
 }
 
 // The same factors as a literal table: kCompNorm[l][c].
-__device__ constexpr double kCompNorm[3][6] = {
+__host__ __device__ constexpr double kCompNorm[3][6] = {
     {1.0, 0, 0, 0, 0, 0},
     {1.0, 1.0, 1.0, 0, 0, 0},
     {0.57735026918962576451, 1.0, 1.0, 0.57735026918962576451, 1.0, 0.57735026918962576451}};
@@ -126,7 +126,7 @@ QCG_HD void hrr1d(const R* g, R ab, R cd, R* out) {
 // Run body(i) for i in [0, N): fully unrolled when the class is small enough to live in
 // registers, a plain loop otherwise (large classes keep their arrays in scratch).
 template <bool UNROLL, int N, typename F>
-__device__ __forceinline__ void for_n(F&& body) {
+QCG_HD void for_n(F&& body) {
     if constexpr (UNROLL) {
 #pragma unroll
         for (int i = 0; i < N; ++i) body(i);
@@ -140,7 +140,7 @@ __device__ __forceinline__ void for_n(F&& body) {
 // [a][b][c][d] over Cartesian components; the primitive-dependent normalisation is folded
 // into ck, the component normalisation is NOT applied here).
 template <int LA, int LB, int LC, int LD, typename R>
-__device__ __forceinline__ void contract_quartet(const PairRec& P, const PairRec& Q,
+QCG_HD void contract_quartet(const PairRec& P, const PairRec& Q,
                                                  const PrimPair<R>* __restrict__ pp,
                                                  const R* __restrict__ rys_tab, R* eri) {
     using S = EriShape<LA, LB, LC, LD>;

@@ -3,16 +3,17 @@
 // in parallel).  Pair class c <-> (la, lb): 0 (s,s) 1 (p,s) 2 (p,p) 3 (d,s) 4 (d,p) 5 (d,d).
 //
 // Work decomposition of the J/K build (one launch per quartet class, bra class >= ket class):
-//   * one workgroup per bra shell pair P (pairs of a class are sorted by Schwarz bound, desc);
-//   * its threads stride over the ket pairs Q of the ket class; for bra class == ket class only
-//     Q at or before P in the sorted list (each unordered pair-pair once = 8-fold symmetry);
-//   * the surviving kets are the prefix where Q_P*Q_Q >= threshold (sorted list, monotone
-//     product), found by binary search on exactly the product the reference tests
-//     (q_ij * q_kl < SCHWARZ_THRESHOLD, integrals/eri/mod.rs:1008-1023); in mixed precision the
-//     prefix splits into an FP64 range (Q_P*Q_Q >= cutoff) and an FP32 range;
-//   * digestion: J_P in registers + wave reduction, J_Q by FP64 atomics, K rows of the bra
-//     shells (all 4 K terms of a quartet land in rows of shell a or b) accumulated in LDS with
-//     ds_add_f64 and flushed once per workgroup.  J = Jacc + Jacc^T, K = Kacc + Kacc^T after.
+//   * pairs of a class are sorted by Schwarz bound (descending); for a bra pair P the surviving
+//     kets of a ket class are the prefix with Q_P*Q_Q >= threshold (monotone product), found by
+//     binary search on exactly the product the reference tests (q_ij * q_kl < SCHWARZ_THRESHOLD,
+//     integrals/eri/mod.rs:1008-1023); for bra class == ket class only kets at or before P
+//     (each unordered pair-pair once = 8-fold symmetry); in mixed precision the prefix splits
+//     into an FP64 range (Q_P*Q_Q >= cutoff) and an FP32 range;
+//   * the host cuts every range into work items (bra pair, ket sub-range) of a per-class size;
+//     one block per item, one thread per ket (several for cheap classes), so every class
+//     launch has many more blocks than the GPU has CUs;
+//   * digestion: J_P in registers + wave reduction, J_Q and the four K terms by FP64 atomics
+//     (all land in L2).  J = Jacc + Jacc^T, K = Kacc + Kacc^T after.
 //   * symmetry factors: 1/2 for a == b shells, 1/2 for c == d shells, 1/2 for P == Q, folded
 //     together with the component normalisation into the density sub-blocks.
 // With the symmetric density this reproduces build_fock's G = J - K/2 (scf/mod.rs:825-881) and
@@ -37,38 +38,25 @@ __device__ __forceinline__ int first_below(const PairRec* __restrict__ ket, int 
     return lo;
 }
 
+// One work item = (bra pair, contiguous range of its surviving kets); the host builds the items
+// with the exact screening predicate, so no quartet is tested twice or skipped.
 template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(256) void jk_kernel(
-    const PairRec* __restrict__ bra, int nbra, const PairRec* __restrict__ ket, int nket, int diag,
-    const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_tab, const double* __restrict__ D, int nbf,
-    double* __restrict__ Jacc, double* __restrict__ Kacc, double thr_lo, double thr_hi, int do_j, int do_k,
-    int use_lds, const int* __restrict__ ket_off, const int* __restrict__ ket_cnt,
+    const int4* __restrict__ items, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket, int diag,
+    const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_tab, const double* __restrict__ D,
+    double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
     const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf) {
     using S = EriShape<LA, LB, LC, LD>;
     constexpr bool U = S::kUnroll;
-    extern __shared__ double klds[];
-    __shared__ int s_q0, s_q1;
-    const int bidx = diag ? (nbra - 1 - (int)blockIdx.x) : (int)blockIdx.x;
-    const PairRec P = bra[bidx];
+    const int4 it = items[blockIdx.x];
+    const PairRec P = bra[it.x];
     const int pidx = P.pos;
-    // the molecule's own matrices and ket list
     ket += ket_off[P.mol];
-    nket = ket_cnt[P.mol];
     D += mat_off[P.mol];
     Jacc += mat_off[P.mol];
     Kacc += mat_off[P.mol];
-    nbf = mol_nbf[P.mol];
+    const int nbf = mol_nbf[P.mol];
     const int tid = threadIdx.x;
-    const int krows = S::NA + S::NB;
-    if (do_k && use_lds)
-        for (int i = tid; i < krows * nbf; i += blockDim.x) klds[i] = 0.0;
-    if (tid == 0) {
-        const int qmax = diag ? pidx + 1 : nket;
-        s_q0 = (thr_hi > 1e300) ? 0 : first_below(ket, qmax, P.Q, thr_hi);
-        s_q1 = first_below(ket, qmax, P.Q, thr_lo);
-    }
-    __syncthreads();
-    const int q0 = s_q0, q1 = s_q1;
 
     // bra density block scaled by the bra component norms
     double Dps[S::NAB], jp[S::NAB];
@@ -79,10 +67,8 @@ __global__ __launch_bounds__(256) void jk_kernel(
     });
     const double deg_p = (P.sa == P.sb) ? 0.5 : 1.0;
 
-    for (int q = q0 + tid; q < q1; q += blockDim.x) {
+    for (int q = it.y + tid; q < it.z; q += blockDim.x) {
         const PairRec Q = ket[q];
-        const double qq = P.Q * Q.Q;
-        if (qq < thr_lo || qq >= thr_hi) continue;
         R eri[S::NQ];
         for_n<U, S::NQ>([&](int i) { eri[i] = R(0); });
         contract_quartet<LA, LB, LC, LD, R>(P, Q, pp, rys_tab, eri);
@@ -127,10 +113,6 @@ __global__ __launch_bounds__(256) void jk_kernel(
                 const int a = i / S::NC, c = i % S::NC;
                 Dac[i] = D[(size_t)(P.bfa + a) * nbf + Q.bfa + c] * kCompNorm[LA][a] * kCompNorm[LC][c];
             });
-            auto kadd = [&](int row_local, int row_global, int col, double val) {
-                if (use_lds) atomicAdd(&klds[(size_t)row_local * nbf + col], val);
-                else atomicAdd(&Kacc[(size_t)row_global * nbf + col], val);
-            };
 #define VIDX(a, b, c, d) ((((a) * S::NB + (b)) * S::NC + (c)) * S::ND + (d))
             for_n<U, S::NA * S::NC>([&](int ac) {
                 const int a = ac / S::NC, c = ac % S::NC;
@@ -139,7 +121,7 @@ __global__ __launch_bounds__(256) void jk_kernel(
                     const int b = bd / S::ND, d = bd % S::ND;
                     s += Dbd[bd] * (double)eri[VIDX(a, b, c, d)];
                 });
-                kadd(a, P.bfa + a, Q.bfa + c, deg * kCompNorm[LA][a] * kCompNorm[LC][c] * s);
+                atomicAdd(&Kacc[(size_t)(P.bfa + a) * nbf + Q.bfa + c], deg * kCompNorm[LA][a] * kCompNorm[LC][c] * s);
             });
             for_n<U, S::NB * S::NC>([&](int bc) {
                 const int b = bc / S::NC, c = bc % S::NC;
@@ -148,7 +130,7 @@ __global__ __launch_bounds__(256) void jk_kernel(
                     const int a = ad / S::ND, d = ad % S::ND;
                     s += Dad[ad] * (double)eri[VIDX(a, b, c, d)];
                 });
-                kadd(S::NA + b, P.bfb + b, Q.bfa + c, deg * kCompNorm[LB][b] * kCompNorm[LC][c] * s);
+                atomicAdd(&Kacc[(size_t)(P.bfb + b) * nbf + Q.bfa + c], deg * kCompNorm[LB][b] * kCompNorm[LC][c] * s);
             });
             for_n<U, S::NA * S::ND>([&](int ad) {
                 const int a = ad / S::ND, d = ad % S::ND;
@@ -157,7 +139,7 @@ __global__ __launch_bounds__(256) void jk_kernel(
                     const int b = bc / S::NC, c = bc % S::NC;
                     s += Dbc[bc] * (double)eri[VIDX(a, b, c, d)];
                 });
-                kadd(a, P.bfa + a, Q.bfb + d, deg * kCompNorm[LA][a] * kCompNorm[LD][d] * s);
+                atomicAdd(&Kacc[(size_t)(P.bfa + a) * nbf + Q.bfb + d], deg * kCompNorm[LA][a] * kCompNorm[LD][d] * s);
             });
             for_n<U, S::NB * S::ND>([&](int bd) {
                 const int b = bd / S::ND, d = bd % S::ND;
@@ -166,7 +148,7 @@ __global__ __launch_bounds__(256) void jk_kernel(
                     const int a = ac / S::NC, c = ac % S::NC;
                     s += Dac[ac] * (double)eri[VIDX(a, b, c, d)];
                 });
-                kadd(S::NA + b, P.bfb + b, Q.bfb + d, deg * kCompNorm[LB][b] * kCompNorm[LD][d] * s);
+                atomicAdd(&Kacc[(size_t)(P.bfb + b) * nbf + Q.bfb + d], deg * kCompNorm[LB][b] * kCompNorm[LD][d] * s);
             });
 #undef VIDX
         }
@@ -182,17 +164,6 @@ __global__ __launch_bounds__(256) void jk_kernel(
                 atomicAdd(&Jacc[(size_t)(P.bfa + a) * nbf + P.bfb + b], s);
             }
         });
-    }
-    if (do_k && use_lds) {
-        __syncthreads();
-        for (int i = tid; i < krows * nbf; i += blockDim.x) {
-            const double val = klds[i];
-            if (val != 0.0) {
-                const int r = i / nbf, col = i - r * nbf;
-                const int row = r < S::NA ? P.bfa + r : P.bfb + (r - S::NA);
-                atomicAdd(&Kacc[(size_t)row * nbf + col], val);
-            }
-        }
     }
 }
 
@@ -243,14 +214,10 @@ __global__ __launch_bounds__(256) void eri_tensor_kernel(
 template <int CB, int CK, typename R>
 static hipError_t launch_jk_t(const JkLaunch& a, hipStream_t st) {
     constexpr int LA = QCG_LA(CB), LB = QCG_LB(CB), LC = QCG_LA(CK), LD = QCG_LB(CK);
-    using S = EriShape<LA, LB, LC, LD>;
-    size_t lds = a.do_k ? (size_t)(S::NA + S::NB) * a.nbf * sizeof(double) : 0;
-    int use_lds = (lds > 0 && lds <= 96 * 1024) ? 1 : 0;
-    if (!use_lds) lds = 0;
-    if (a.nbra <= 0 || a.nket <= 0) return hipSuccess;
-    hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, R>), dim3(a.nbra), dim3(256), lds, st, a.bra, a.nbra, a.ket,
-                       a.nket, a.diag, (const PrimPair<R>*)a.pp, (const R*)a.rys, a.D, a.nbf, a.Jacc, a.Kacc,
-                       a.thr_lo, a.thr_hi, a.do_j, a.do_k, use_lds, a.ket_off, a.ket_cnt, a.mat_off, a.mol_nbf);
+    if (a.nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, R>), dim3(a.nitems), dim3(a.block), 0, st, a.items, a.bra, a.ket,
+                       a.diag, (const PrimPair<R>*)a.pp, (const R*)a.rys, a.D, a.Jacc, a.Kacc, a.do_j, a.do_k,
+                       a.ket_off, a.mat_off, a.mol_nbf);
     return hipGetLastError();
 }
 

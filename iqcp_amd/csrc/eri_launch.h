@@ -26,6 +26,11 @@ struct JkLaunch {
     const int* ket_cnt;
     const int64_t* mat_off;
     const int* mol_nbf;
+    // J/K work items: (index in the bra class list, ket begin, ket end, 0), ket indices relative to
+    // the molecule's list; one block of `block` threads per item
+    const int4* items;
+    int nitems;
+    int block;
 };
 
 hipError_t launch_jk_cb0(int ck, bool fp32, const JkLaunch& a, hipStream_t st);

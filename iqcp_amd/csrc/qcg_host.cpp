@@ -48,6 +48,16 @@ int qcg_create(qcg_context** out, int device) {
     for (size_t i = 0; i < nt; ++i) tf[i] = (float)qcg_rys_coef[i];
     if (hipMemcpy(c->d_rys, qcg_rys_coef, nt * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return QCG_DEVICE;
     if (hipMemcpy(c->d_rys_f, tf.data(), nt * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return QCG_DEVICE;
+    // default tiles: several kets per thread for the cheapest classes
+    int cls = 0;
+    for (int cb = kNumPairClasses - 1; cb >= 0; --cb)
+        for (int ck = cb; ck >= 0; --ck, ++cls) {
+            const int la = cb == 0 ? 0 : cb <= 2 ? 1 : 2, lb = (cb == 0 || cb == 1 || cb == 3) ? 0 : (cb == 2 || cb == 4) ? 1 : 2;
+            const int lc = ck == 0 ? 0 : ck <= 2 ? 1 : 2, ld = (ck == 0 || ck == 1 || ck == 3) ? 0 : (ck == 2 || ck == 4) ? 1 : 2;
+            const int nq = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+            c->tile[cls].block = 64;
+            c->tile[cls].reps = nq <= 3 ? 4 : nq <= 9 ? 2 : 1;
+        }
     *out = c.release();
     return QCG_OK;
 }
@@ -60,6 +70,13 @@ void qcg_destroy(qcg_context* c) {
     c->owned.clear();
     if (c->ev_made)
         for (auto& e : c->ev) (void)hipEventDestroy(e);
+    if (c->ev_fork) {
+        (void)hipEventDestroy(c->ev_fork);
+        for (int i = 0; i < qcg_context::kStreams; ++i) {
+            (void)hipEventDestroy(c->ev_join[i]);
+            (void)hipStreamDestroy(c->aux[i]);
+        }
+    }
     if (c->blas) (void)rocblas_destroy_handle(c->blas);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -311,10 +328,63 @@ int qcg_screen_counts(qcg_context* c, int64_t* n_kept, int64_t* n_fp32) {
 
 }  // extern "C"
 
+static int build_plan(qcg_context* c) {
+    const bool mixed = c->precision == QCG_PREC_MIXED && c->fp32_cutoff > c->schwarz_thr;
+    const double key[4] = {(double)c->precision, c->fp32_cutoff, c->schwarz_thr, (double)c->tile_sig};
+    if (c->plan_sig == 1 && std::memcmp(key, c->plan_key, sizeof(key)) == 0) return QCG_OK;
+    c->items.clear();
+    const int nmol = c->nmol;
+    auto first_below = [&](int base, int qmax, double Qp, double thr) {
+        int lo = 0, hi = qmax;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (Qp * c->pairs[base + mid].Q >= thr) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    int cls = 0;
+    for (int cb = kNumPairClasses - 1; cb >= 0; --cb)
+        for (int ck = cb; ck >= 0; --ck, ++cls) {
+            const int ch = std::max(1, c->tile[cls].block * c->tile[cls].reps);
+            for (int part = 0; part < 2; ++part) {
+                c->item_off[cls][part] = (int)c->items.size();
+                if (part == 1 && !mixed) {
+                    c->item_cnt[cls][part] = 0;
+                    continue;
+                }
+                const double lo_thr = (part == 0 && mixed) ? c->fp32_cutoff : c->schwarz_thr;
+                const bool has_hi = part == 1;
+                for (int m = 0; m < nmol; ++m) {
+                    const int nb = c->ket_cnt_h[cb * nmol + m], nk = c->ket_cnt_h[ck * nmol + m];
+                    const int pbase = c->ket_off_h[cb * nmol + m];
+                    const int kbase = c->class_off[ck] + c->ket_off_h[ck * nmol + m];
+                    for (int p = 0; p < nb; ++p) {
+                        const double Qp = c->pairs[c->class_off[cb] + pbase + p].Q;
+                        const int qmax = (cb == ck) ? p + 1 : nk;
+                        const int e = first_below(kbase, qmax, Qp, lo_thr);
+                        const int s = has_hi ? std::min(e, first_below(kbase, qmax, Qp, c->fp32_cutoff)) : 0;
+                        for (int q0 = s; q0 < e; q0 += ch) c->items.push_back(make_int4(pbase + p, q0, std::min(q0 + ch, e), 0));
+                    }
+                }
+                c->item_cnt[cls][part] = (int)c->items.size() - c->item_off[cls][part];
+            }
+        }
+    c->release(c->d_items);
+    c->d_items = nullptr;
+    HIPCHK(c, c->alloc(&c->d_items, c->items.size()));
+    HIPCHK(c, hipMemcpy(c->d_items, c->items.data(), c->items.size() * sizeof(int4), hipMemcpyHostToDevice));
+    std::memcpy(c->plan_key, key, sizeof(key));
+    c->plan_sig = 1;
+    return QCG_OK;
+}
+
 int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStream_t st) {
     const int nbf = c->max_nbf;
     const size_t n2 = (size_t)c->mat_total;
     const int do_j = dJ != nullptr, do_k = dK != nullptr;
+    int rc = build_plan(c);
+    if (rc) return rc;
     if (do_j) HIPCHK(c, launch_fill(c->d_Jacc, n2, 0.0, st));
     if (do_k) HIPCHK(c, launch_fill(c->d_Kacc, n2, 0.0, st));
     const bool mixed = c->precision == QCG_PREC_MIXED && c->fp32_cutoff > c->schwarz_thr;
@@ -322,44 +392,57 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
         for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
         c->ev_made = true;
     }
+    // class kernels spread over the auxiliary streams (forked from and joined back to st);
+    // in profiling mode everything runs on st so the events bracket single kernels
+    const bool fork = !c->profiling;
+    if (fork) {
+        if (!c->ev_fork) {
+            HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+            for (int i = 0; i < qcg_context::kStreams; ++i) {
+                HIPCHK(c, hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking));
+                HIPCHK(c, hipEventCreateWithFlags(&c->ev_join[i], hipEventDisableTiming));
+            }
+        }
+        HIPCHK(c, hipEventRecord(c->ev_fork, st));
+        for (int i = 0; i < qcg_context::kStreams; ++i) HIPCHK(c, hipStreamWaitEvent(c->aux[i], c->ev_fork, 0));
+    }
     std::memset(c->ev_recorded, 0, sizeof(c->ev_recorded));
-    int cls = 0;
+    int cls = 0, launched = 0;
     for (int cb = kNumPairClasses - 1; cb >= 0; --cb)
         for (int ck = cb; ck >= 0; --ck, ++cls) {
-            JkLaunch a{};
-            a.bra = c->d_pairs + c->class_off[cb];
-            a.nbra = c->class_off[cb + 1] - c->class_off[cb];
-            a.ket = c->d_pairs + c->class_off[ck];
-            a.nket = c->class_off[ck + 1] - c->class_off[ck];
-            a.diag = cb == ck;
-            a.D = dD;
-            a.nbf = nbf;
-            a.Jacc = c->d_Jacc;
-            a.Kacc = c->d_Kacc;
-            a.do_j = do_j;
-            a.do_k = do_k;
-            a.ket_off = c->d_ket_off + ck * c->nmol;
-            a.ket_cnt = c->d_ket_cnt + ck * c->nmol;
-            a.mat_off = c->d_mat_off;
-            a.mol_nbf = c->d_mol_nbf;
-            a.pp = c->d_pp;
-            a.rys = c->d_rys;
-            a.thr_lo = mixed ? c->fp32_cutoff : c->schwarz_thr;
-            a.thr_hi = 1e308 * 10.0;
-            if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4 * cls + 0], st));
-            HIPCHK(c, launch_jk(cb, ck, false, a, st));
-            if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4 * cls + 1], st));
-            c->ev_recorded[cls][0] = c->profiling;
-            if (mixed) {
-                a.pp = c->d_ppf;
-                a.rys = c->d_rys_f;
-                a.thr_lo = c->schwarz_thr;
-                a.thr_hi = c->fp32_cutoff;
-                if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4 * cls + 2], st));
-                HIPCHK(c, launch_jk(cb, ck, true, a, st));
-                if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4 * cls + 3], st));
-                c->ev_recorded[cls][1] = c->profiling;
+            for (int part = 0; part < 2; ++part) {
+                if (part == 1 && !mixed) continue;
+                JkLaunch a{};
+                a.bra = c->d_pairs + c->class_off[cb];
+                a.ket = c->d_pairs + c->class_off[ck];
+                a.diag = cb == ck;
+                a.D = dD;
+                a.nbf = nbf;
+                a.Jacc = c->d_Jacc;
+                a.Kacc = c->d_Kacc;
+                a.do_j = do_j;
+                a.do_k = do_k;
+                a.ket_off = c->d_ket_off + ck * c->nmol;
+                a.ket_cnt = c->d_ket_cnt + ck * c->nmol;
+                a.mat_off = c->d_mat_off;
+                a.mol_nbf = c->d_mol_nbf;
+                a.items = c->d_items + c->item_off[cls][part];
+                a.nitems = c->item_cnt[cls][part];
+                a.block = c->tile[cls].block;
+                a.pp = part ? (const void*)c->d_ppf : (const void*)c->d_pp;
+                a.rys = part ? (const void*)c->d_rys_f : (const void*)c->d_rys;
+                hipStream_t s = fork ? c->aux[launched % qcg_context::kStreams] : st;
+                if (a.nitems > 0) ++launched;
+                if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4 * cls + 2 * part], s));
+                HIPCHK(c, launch_jk(cb, ck, part == 1, a, s));
+                if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4 * cls + 2 * part + 1], s));
+                c->ev_recorded[cls][part] = c->profiling;
             }
+        }
+    if (fork)
+        for (int i = 0; i < qcg_context::kStreams; ++i) {
+            HIPCHK(c, hipEventRecord(c->ev_join[i], c->aux[i]));
+            HIPCHK(c, hipStreamWaitEvent(st, c->ev_join[i], 0));
         }
     if (c->nmol == 1) HIPCHK(c, launch_symmetrize(c->d_Jacc, c->d_Kacc, dJ, dK, nbf, st));
     else HIPCHK(c, launch_symmetrize_batch(c->d_Jacc, c->d_Kacc, dJ, dK, c->nmol, c->d_mat_off, c->d_mol_nbf, nbf, st));
@@ -367,6 +450,48 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
 }
 
 extern "C" {
+
+// Autotuner: time every class kernel under each candidate tile (threads per block x kets per
+// thread) on the given density and keep the fastest candidate per class.
+int qcg_autotune(qcg_context* c, const double* dD, int iters, int* chosen) {
+    if (!c || !dD) return QCG_INVALID_ARG;
+    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int cand[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {256, 1}, {64, 8}};
+    const int ncand = sizeof(cand) / sizeof(cand[0]);
+    std::vector<double> best(21, 1e300);
+    std::vector<int> pick(21, 0);
+    const bool prof = c->profiling;
+    c->profiling = true;
+    iters = std::max(1, iters);
+    for (int k = 0; k < ncand; ++k) {
+        for (int i = 0; i < 21; ++i) c->tile[i] = {cand[k][0], cand[k][1]};
+        c->tile_sig += 1;
+        std::vector<double> tot(21, 0.0);
+        for (int it = 0; it < iters + 1; ++it) {
+            int rc = run_jk(c, dD, c->d_J, c->d_K, c->stream);
+            if (rc) { c->profiling = prof; return rc; }
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            if (it == 0) continue;  // warm-up
+            for (int i = 0; i < 21; ++i)
+                for (int part = 0; part < 2; ++part)
+                    if (c->ev_recorded[i][part]) {
+                        float ms = 0.f;
+                        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[4 * i + 2 * part], c->ev[4 * i + 2 * part + 1]));
+                        tot[i] += ms;
+                    }
+        }
+        for (int i = 0; i < 21; ++i)
+            if (tot[i] < best[i]) { best[i] = tot[i]; pick[i] = k; }
+    }
+    for (int i = 0; i < 21; ++i) {
+        c->tile[i] = {cand[pick[i]][0], cand[pick[i]][1]};
+        if (chosen) { chosen[2 * i] = c->tile[i].block; chosen[2 * i + 1] = c->tile[i].reps; }
+    }
+    c->tile_sig += 1;
+    c->profiling = prof;
+    return QCG_OK;
+}
 
 int qcg_set_profiling(qcg_context* c, int on) {
     if (!c) return QCG_INVALID_ARG;

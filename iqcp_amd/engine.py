@@ -37,7 +37,7 @@ EXPORTS = [
     "qcg_rhf_scf", "qcg_build_fock_device", "qcg_set_profiling", "qcg_class_stats",
     "qcg_build_grid", "qcg_set_grid", "qcg_get_grid", "qcg_xc_eval", "qcg_xc", "qcg_xc_device", "qcg_ks_scf",
     "qcg_eri_gradient", "qcg_one_electron_gradient", "qcg_rhf_gradient",
-    "qcg_set_basis_batch", "qcg_batch_build_jk", "qcg_batch_rhf_scf",
+    "qcg_set_basis_batch", "qcg_batch_build_jk", "qcg_batch_rhf_scf", "qcg_autotune",
 ]
 
 
@@ -218,6 +218,12 @@ class Engine:
         self._check(self.L.qcg_build_fock_device(self.ctx, ctypes.c_void_p(dH_ptr), ctypes.c_void_p(dD_ptr),
                                                  ctypes.c_double(k_factor), ctypes.c_void_p(dF_ptr),
                                                  ctypes.c_void_p(stream_ptr)))
+
+    def autotune(self, dD_ptr: int, iters: int = 2):
+        """Pick the fastest tile per quartet class on a device density; returns [(block, kets/thread)]."""
+        ch = (ctypes.c_int * 42)()
+        self._check(self.L.qcg_autotune(self.ctx, ctypes.c_void_p(dD_ptr), int(iters), ch))
+        return [(ch[2 * i], ch[2 * i + 1]) for i in range(21)]
 
     def set_profiling(self, on: bool):
         self._check(self.L.qcg_set_profiling(self.ctx, int(bool(on))))
