@@ -37,8 +37,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build runs the Fock build on a single MI355X "
-              "(multi-GPU sharding of the quartet queue is described in DESIGN.md, not run).", file=sys.stderr)
+        print("bench.py measures one GPU only: the Fock build runs on a single MI355X; sharding the "
+              "quartet work items over several GPUs is designed but not implemented.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -136,9 +136,15 @@ typedef struct qcg_class_stat {
 } qcg_class_stat;
 int qcg_set_profiling(qcg_context* ctx, int on);
 /* Autotuner for the J/K kernels: for every quartet class, time each candidate tile (threads per
- * block x kets per thread) on the device density dD (iters timed builds per candidate) and keep
- * the fastest.  chosen (optional, 42 ints) receives (block, kets/thread) per class. */
+ * block x kets per thread, or block 0 = one wavefront per contracted quartet, kets per work item)
+ * on the device density dD (iters timed builds per candidate) and keep the fastest.  chosen
+ * (optional, 42 ints) receives (block, kets) per class. */
 int qcg_autotune(qcg_context* ctx, const double* dD, int iters, int* chosen);
+/* Set / read the J/K tiles directly (42 ints, same layout as qcg_autotune's chosen; classes in the
+ * order (dd|dd), (dd|dp), ..., (ss|ss)).  No counterpart in qc-core: launch configuration only.
+ * Block 0 on a class with fewer than 27 components runs the per-lane kernel with 64 threads. */
+int qcg_set_tiles(qcg_context* ctx, const int* tiles);
+int qcg_get_tiles(qcg_context* ctx, int* tiles);
 /* out must hold 21 entries; *n receives the number written. */
 int qcg_class_stats(qcg_context* ctx, qcg_class_stat* out, int* n);
 

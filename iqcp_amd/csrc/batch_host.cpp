@@ -55,10 +55,16 @@ int qcg_batch_rhf_scf(qcg_context* c, const qcg_scf_config* cfg, int use_graph, 
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     if (use_graph) {
-        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = run_jk(c, c->d_D, c->d_J, c->d_K, c->stream);
-        hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        int rc = prepare_jk(c);  // plan and streams exist before capture: run_jk then only enqueues
         if (rc) return rc;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = run_jk(c, c->d_D, c->d_J, c->d_K, c->stream);
+        hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipGetLastError();  // do not leave the capture error for the next call to find
+            return rc;
+        }
         HIPCHK(c, e);
         HIPCHK(c, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     }

@@ -57,6 +57,7 @@ struct qcg_context {
     // J/K work plan: items per (class pair, precision part), rebuilt when options or tiles change
     struct Tile { int block = 64, reps = 1; };
     Tile tile[21];
+    int class_nq[21] = {};  // Cartesian components per quartet class
     std::vector<int4> items;
     int item_off[21][2] = {}, item_cnt[21][2] = {};
     int4* d_items = nullptr;
@@ -122,6 +123,9 @@ inline int fail(qcg_context* c, int code, const std::string& msg) {
 
 // Integral-direct J/K into Jacc/Kacc then J = Jacc + Jacc^T, K = Kacc + Kacc^T (dJ/dK may be null).
 int run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStream_t st);
+// Everything run_jk needs that allocates or creates runtime objects (work-item plan, auxiliary
+// streams, fork/join events), done ahead so that run_jk itself can be captured into a graph.
+int prepare_jk(qcg_context* c);
 double radial_norm(double a, int l);
 
 #define HIPCHK(ctx, expr)                                                                        \

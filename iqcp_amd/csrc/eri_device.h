@@ -89,8 +89,8 @@ QCG_HD void vrr2d(R* g, R c00, R c0p, R b00, R b10, R b01) {
 
 // 1-D horizontal transfer of one axis: out[((ia*(LB+1)+ib)*(LC+1)+ic)*(LD+1)+id]
 // (i,j,k,l+1) = (i,j,k+1,l) + CD (i,j,k,l);  (i,j+1,k,l) = (i+1,j,k,l) + AB (i,j,k,l)
-template <int LA, int LB, int LC, int LD, typename R>
-QCG_HD void hrr1d(const R* g, R ab, R cd, R* out) {
+template <int LA, int LB, int LC, int LD, typename R, typename Put>
+QCG_HD void hrr1d_put(const R* g, R ab, R cd, Put&& put) {
     constexpr int LAB = LA + LB, LCD = LC + LD, GC = LCD + 1;
     R t[(LAB + 1) * (LCD + 1) * (LD + 1)];
 #define T3(n, k, l) t[((n) * (LCD + 1) + (k)) * (LD + 1) + (l)]
@@ -118,9 +118,14 @@ QCG_HD void hrr1d(const R* g, R ab, R cd, R* out) {
 #pragma unroll
             for (int i = 0; i <= LA; ++i)
 #pragma unroll
-                for (int j = 0; j <= LB; ++j) out[((i * (LB + 1) + j) * (LC + 1) + k) * (LD + 1) + l] = u[i * (LB + 1) + j];
+                for (int j = 0; j <= LB; ++j) put(((i * (LB + 1) + j) * (LC + 1) + k) * (LD + 1) + l, u[i * (LB + 1) + j]);
         }
 #undef T3
+}
+
+template <int LA, int LB, int LC, int LD, typename R>
+QCG_HD void hrr1d(const R* g, R ab, R cd, R* out) {
+    hrr1d_put<LA, LB, LC, LD, R>(g, ab, cd, [&](int h, R v) { out[h] = v; });
 }
 
 // Run body(i) for i in [0, N): fully unrolled when the class is small enough to live in

@@ -10,12 +10,16 @@
 // (the sum of 1/2 D_pq D_rs - c D_pr D_qs over the 8 index permutations).  Derivatives of a
 // primitive with respect to its centre: d/dA_x phi_i = 2 alpha phi_(i+1) - i phi_(i-1), applied to
 // the 1-D Rys factors for centres A, B, C; centre D by translational invariance, -(A + B + C).
-// The density weight W is formed once per contracted quartet; the primitive/root loop then
-// accumulates nine scalars (A, B, C x 3 directions): an adjoint contraction, no derivative
-// integrals are ever stored.  Gradients of the bra atoms accumulate in registers, the ket atoms
-// in LDS (natom x 3), flushed with FP64 atomics once per workgroup.
-#include "eri_grad_device.h"
+//
+// Wave-cooperative (eri_wave.h): one wavefront per work item (bra pair, range of kets), one
+// contracted quartet at a time.  Phase A writes the 1-D transfer tables with powers up to
+// (la+1, lb+1, lc+1, ld) for (L+1)/2 + 1 Rys nodes (the derivative integrals have total angular
+// momentum L+1) and the three primitive exponents of every primitive quartet to LDS; phase B has
+// the lanes over Cartesian components: W_k times the nine derivative products (A, B, C x xyz),
+// summed over the entries.  No derivative integral is stored; the density weights W are formed
+// once per quartet from density blocks staged in LDS.
 #include "eri_launch.h"
+#include "eri_wave.h"
 
 #ifndef QCG_CB
 #error "compile with -DQCG_CB=<bra pair class>"
@@ -23,76 +27,156 @@
 
 namespace qcg {
 
-__device__ __forceinline__ int gfirst_below(const PairRec* __restrict__ ket, int qmax, double Qp, double thr) {
-    int lo = 0, hi = qmax;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (Qp * ket[mid].Q >= thr) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
+template <int LA, int LB, int LC, int LD>
+struct GradWave {
+    using S = EriShape<LA, LB, LC, LD>;
+    static constexpr int N = (LA + LB + LC + LD + 1) / 2 + 1;
+    using T = WaveTable<LA + 1, LB + 1, LC + 1, LD, N, double>;
+    static constexpr int DB = S::NAB + S::NCD + S::NA * S::NC + S::NB * S::ND + S::NA * S::ND + S::NB * S::NC;
+    static constexpr int WAVE_D = T::DOUBLES + 3 * T::PQ_CAP + S::NQ + DB;
+    // waves per block: four while their LDS fits in 64 KB, else two
+    static constexpr int WPB = 4 * WAVE_D * 8 <= 65536 ? 4 : 2;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
 }
 
 template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(256) void grad_kernel(const PairRec* __restrict__ bra, int nbra,
-                                                   const PairRec* __restrict__ ket, int nket, int diag,
-                                                   const PrimPair<double>* __restrict__ pp,
-                                                   const double* __restrict__ rys_tab, const double* __restrict__ D,
-                                                   int nbf, double xfac, double thr, int natom,
-                                                   double* __restrict__ grad) {
+__global__ __launch_bounds__(256) void grad_wave_kernel(const int4* __restrict__ items, int nitems,
+                                                        const PairRec* __restrict__ bra, const PairRec* __restrict__ ket,
+                                                        int diag, const PrimPair<double>* __restrict__ pp,
+                                                        const double* __restrict__ rys_tab,
+                                                        const double* __restrict__ D, int nbf_, double xfac,
+                                                        double* __restrict__ grad) {
     using S = EriShape<LA, LB, LC, LD>;
-    // extended tables: powers up to (LA+1, LB+1, LC+1, LD)
-    constexpr int EA = LA + 1, EB = LB + 1, EC = LC + 1, ED = LD;
-    constexpr int NB_ = LA + LB + 2, NK_ = LC + LD + 1;
-    constexpr int GE = (NB_ + 1) * (NK_ + 1);
-    constexpr int HE = (EA + 1) * (EB + 1) * (EC + 1) * (ED + 1);
-    constexpr int N = (NB_ + NK_) / 2 + 1;
-    constexpr bool U = S::kUnroll;
-    extern __shared__ double gl[];  // natom * 3 ket-atom gradient
-    __shared__ int s_q1;
-    const int pidx = diag ? (nbra - 1 - (int)blockIdx.x) : (int)blockIdx.x;
-    const PairRec P = bra[pidx];
-    const int tid = threadIdx.x;
-    for (int i = tid; i < 3 * natom; i += blockDim.x) gl[i] = 0.0;
-    if (tid == 0) s_q1 = gfirst_below(ket, diag ? pidx + 1 : nket, P.Q, thr);
-    __syncthreads();
-    const int q1 = s_q1;
+    using G = GradWave<LA, LB, LC, LD>;
+    using TB = typename G::T;
+    constexpr int N = G::N, HT = TB::HT, ES = TB::ES;
+    constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND, NCD = S::NCD;
+    constexpr int TB1 = LB + 2, TC1 = LC + 2, TD1 = LD + 1;
+    constexpr int sI = TB1 * TC1 * TD1, sJ = TC1 * TD1, sK = TD1;
+    __shared__ double smem[G::WPB * G::WAVE_D];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int item = blockIdx.x * G::WPB + wid;
+    if (item >= nitems) return;  // waves are independent: no block-wide barrier below
+    double* tbl = smem + wid * G::WAVE_D;
+    double* alph = tbl + TB::DOUBLES;
+    double* Wl = alph + 3 * TB::PQ_CAP;
+    double* Dab = Wl + S::NQ;
+    double* Dcd = Dab + S::NAB;
+    double* Dac = Dcd + NCD;
+    double* Dbd = Dac + NA * NC;
+    double* Dad = Dbd + NB * ND;
+    double* Dbc = Dad + NA * ND;
+    const size_t nbf = (size_t)nbf_;
+
+    const int4 it = items[item];
+    const PairRec P = bra[it.x];
+    const int pidx = P.pos;
+    for (int o = lane; o < S::NAB; o += 64) Dab[o] = D[(P.bfa + o / NB) * nbf + P.bfb + o % NB];
+    const double deg_p = (P.sa == P.sb) ? 0.5 : 1.0;
     double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0};
-    const double two_pi_52 = 2.0 * kPi * kPi * 1.7724538509055160273;
-    auto hidx = [](int i, int j, int k, int l) { return ((i * (EB + 1) + j) * (EC + 1) + k) * (ED + 1) + l; };
-    for (int q = tid; q < q1; q += blockDim.x) {
+
+    for (int q = it.y; q < it.z; ++q) {
         const PairRec Q = ket[q];
-        if (P.Q * Q.Q < thr) continue;
-        const double deg = ((P.sa == P.sb) ? 0.5 : 1.0) * ((Q.sa == Q.sb) ? 0.5 : 1.0) * ((diag && q == pidx) ? 0.5 : 1.0);
-        double W[S::NQ];
-        grad_weights<LA, LB, LC, LD>(P, Q, D, nbf, xfac, deg, W);
-        double gq[9];
-        grad_quartet<LA, LB, LC, LD>(P, Q, pp, rys_tab, W, gq);
-        // centre D = -(A + B + C)
+        const double deg = deg_p * ((Q.sa == Q.sb) ? 0.5 : 1.0) * ((diag && q == pidx) ? 0.5 : 1.0);
+        wave_sync();  // previous quartet's readers of the weights and density blocks are done
+        for (int o = lane; o < NCD; o += 64) Dcd[o] = D[(Q.bfa + o / ND) * nbf + Q.bfb + o % ND];
+        for (int o = lane; o < NA * NC; o += 64) Dac[o] = D[(P.bfa + o / NC) * nbf + Q.bfa + o % NC];
+        for (int o = lane; o < NB * ND; o += 64) Dbd[o] = D[(P.bfb + o / ND) * nbf + Q.bfb + o % ND];
+        for (int o = lane; o < NA * ND; o += 64) Dad[o] = D[(P.bfa + o / ND) * nbf + Q.bfb + o % ND];
+        for (int o = lane; o < NB * NC; o += 64) Dbc[o] = D[(P.bfb + o / NC) * nbf + Q.bfa + o % NC];
+        wave_sync();
+        for (int k = lane; k < S::NQ; k += 64) {
+            const int d = k % ND, c = (k / ND) % NC, b = (k / NCD) % NB, a = k / (NCD * NB);
+            Wl[k] = 4.0 * deg *
+                    (Dab[a * NB + b] * Dcd[c * ND + d] - xfac * (Dac[a * NC + c] * Dbd[b * ND + d] + Dad[a * ND + d] * Dbc[b * NC + c])) *
+                    kCompNorm[LA][a] * kCompNorm[LB][b] * kCompNorm[LC][c] * kCompNorm[LD][d];
+        }
+        double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // A xyz, B xyz, C xyz
+        const int npq = P.npp * Q.npp;
+        for (int pq0 = 0; pq0 < npq; pq0 += TB::PQ_CAP) {
+            const int npc = min(TB::PQ_CAP, npq - pq0);
+            wave_sync();
+            wave_tables<LA + 1, LB + 1, LC + 1, LD, N, ES, double>(P, Q, pp, rys_tab, pq0, npc, tbl, lane);
+            for (int u = lane; u < npc; u += 64) {
+                const int pq = pq0 + u, i = pq / Q.npp, j = pq - i * Q.npp;
+                alph[3 * u] = pp[P.pp_off + i].pad0;
+                alph[3 * u + 1] = pp[P.pp_off + i].pad1;
+                alph[3 * u + 2] = pp[Q.pp_off + j].pad0;
+            }
+            wave_sync();
+            const int ne = npc * N;
+#pragma unroll 1
+            for (int k = lane; k < S::NQ; k += 64) {
+                const int d = k % ND, c = (k / ND) % NC, b = (k / NCD) % NB, a = k / (NCD * NB);
+                const double w = Wl[k];
+                // per axis: row pointers of I(i,j,k,l), I(i+1..), I(i-1..), I(.,j+1,..), ... and the powers
+                const double* r0[3];
+                const double *rAp[3], *rAm[3], *rBp[3], *rBm[3], *rCp[3], *rCm[3];
+                double fi[3], fj[3], fk[3];
+#pragma unroll
+                for (int x = 0; x < 3; ++x) {
+                    const int ia = x == 0 ? cart_x(LA, a) : x == 1 ? cart_y(LA, a) : cart_z(LA, a);
+                    const int ib = x == 0 ? cart_x(LB, b) : x == 1 ? cart_y(LB, b) : cart_z(LB, b);
+                    const int ic = x == 0 ? cart_x(LC, c) : x == 1 ? cart_y(LC, c) : cart_z(LC, c);
+                    const int id = x == 0 ? cart_x(LD, d) : x == 1 ? cart_y(LD, d) : cart_z(LD, d);
+                    const int h0 = ((ia * TB1 + ib) * TC1 + ic) * TD1 + id;
+                    const double* base = tbl + x * HT * ES;
+                    r0[x] = base + h0 * ES;
+                    rAp[x] = base + (h0 + sI) * ES;
+                    rAm[x] = base + (ia ? h0 - sI : h0) * ES;
+                    rBp[x] = base + (h0 + sJ) * ES;
+                    rBm[x] = base + (ib ? h0 - sJ : h0) * ES;
+                    rCp[x] = base + (h0 + sK) * ES;
+                    rCm[x] = base + (ic ? h0 - sK : h0) * ES;
+                    fi[x] = ia;
+                    fj[x] = ib;
+                    fk[x] = ic;
+                }
+                double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                for (int e = 0; e < ne; ++e) {
+                    const int pql = e / N;
+                    const double two_a = 2.0 * alph[3 * pql], two_b = 2.0 * alph[3 * pql + 1],
+                                 two_c = 2.0 * alph[3 * pql + 2];
+                    const double i0x = r0[0][e], i0y = r0[1][e], i0z = r0[2][e];
+                    const double iyz = i0y * i0z, ixz = i0x * i0z, ixy = i0x * i0y;
+                    const double pr[3] = {iyz, ixz, ixy};
+#pragma unroll
+                    for (int x = 0; x < 3; ++x) {
+                        s[x] += (two_a * rAp[x][e] - fi[x] * rAm[x][e]) * pr[x];
+                        s[3 + x] += (two_b * rBp[x][e] - fj[x] * rBm[x][e]) * pr[x];
+                        s[6 + x] += (two_c * rCp[x][e] - fk[x] * rCm[x][e]) * pr[x];
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < 9; ++t) acc[t] += w * s[t];
+            }
+        }
+        // ket atoms: C directly, D = -(A + B + C) of this quartet (translational invariance)
 #pragma unroll
         for (int x = 0; x < 3; ++x) {
-            gA[x] += gq[x];
-            gB[x] += gq[3 + x];
-            atomicAdd(&gl[3 * Q.atom_a + x], gq[6 + x]);
-            atomicAdd(&gl[3 * Q.atom_b + x], -(gq[x] + gq[3 + x] + gq[6 + x]));
+            gA[x] += acc[x];
+            gB[x] += acc[3 + x];
+            const double cx = wave_sum(acc[6 + x]);
+            const double sx = wave_sum(acc[x] + acc[3 + x] + acc[6 + x]);
+            if (lane == 0) {
+                if (cx != 0.0) atomicAdd(&grad[3 * Q.atom_a + x], cx);
+                if (sx != 0.0) atomicAdd(&grad[3 * Q.atom_b + x], -sx);
+            }
         }
     }
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
-        double a = gA[x], b = gB[x];
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            a += __shfl_xor(a, off, 64);
-            b += __shfl_xor(b, off, 64);
-        }
-        if ((tid & 63) == 0) {
-            atomicAdd(&gl[3 * P.atom_a + x], a);
-            atomicAdd(&gl[3 * P.atom_b + x], b);
+        const double ax = wave_sum(gA[x]), bx = wave_sum(gB[x]);
+        if (lane == 0) {
+            if (ax != 0.0) atomicAdd(&grad[3 * P.atom_a + x], ax);
+            if (bx != 0.0) atomicAdd(&grad[3 * P.atom_b + x], bx);
         }
     }
-    __syncthreads();
-    for (int i = tid; i < 3 * natom; i += blockDim.x)
-        if (gl[i] != 0.0) atomicAdd(&grad[i], gl[i]);
 }
 
 #define QCG_LA(c) ((c) == 0 ? 0 : (c) <= 2 ? 1 : 2)
@@ -101,10 +185,10 @@ __global__ __launch_bounds__(256) void grad_kernel(const PairRec* __restrict__ b
 template <int CB, int CK>
 static hipError_t launch_grad_t(const GradLaunch& a, hipStream_t st) {
     constexpr int LA = QCG_LA(CB), LB = QCG_LB(CB), LC = QCG_LA(CK), LD = QCG_LB(CK);
-    if (a.nbra <= 0 || a.nket <= 0) return hipSuccess;
-    const size_t lds = 3 * (size_t)a.natom * sizeof(double);
-    hipLaunchKernelGGL((grad_kernel<LA, LB, LC, LD>), dim3(a.nbra), dim3(256), lds, st, a.bra, a.nbra, a.ket, a.nket,
-                       a.diag, a.pp, a.rys, a.D, a.nbf, a.xfac, a.thr, a.natom, a.grad);
+    using G = GradWave<LA, LB, LC, LD>;
+    if (a.nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL((grad_wave_kernel<LA, LB, LC, LD>), dim3((a.nitems + G::WPB - 1) / G::WPB), dim3(64 * G::WPB),
+                       0, st, a.items, a.nitems, a.bra, a.ket, a.diag, a.pp, a.rys, a.D, a.nbf, a.xfac, a.grad);
     return hipGetLastError();
 }
 

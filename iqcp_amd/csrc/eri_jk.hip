@@ -20,6 +20,7 @@
 // build_jk (dft/ks_scf.rs:1369-1394).
 #include "eri_device.h"
 #include "eri_launch.h"
+#include "eri_wave.h"
 
 #ifndef QCG_CB
 #error "compile with -DQCG_CB=<bra pair class>"
@@ -167,6 +168,155 @@ __global__ __launch_bounds__(256) void jk_kernel(
     }
 }
 
+// Wave-cooperative J/K for the classes with many components (eri_wave.h): four wavefronts per
+// block, each takes one work item (bra pair, ket range) and walks its kets one quartet at a time.
+// Digestion of a contracted quartet (phase C): the quartet v (norms and symmetry factor applied)
+// goes to LDS, lanes over output elements form
+//   J_ab += 2 sum_cd D_cd v   (kept in a register of lane ab until the item ends),
+//   J_cd += 2 sum_ab D_ab v,  K_ac += sum_bd D_bd v,  K_bc += sum_ad D_ad v,
+//   K_ad += sum_bc D_bc v,    K_bd += sum_ac D_ac v   (FP64 atomics per quartet).
+template <int LA, int LB, int LC, int LD, typename R>
+struct JkWave {
+    using S = EriShape<LA, LB, LC, LD>;
+    using T = WaveTable<LA, LB, LC, LD, S::NROOTS, R>;
+    static constexpr int VD = T::DOUBLES > S::NQ ? T::DOUBLES : S::NQ;  // table, then the quartet
+    static constexpr int DB = S::NAB + S::NCD + S::NB * S::ND + S::NA * S::ND + S::NB * S::NC + S::NA * S::NC;
+    static constexpr int WAVE_D = VD + DB;
+    static constexpr int KPL = (S::NQ + 63) / 64;
+    static constexpr bool kEnabled = S::NQ >= 27;
+};
+
+template <int LA, int LB, int LC, int LD, typename R>
+__global__ __launch_bounds__(256) void jk_wave_kernel(
+    const int4* __restrict__ items, int nitems, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket,
+    int diag, const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_tab, const double* __restrict__ D,
+    double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
+    const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf) {
+    using S = EriShape<LA, LB, LC, LD>;
+    using W = JkWave<LA, LB, LC, LD, R>;
+    using TB = typename W::T;
+    constexpr int N = S::NROOTS, HT = TB::HT, ES = TB::ES, KPL = W::KPL;
+    constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND, NCD = S::NCD;
+    __shared__ double smem[4 * W::WAVE_D];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int item = blockIdx.x * 4 + wid;
+    if (item >= nitems) return;  // no block-wide barriers below: waves are independent
+    double* V = smem + wid * W::WAVE_D;
+    R* tbl = reinterpret_cast<R*>(V);
+    double* Dab = V + W::VD;
+    double* Dcd = Dab + S::NAB;
+    double* Dbd = Dcd + NCD;
+    double* Dad = Dbd + NB * ND;
+    double* Dbc = Dad + NA * ND;
+    double* Dac = Dbc + NB * NC;
+
+    const int4 it = items[item];
+    const PairRec P = bra[it.x];
+    const int pidx = P.pos;
+    ket += ket_off[P.mol];
+    D += mat_off[P.mol];
+    Jacc += mat_off[P.mol];
+    Kacc += mat_off[P.mol];
+    const size_t nbf = (size_t)mol_nbf[P.mol];
+
+    constexpr const CompTab<LA, LB, LC, LD>& ct = kCompTab<LA, LB, LC, LD>;
+    int ox[KPL], oy[KPL], oz[KPL];
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) {
+        const int k = lane + 64 * j;
+        const int kk = k < S::NQ ? k : 0;
+        ox[j] = ct.hx[kk] * ES;
+        oy[j] = (HT + ct.hy[kk]) * ES;
+        oz[j] = (2 * HT + ct.hz[kk]) * ES;
+    }
+    for (int o = lane; o < S::NAB; o += 64) Dab[o] = D[(P.bfa + o / NB) * nbf + P.bfb + o % NB];
+    double jp = 0.0;  // J_ab of lane ab over the item's kets
+    const double deg_p = (P.sa == P.sb) ? 0.5 : 1.0;
+
+    for (int q = it.y; q < it.z; ++q) {
+        const PairRec Q = ket[q];
+        R acc[KPL];
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) acc[j] = R(0);
+        const int npq = P.npp * Q.npp;
+        for (int pq0 = 0; pq0 < npq; pq0 += TB::PQ_CAP) {
+            const int npc = min(TB::PQ_CAP, npq - pq0);
+            wave_sync();  // earlier readers of the table (and of V, which aliases it) are done
+            wave_tables<LA, LB, LC, LD, N, ES, R>(P, Q, pp, rys_tab, pq0, npc, tbl, lane);
+            wave_sync();
+            const int ne = npc * N;
+            for (int e = 0; e < ne; ++e) {
+#pragma unroll
+                for (int j = 0; j < KPL; ++j) acc[j] += tbl[ox[j] + e] * tbl[oy[j] + e] * tbl[oz[j] + e];
+            }
+        }
+        wave_sync();
+        const double deg = deg_p * ((Q.sa == Q.sb) ? 0.5 : 1.0) * ((diag && q == pidx) ? 0.5 : 1.0);
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) {
+            const int k = lane + 64 * j;
+            if (k < S::NQ) {
+                const int d = k % ND, c = (k / ND) % NC, b = (k / NCD) % NB, a = k / (NCD * NB);
+                V[k] = (double)acc[j] * (kCompNorm[LA][a] * kCompNorm[LB][b] * kCompNorm[LC][c] * kCompNorm[LD][d] * deg);
+            }
+        }
+        if (do_j)
+            for (int o = lane; o < NCD; o += 64) Dcd[o] = D[(Q.bfa + o / ND) * nbf + Q.bfb + o % ND];
+        if (do_k) {
+            for (int o = lane; o < NB * ND; o += 64) Dbd[o] = D[(P.bfb + o / ND) * nbf + Q.bfb + o % ND];
+            for (int o = lane; o < NA * ND; o += 64) Dad[o] = D[(P.bfa + o / ND) * nbf + Q.bfb + o % ND];
+            for (int o = lane; o < NB * NC; o += 64) Dbc[o] = D[(P.bfb + o / NC) * nbf + Q.bfa + o % NC];
+            for (int o = lane; o < NA * NC; o += 64) Dac[o] = D[(P.bfa + o / NC) * nbf + Q.bfa + o % NC];
+        }
+        wave_sync();
+#define VAT(a, b, c, d) V[(((a) * NB + (b)) * NC + (c)) * ND + (d)]
+        if (do_j) {
+            if (lane < S::NAB) {
+                double s = 0.0;
+                for (int cd = 0; cd < NCD; ++cd) s += Dcd[cd] * V[lane * NCD + cd];
+                jp += 2.0 * s;
+            }
+            for (int o = lane; o < NCD; o += 64) {
+                double s = 0.0;
+                for (int ab = 0; ab < S::NAB; ++ab) s += Dab[ab] * V[ab * NCD + o];
+                atomicAdd(&Jacc[(Q.bfa + o / ND) * nbf + Q.bfb + o % ND], 2.0 * s);
+            }
+        }
+        if (do_k) {
+            for (int o = lane; o < NA * NC; o += 64) {
+                const int a = o / NC, c = o % NC;
+                double s = 0.0;
+                for (int b = 0; b < NB; ++b)
+                    for (int d = 0; d < ND; ++d) s += Dbd[b * ND + d] * VAT(a, b, c, d);
+                atomicAdd(&Kacc[(P.bfa + a) * nbf + Q.bfa + c], s);
+            }
+            for (int o = lane; o < NB * NC; o += 64) {
+                const int b = o / NC, c = o % NC;
+                double s = 0.0;
+                for (int a = 0; a < NA; ++a)
+                    for (int d = 0; d < ND; ++d) s += Dad[a * ND + d] * VAT(a, b, c, d);
+                atomicAdd(&Kacc[(P.bfb + b) * nbf + Q.bfa + c], s);
+            }
+            for (int o = lane; o < NA * ND; o += 64) {
+                const int a = o / ND, d = o % ND;
+                double s = 0.0;
+                for (int b = 0; b < NB; ++b)
+                    for (int c = 0; c < NC; ++c) s += Dbc[b * NC + c] * VAT(a, b, c, d);
+                atomicAdd(&Kacc[(P.bfa + a) * nbf + Q.bfb + d], s);
+            }
+            for (int o = lane; o < NB * ND; o += 64) {
+                const int b = o / ND, d = o % ND;
+                double s = 0.0;
+                for (int a = 0; a < NA; ++a)
+                    for (int c = 0; c < NC; ++c) s += Dac[a * NC + c] * VAT(a, b, c, d);
+                atomicAdd(&Kacc[(P.bfb + b) * nbf + Q.bfb + d], s);
+            }
+        }
+#undef VAT
+    }
+    if (do_j && lane < S::NAB && jp != 0.0) atomicAdd(&Jacc[(P.bfa + lane / NB) * nbf + P.bfb + lane % NB], jp);
+}
+
 __device__ __forceinline__ int64_t pair_index64(int64_t i, int64_t j) {
     return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
 }
@@ -215,7 +365,15 @@ template <int CB, int CK, typename R>
 static hipError_t launch_jk_t(const JkLaunch& a, hipStream_t st) {
     constexpr int LA = QCG_LA(CB), LB = QCG_LB(CB), LC = QCG_LA(CK), LD = QCG_LB(CK);
     if (a.nitems <= 0) return hipSuccess;
-    hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, R>), dim3(a.nitems), dim3(a.block), 0, st, a.items, a.bra, a.ket,
+    if (a.block == 0) {  // wave-cooperative mode: four items per block
+        if constexpr (JkWave<LA, LB, LC, LD, R>::kEnabled) {
+            hipLaunchKernelGGL((jk_wave_kernel<LA, LB, LC, LD, R>), dim3((a.nitems + 3) / 4), dim3(256), 0, st,
+                               a.items, a.nitems, a.bra, a.ket, a.diag, (const PrimPair<R>*)a.pp, (const R*)a.rys,
+                               a.D, a.Jacc, a.Kacc, a.do_j, a.do_k, a.ket_off, a.mat_off, a.mol_nbf);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, R>), dim3(a.nitems), dim3(a.block > 0 ? a.block : 64), 0, st, a.items, a.bra, a.ket,
                        a.diag, (const PrimPair<R>*)a.pp, (const R*)a.rys, a.D, a.Jacc, a.Kacc, a.do_j, a.do_k,
                        a.ket_off, a.mat_off, a.mol_nbf);
     return hipGetLastError();

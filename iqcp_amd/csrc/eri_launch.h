@@ -84,6 +84,9 @@ struct GradLaunch {
     double thr;
     int natom;
     double* grad;  // 3 * natom, accumulated
+    // work items (index in the bra class list, ket begin, ket end, 0): one wavefront each
+    const int4* items;
+    int nitems;
 };
 
 hipError_t launch_grad_cb0(int ck, const GradLaunch& a, hipStream_t st);

@@ -1,6 +1,8 @@
 // Host side of the analytic nuclear gradient: the adjoint ERI kernels on the device plus the
 // one-electron and nuclear-repulsion terms on the host (rhf_gradient, scf/gradient.rs:900-976).
+#include <algorithm>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "context.h"
@@ -9,9 +11,38 @@ using namespace qcg;
 
 namespace {
 
+// Work items of the gradient kernels: every bra pair's surviving kets (exact screening predicate
+// Q_P Q_Q >= threshold on the Q-sorted lists, as for J/K) cut into ranges of kKets.
+constexpr int kGradKets = 4;
+
 int eri_grad(qcg_context* c, const double* dD, double hf_frac, double* dgrad, hipStream_t st) {
+    std::vector<int4> items;
+    int off[21], cnt[21];
+    int cls = 0;
     for (int cb = kNumPairClasses - 1; cb >= 0; --cb)
-        for (int ck = cb; ck >= 0; --ck) {
+        for (int ck = cb; ck >= 0; --ck, ++cls) {
+            off[cls] = (int)items.size();
+            const int nb = c->class_off[cb + 1] - c->class_off[cb], nk = c->class_off[ck + 1] - c->class_off[ck];
+            for (int p = 0; p < nb; ++p) {
+                const double Qp = c->pairs[c->class_off[cb] + p].Q;
+                int lo = 0, hi = (cb == ck) ? p + 1 : nk;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (Qp * c->pairs[c->class_off[ck] + mid].Q >= c->schwarz_thr) lo = mid + 1;
+                    else hi = mid;
+                }
+                for (int q0 = 0; q0 < lo; q0 += kGradKets) items.push_back(make_int4(p, q0, std::min(q0 + kGradKets, lo), 0));
+            }
+            cnt[cls] = (int)items.size() - off[cls];
+        }
+    if (items.empty()) return QCG_OK;
+    int4* ditems = nullptr;
+    HIPCHK(c, c->alloc(&ditems, items.size()));
+    HIPCHK(c, hipMemcpyAsync(ditems, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, st));
+    cls = 0;
+    int rc = QCG_OK;
+    for (int cb = kNumPairClasses - 1; cb >= 0 && rc == QCG_OK; --cb)
+        for (int ck = cb; ck >= 0; --ck, ++cls) {
             GradLaunch a{};
             a.bra = c->d_pairs + c->class_off[cb];
             a.nbra = c->class_off[cb + 1] - c->class_off[cb];
@@ -26,8 +57,19 @@ int eri_grad(qcg_context* c, const double* dD, double hf_frac, double* dgrad, hi
             a.thr = c->schwarz_thr;
             a.natom = c->basis.natom;
             a.grad = dgrad;
-            HIPCHK(c, launch_grad(cb, ck, a, st));
+            a.items = ditems + off[cls];
+            a.nitems = cnt[cls];
+            const hipError_t e = launch_grad(cb, ck, a, st);
+            if (e != hipSuccess) {
+                rc = fail(c, QCG_DEVICE, std::string("launch_grad: ") + hipGetErrorString(e));
+                break;
+            }
         }
+    // the item buffer is released only after the kernels that read it have finished
+    const hipError_t es = hipStreamSynchronize(st);
+    c->release(ditems);
+    if (rc) return rc;
+    HIPCHK(c, es);
     return QCG_OK;
 }
 
@@ -38,6 +80,7 @@ extern "C" {
 int qcg_eri_gradient(qcg_context* c, const double* D, double hf_frac, double* grad) {
     if (!c || !D || !grad) return QCG_INVALID_ARG;
     if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    if (c->nmol != 1) return fail(c, QCG_INVALID_ARG, "gradient: one molecule per context");
     HIPCHK(c, hipSetDevice(c->device));
     const int nbf = c->basis.nbf, n3 = 3 * c->basis.natom;
     const size_t n2 = (size_t)nbf * nbf;

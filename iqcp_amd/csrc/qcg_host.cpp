@@ -55,8 +55,9 @@ int qcg_create(qcg_context** out, int device) {
             const int la = cb == 0 ? 0 : cb <= 2 ? 1 : 2, lb = (cb == 0 || cb == 1 || cb == 3) ? 0 : (cb == 2 || cb == 4) ? 1 : 2;
             const int lc = ck == 0 ? 0 : ck <= 2 ? 1 : 2, ld = (ck == 0 || ck == 1 || ck == 3) ? 0 : (ck == 2 || ck == 4) ? 1 : 2;
             const int nq = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
-            c->tile[cls].block = 64;
-            c->tile[cls].reps = nq <= 3 ? 4 : nq <= 9 ? 2 : 1;
+            c->tile[cls].block = nq >= 81 ? 0 : 64;  // block 0: wave-cooperative kernel
+            c->tile[cls].reps = nq <= 3 ? 4 : nq <= 9 ? 2 : nq >= 81 ? 2 : 1;
+            c->class_nq[cls] = nq;
         }
     *out = c.release();
     return QCG_OK;
@@ -346,7 +347,7 @@ static int build_plan(qcg_context* c) {
     int cls = 0;
     for (int cb = kNumPairClasses - 1; cb >= 0; --cb)
         for (int ck = cb; ck >= 0; --ck, ++cls) {
-            const int ch = std::max(1, c->tile[cls].block * c->tile[cls].reps);
+            const int ch = std::max(1, c->tile[cls].block > 0 ? c->tile[cls].block * c->tile[cls].reps : c->tile[cls].reps);
             for (int part = 0; part < 2; ++part) {
                 c->item_off[cls][part] = (int)c->items.size();
                 if (part == 1 && !mixed) {
@@ -379,11 +380,24 @@ static int build_plan(qcg_context* c) {
     return QCG_OK;
 }
 
+int qcg::prepare_jk(qcg_context* c) {
+    int rc = build_plan(c);
+    if (rc) return rc;
+    if (!c->ev_fork) {
+        HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        for (int i = 0; i < qcg_context::kStreams; ++i) {
+            HIPCHK(c, hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking));
+            HIPCHK(c, hipEventCreateWithFlags(&c->ev_join[i], hipEventDisableTiming));
+        }
+    }
+    return QCG_OK;
+}
+
 int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStream_t st) {
     const int nbf = c->max_nbf;
     const size_t n2 = (size_t)c->mat_total;
     const int do_j = dJ != nullptr, do_k = dK != nullptr;
-    int rc = build_plan(c);
+    int rc = prepare_jk(c);
     if (rc) return rc;
     if (do_j) HIPCHK(c, launch_fill(c->d_Jacc, n2, 0.0, st));
     if (do_k) HIPCHK(c, launch_fill(c->d_Kacc, n2, 0.0, st));
@@ -396,13 +410,6 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
     // in profiling mode everything runs on st so the events bracket single kernels
     const bool fork = !c->profiling;
     if (fork) {
-        if (!c->ev_fork) {
-            HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-            for (int i = 0; i < qcg_context::kStreams; ++i) {
-                HIPCHK(c, hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking));
-                HIPCHK(c, hipEventCreateWithFlags(&c->ev_join[i], hipEventDisableTiming));
-            }
-        }
         HIPCHK(c, hipEventRecord(c->ev_fork, st));
         for (int i = 0; i < qcg_context::kStreams; ++i) HIPCHK(c, hipStreamWaitEvent(c->aux[i], c->ev_fork, 0));
     }
@@ -457,7 +464,9 @@ int qcg_autotune(qcg_context* c, const double* dD, int iters, int* chosen) {
     if (!c || !dD) return QCG_INVALID_ARG;
     if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
     HIPCHK(c, hipSetDevice(c->device));
-    const int cand[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {256, 1}, {64, 8}};
+    // {threads per block, kets per thread}; block 0 = one wave per work item of `reps` kets
+    // (wave-cooperative kernel, classes with at least 27 components only)
+    const int cand[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {256, 1}, {64, 8}, {0, 1}, {0, 2}, {0, 4}};
     const int ncand = sizeof(cand) / sizeof(cand[0]);
     std::vector<double> best(21, 1e300);
     std::vector<int> pick(21, 0);
@@ -465,7 +474,9 @@ int qcg_autotune(qcg_context* c, const double* dD, int iters, int* chosen) {
     c->profiling = true;
     iters = std::max(1, iters);
     for (int k = 0; k < ncand; ++k) {
-        for (int i = 0; i < 21; ++i) c->tile[i] = {cand[k][0], cand[k][1]};
+        for (int i = 0; i < 21; ++i)
+            c->tile[i] = (cand[k][0] == 0 && c->class_nq[i] < 27) ? qcg_context::Tile{64, 1}
+                                                                  : qcg_context::Tile{cand[k][0], cand[k][1]};
         c->tile_sig += 1;
         std::vector<double> tot(21, 0.0);
         for (int it = 0; it < iters + 1; ++it) {
@@ -485,11 +496,32 @@ int qcg_autotune(qcg_context* c, const double* dD, int iters, int* chosen) {
             if (tot[i] < best[i]) { best[i] = tot[i]; pick[i] = k; }
     }
     for (int i = 0; i < 21; ++i) {
-        c->tile[i] = {cand[pick[i]][0], cand[pick[i]][1]};
+        c->tile[i] = (cand[pick[i]][0] == 0 && c->class_nq[i] < 27) ? qcg_context::Tile{64, 1}
+                                                                    : qcg_context::Tile{cand[pick[i]][0], cand[pick[i]][1]};
         if (chosen) { chosen[2 * i] = c->tile[i].block; chosen[2 * i + 1] = c->tile[i].reps; }
     }
     c->tile_sig += 1;
     c->profiling = prof;
+    return QCG_OK;
+}
+
+int qcg_set_tiles(qcg_context* c, const int* tiles) {
+    if (!c || !tiles) return QCG_INVALID_ARG;
+    for (int i = 0; i < 21; ++i)
+        if (tiles[2 * i] < 0 || tiles[2 * i] > 1024 || (tiles[2 * i] % 64) != 0 || tiles[2 * i + 1] < 1 ||
+            tiles[2 * i + 1] > 64)
+            return fail(c, QCG_INVALID_ARG, "tile: block must be 0 or a multiple of 64 up to 1024, kets 1..64");
+    for (int i = 0; i < 21; ++i) c->tile[i] = {tiles[2 * i], tiles[2 * i + 1]};
+    c->tile_sig += 1;
+    return QCG_OK;
+}
+
+int qcg_get_tiles(qcg_context* c, int* tiles) {
+    if (!c || !tiles) return QCG_INVALID_ARG;
+    for (int i = 0; i < 21; ++i) {
+        tiles[2 * i] = c->tile[i].block;
+        tiles[2 * i + 1] = c->tile[i].reps;
+    }
     return QCG_OK;
 }
 

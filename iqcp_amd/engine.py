@@ -38,6 +38,7 @@ EXPORTS = [
     "qcg_build_grid", "qcg_set_grid", "qcg_get_grid", "qcg_xc_eval", "qcg_xc", "qcg_xc_device", "qcg_ks_scf",
     "qcg_eri_gradient", "qcg_one_electron_gradient", "qcg_rhf_gradient",
     "qcg_set_basis_batch", "qcg_batch_build_jk", "qcg_batch_rhf_scf", "qcg_autotune",
+    "qcg_set_tiles", "qcg_get_tiles",
 ]
 
 
@@ -224,6 +225,19 @@ class Engine:
         ch = (ctypes.c_int * 42)()
         self._check(self.L.qcg_autotune(self.ctx, ctypes.c_void_p(dD_ptr), int(iters), ch))
         return [(ch[2 * i], ch[2 * i + 1]) for i in range(21)]
+
+    def set_tiles(self, tiles):
+        """Set (block, kets) per quartet class; block 0 selects the wave-cooperative kernel."""
+        t = list(tiles)
+        if len(t) != 21:
+            raise ValueError("21 (block, kets) tiles expected")
+        arr = (ctypes.c_int * 42)(*[int(v) for bk in t for v in bk])
+        self._check(self.L.qcg_set_tiles(self.ctx, arr))
+
+    def get_tiles(self):
+        arr = (ctypes.c_int * 42)()
+        self._check(self.L.qcg_get_tiles(self.ctx, arr))
+        return [(arr[2 * i], arr[2 * i + 1]) for i in range(21)]
 
     def set_profiling(self, on: bool):
         self._check(self.L.qcg_set_profiling(self.ctx, int(bool(on))))

@@ -151,6 +151,36 @@ def test_gpu_jk_propane_vs_oracle_and_mixed_bound():
     assert np.abs(jm.K - Ko).max() < 1e-8
 
 
+@pytest.mark.parametrize("tile", [(64, 1), (128, 2), (0, 1), (0, 3)])
+def test_gpu_jk_tiles_vs_oracle(tile):
+    """Every quartet class through one launch configuration: per-lane kernel with several kets per
+    thread, or (block 0) the wave-cooperative kernel for every class with >= 27 components.  NH3
+    6-31+G* has every class (dd|dd) .. (ss|ss) with contracted s/p shells."""
+    b = bench_molecule("NH3", "6-31+g*")
+    eri, _ = oracle.eri_compressed(b)
+    eng = _eng(b)
+    eng.set_tiles([tile] * 21)
+    assert eng.get_tiles() == [tile] * 21
+    D = random_density(b.nbf, seed=7)
+    Jo, Ko = oracle.build_jk(D, eri, b.nbf)
+    jk = eng.build_jk(D)
+    np.testing.assert_allclose(jk.J, Jo, atol=1e-11, rtol=0)
+    np.testing.assert_allclose(jk.K, Ko, atol=1e-11, rtol=0)
+    eng.set_options("mixed", 1e-6)
+    jm = eng.build_jk(D)
+    assert np.abs(jm.J - Jo).max() < 1e-8
+    assert np.abs(jm.K - Ko).max() < 1e-8
+
+
+def test_gpu_set_tiles_rejects_bad_tiles():
+    b = bench_molecule("H2O", "sto-3g")
+    eng = _eng(b)
+    with pytest.raises(Exception):
+        eng.set_tiles([(32, 1)] * 21)
+    with pytest.raises(Exception):
+        eng.set_tiles([(64, 0)] * 21)
+
+
 # ------------------------------------------------------------------ SCF energies vs IQCP
 @pytest.mark.parametrize("mol,bas", [("H2O", "sto-3g"), ("H2O", "6-31g*"), ("CH4", "6-31g*"),
                                      ("NH3", "6-31+g*"), ("HF", "6-31g"), ("C6H6", "sto-3g")])
