@@ -35,7 +35,22 @@ def main():
     ap.add_argument("--fp32-cutoff", type=float, default=1e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default tiles instead of autotuning (profiling runs)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (5 times): the reference kernel whose "
+                         "FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic figures")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = 1 << 27  # 2^27 doubles = 1 GiB
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read_per_launch": n * 8, "bytes_written_per_launch": n * 8, "launches": 5}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the Fock build runs on a single MI355X; sharding the "
               "quartet work items over several GPUs is designed but not implemented.", file=sys.stderr)
@@ -67,7 +82,8 @@ def main():
 
     def timed(prec):
         eng.set_options(prec, args.fp32_cutoff)
-        tiles[prec] = eng.autotune(dD.data_ptr(), 2)  # per-class tile choice for this precision mode
+        # per-class tile choice for this precision mode
+        tiles[prec] = eng.get_tiles() if args.no_autotune else eng.autotune(dD.data_ptr(), 2)
         with torch.cuda.stream(stream):
             for _ in range(args.warmup):
                 eng.build_fock_device(dH.data_ptr(), dD.data_ptr(), dF.data_ptr(), 0.5, stream.cuda_stream)
@@ -89,7 +105,7 @@ def main():
     per_kernel = {}
     for prec in ("fp64", "mixed"):
         eng.set_options(prec, args.fp32_cutoff)
-        eng.autotune(dD.data_ptr(), 2)
+        eng.set_tiles(tiles[prec])
         eng.set_profiling(True)
         acc = None
         with torch.cuda.stream(stream):
@@ -140,7 +156,7 @@ def main():
         nthr = min(16, os.cpu_count() or 1)
         n_done, t = 0, time.perf_counter()
         Fc = None
-        while n_done < 5 and (n_done == 0 or time.perf_counter() - t < args.cpu_seconds):
+        while n_done == 0 or time.perf_counter() - t < args.cpu_seconds:
             Fc = oracle.build_fock_omp(H, D, eri, b.nbf, nthr)
             n_done += 1
         el = time.perf_counter() - t

@@ -39,6 +39,10 @@ struct qcg_context {
     std::vector<int> mol_nbf_h;
     std::vector<int64_t> mat_off_h;          // offset of each molecule's nbf^2 block
     std::vector<int> ket_off_h, ket_cnt_h;   // [class][mol], relative to the class start
+    // contraction buckets of each (class, molecule) list: pairs are ordered by surviving
+    // primitive-pair count (descending), then Schwarz bound (descending); bucket_bnd[class][mol]
+    // holds the bucket boundaries 0 = b_0 < b_1 < ... < b_n = count (positions in the list)
+    std::vector<std::vector<int>> bucket_bnd;
     int* d_mol_nbf = nullptr;
     int64_t* d_mat_off = nullptr;
     int* d_ket_off = nullptr;
@@ -127,6 +131,42 @@ int run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStream_t
 // streams, fork/join events), done ahead so that run_jk itself can be captured into a graph.
 int prepare_jk(qcg_context* c);
 double radial_norm(double a, int l);
+
+// First position in [lo, hi) of the list starting at pairs[base] with Qp * Q < thr (Q descending
+// inside [lo, hi)).
+inline int prefix_end(const qcg_context* c, int base, int lo, int hi, double Qp, double thr) {
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (Qp * c->pairs[base + mid].Q >= thr) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// The surviving kets of bra pair p (position in molecule m's class-cb list) in molecule m's
+// class-ck list, one call f(begin, split, end) per contraction bucket: [begin, end) are the kets
+// with Q_P Q_Q >= thr (the reference's screen, integrals/eri/mod.rs:1008-1023), [begin, split)
+// those with Q_P Q_Q >= cut (cut <= thr: split == end).  For cb == ck only kets at positions <= p
+// (each unordered pair of pairs once).  Positions are relative to the molecule's class-ck list.
+template <typename F>
+void for_ket_ranges(const qcg_context* c, int cb, int ck, int m, int p, double thr, double cut, F&& f) {
+    const int nmol = c->nmol;
+    const int base = c->class_off[ck] + c->ket_off_h[ck * nmol + m];
+    const double Qp = c->pairs[c->class_off[cb] + c->ket_off_h[cb * nmol + m] + p].Q;
+    const std::vector<int>& bnd = c->bucket_bnd[ck * nmol + m];
+    for (size_t b = 0; b + 1 < bnd.size(); ++b) {
+        const int bs = bnd[b];
+        int be = bnd[b + 1];
+        if (cb == ck) {
+            if (bs > p) break;
+            be = std::min(be, p + 1);
+        }
+        const int e = prefix_end(c, base, bs, be, Qp, thr);
+        if (e == bs) continue;
+        const int sp = cut > thr ? prefix_end(c, base, bs, e, Qp, cut) : e;
+        f(bs, sp, e);
+    }
+}
 
 #define HIPCHK(ctx, expr)                                                                        \
     do {                                                                                         \

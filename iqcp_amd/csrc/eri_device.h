@@ -143,8 +143,9 @@ QCG_HD void for_n(F&& body) {
 
 // Accumulate the contracted quartet (bra pair P | ket pair Q) into eri[NQ] (row-major
 // [a][b][c][d] over Cartesian components; the primitive-dependent normalisation is folded
-// into ck, the component normalisation is NOT applied here).
-template <int LA, int LB, int LC, int LD, typename R>
+// into ck, the component normalisation is NOT applied here).  TAB_N: rys_tab already points at
+// the coefficients of this class's Rys order (e.g. a copy staged in LDS).
+template <int LA, int LB, int LC, int LD, typename R, bool TAB_N = false>
 QCG_HD void contract_quartet(const PairRec& P, const PairRec& Q,
                                                  const PrimPair<R>* __restrict__ pp,
                                                  const R* __restrict__ rys_tab, R* eri) {
@@ -170,7 +171,8 @@ QCG_HD void contract_quartet(const PairRec& P, const PairRec& Q,
             const R T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
             const R pref = two_pi_52 / (p * q * sqrt(pq_sum)) * a.ck * c.ck;
             R rx[N], rw[N];
-            rys_eval<R, N>(T, rys_tab, rx, rw);
+            if constexpr (TAB_N) rys_eval_tab<R, N>(T, rys_tab, rx, rw);
+            else rys_eval<R, N>(T, rys_tab, rx, rw);
             const R hq = R(0.5) / q;
             const R qf = q * ipq, pf = p * ipq;
             const R PAx = a.px - Ax, PAy = a.py - Ay, PAz = a.pz - Az;

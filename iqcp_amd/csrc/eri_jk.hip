@@ -3,17 +3,21 @@
 // in parallel).  Pair class c <-> (la, lb): 0 (s,s) 1 (p,s) 2 (p,p) 3 (d,s) 4 (d,p) 5 (d,d).
 //
 // Work decomposition of the J/K build (one launch per quartet class, bra class >= ket class):
-//   * pairs of a class are sorted by Schwarz bound (descending); for a bra pair P the surviving
-//     kets of a ket class are the prefix with Q_P*Q_Q >= threshold (monotone product), found by
-//     binary search on exactly the product the reference tests (q_ij * q_kl < SCHWARZ_THRESHOLD,
-//     integrals/eri/mod.rs:1008-1023); for bra class == ket class only kets at or before P
-//     (each unordered pair-pair once = 8-fold symmetry); in mixed precision the prefix splits
-//     into an FP64 range (Q_P*Q_Q >= cutoff) and an FP32 range;
+//   * pairs of a class are ordered by contraction bucket (number of primitive pairs), then by
+//     Schwarz bound (descending); for a bra pair P the surviving kets of a bucket are the prefix
+//     with Q_P*Q_Q >= threshold (monotone product), found by binary search on exactly the product
+//     the reference tests (q_ij * q_kl < SCHWARZ_THRESHOLD, integrals/eri/mod.rs:1008-1023); for
+//     bra class == ket class only kets at or before P (each unordered pair-pair once = 8-fold
+//     symmetry); in mixed precision each prefix splits into an FP64 range (Q_P*Q_Q >= cutoff) and
+//     an FP32 range;
 //   * the host cuts every range into work items (bra pair, ket sub-range) of a per-class size;
 //     one block per item, one thread per ket (several for cheap classes), so every class
 //     launch has many more blocks than the GPU has CUs;
-//   * digestion: J_P in registers + wave reduction, J_Q and the four K terms by FP64 atomics
-//     (all land in L2).  J = Jacc + Jacc^T, K = Kacc + Kacc^T after.
+//   * digestion: J_P in registers + wave reduction; J_Q by FP64 atomics; the four K terms all
+//     land in the rows of the bra functions, so they are summed in LDS rows (one block = one bra
+//     pair) and flushed with one global atomic per touched element at the end of the block (device
+//     atomics are performed memory-side on this GPU, so every global atomic is HBM write
+//     traffic).  J = Jacc + Jacc^T, K = Kacc + Kacc^T after.
 //   * symmetry factors: 1/2 for a == b shells, 1/2 for c == d shells, 1/2 for P == Q, folded
 //     together with the component normalisation into the density sub-blocks.
 // With the symmetric density this reproduces build_fock's G = J - K/2 (scf/mod.rs:825-881) and
@@ -28,17 +32,6 @@
 
 namespace qcg {
 
-// first index q in [0, qmax) with Qp * ket[q].Q < thr (ket sorted by Q descending)
-__device__ __forceinline__ int first_below(const PairRec* __restrict__ ket, int qmax, double Qp, double thr) {
-    int lo = 0, hi = qmax;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (Qp * ket[mid].Q >= thr) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
 // One work item = (bra pair, contiguous range of its surviving kets); the host builds the items
 // with the exact screening predicate, so no quartet is tested twice or skipped.
 template <int LA, int LB, int LC, int LD, typename R>
@@ -46,9 +39,20 @@ __global__ __launch_bounds__(256) void jk_kernel(
     const int4* __restrict__ items, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket, int diag,
     const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_tab, const double* __restrict__ D,
     double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
-    const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf) {
+    const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf, int krow) {
     using S = EriShape<LA, LB, LC, LD>;
     constexpr bool U = S::kUnroll;
+    // krow != 0: the four K terms of the item (all in the rows of the bra functions a and b) are
+    // summed in LDS rows kl[a][*], kl[NA + b][*] and flushed once per block
+    extern __shared__ double kl[];
+    // the class's Rys coefficients staged in LDS (node/weight fits are gathered per primitive
+    // quartet at lane-dependent intervals: LDS serves those gathers far faster than L1)
+    constexpr int NR = S::NROOTS;
+    constexpr bool kRysLds = NR <= 3;
+    constexpr int RT = kRysLds ? rys_tab_size<NR>() : 1;
+    __shared__ R rys_s[RT];
+    if constexpr (kRysLds)
+        for (int i = threadIdx.x; i < RT; i += blockDim.x) rys_s[i] = rys_tab[qcg_rys_off[NR] + i];
     const int4 it = items[blockIdx.x];
     const PairRec P = bra[it.x];
     const int pidx = P.pos;
@@ -58,6 +62,13 @@ __global__ __launch_bounds__(256) void jk_kernel(
     Kacc += mat_off[P.mol];
     const int nbf = mol_nbf[P.mol];
     const int tid = threadIdx.x;
+    const int nrow = krow ? (S::NA + S::NB) * nbf : 0;
+    for (int i = tid; i < nrow; i += blockDim.x) kl[i] = 0.0;
+    __syncthreads();
+    auto kadd = [&](bool row_b, int r, int col, double v) {
+        if (krow) atomicAdd(&kl[(row_b ? S::NA + r : r) * nbf + col], v);
+        else atomicAdd(&Kacc[(size_t)((row_b ? P.bfb : P.bfa) + r) * nbf + col], v);
+    };
 
     // bra density block scaled by the bra component norms
     double Dps[S::NAB], jp[S::NAB];
@@ -72,7 +83,8 @@ __global__ __launch_bounds__(256) void jk_kernel(
         const PairRec Q = ket[q];
         R eri[S::NQ];
         for_n<U, S::NQ>([&](int i) { eri[i] = R(0); });
-        contract_quartet<LA, LB, LC, LD, R>(P, Q, pp, rys_tab, eri);
+        if constexpr (kRysLds) contract_quartet<LA, LB, LC, LD, R, true>(P, Q, pp, rys_s, eri);
+        else contract_quartet<LA, LB, LC, LD, R>(P, Q, pp, rys_tab, eri);
         const double deg = deg_p * ((Q.sa == Q.sb) ? 0.5 : 1.0) * ((diag && q == pidx) ? 0.5 : 1.0);
         if (do_j) {
             double Dqs[S::NCD];
@@ -122,7 +134,7 @@ __global__ __launch_bounds__(256) void jk_kernel(
                     const int b = bd / S::ND, d = bd % S::ND;
                     s += Dbd[bd] * (double)eri[VIDX(a, b, c, d)];
                 });
-                atomicAdd(&Kacc[(size_t)(P.bfa + a) * nbf + Q.bfa + c], deg * kCompNorm[LA][a] * kCompNorm[LC][c] * s);
+                kadd(false, a, Q.bfa + c, deg * kCompNorm[LA][a] * kCompNorm[LC][c] * s);
             });
             for_n<U, S::NB * S::NC>([&](int bc) {
                 const int b = bc / S::NC, c = bc % S::NC;
@@ -131,7 +143,7 @@ __global__ __launch_bounds__(256) void jk_kernel(
                     const int a = ad / S::ND, d = ad % S::ND;
                     s += Dad[ad] * (double)eri[VIDX(a, b, c, d)];
                 });
-                atomicAdd(&Kacc[(size_t)(P.bfb + b) * nbf + Q.bfa + c], deg * kCompNorm[LB][b] * kCompNorm[LC][c] * s);
+                kadd(true, b, Q.bfa + c, deg * kCompNorm[LB][b] * kCompNorm[LC][c] * s);
             });
             for_n<U, S::NA * S::ND>([&](int ad) {
                 const int a = ad / S::ND, d = ad % S::ND;
@@ -140,7 +152,7 @@ __global__ __launch_bounds__(256) void jk_kernel(
                     const int b = bc / S::NC, c = bc % S::NC;
                     s += Dbc[bc] * (double)eri[VIDX(a, b, c, d)];
                 });
-                atomicAdd(&Kacc[(size_t)(P.bfa + a) * nbf + Q.bfb + d], deg * kCompNorm[LA][a] * kCompNorm[LD][d] * s);
+                kadd(false, a, Q.bfb + d, deg * kCompNorm[LA][a] * kCompNorm[LD][d] * s);
             });
             for_n<U, S::NB * S::ND>([&](int bd) {
                 const int b = bd / S::ND, d = bd % S::ND;
@@ -149,7 +161,7 @@ __global__ __launch_bounds__(256) void jk_kernel(
                     const int a = ac / S::NC, c = ac % S::NC;
                     s += Dac[ac] * (double)eri[VIDX(a, b, c, d)];
                 });
-                atomicAdd(&Kacc[(size_t)(P.bfb + b) * nbf + Q.bfb + d], deg * kCompNorm[LB][b] * kCompNorm[LD][d] * s);
+                kadd(true, b, Q.bfb + d, deg * kCompNorm[LB][b] * kCompNorm[LD][d] * s);
             });
 #undef VIDX
         }
@@ -165,6 +177,17 @@ __global__ __launch_bounds__(256) void jk_kernel(
                 atomicAdd(&Jacc[(size_t)(P.bfa + a) * nbf + P.bfb + b], s);
             }
         });
+    }
+    if (krow) {
+        __syncthreads();
+        for (int i = tid; i < nrow; i += blockDim.x) {
+            const double v = kl[i];
+            if (v != 0.0) {
+                const int r = i / nbf, col = i - r * nbf;
+                const int row = r < S::NA ? P.bfa + r : P.bfb + (r - S::NA);
+                atomicAdd(&Kacc[(size_t)row * nbf + col], v);
+            }
+        }
     }
 }
 
@@ -330,12 +353,9 @@ __global__ __launch_bounds__(256) void eri_tensor_kernel(
     double thr) {
     using S = EriShape<LA, LB, LC, LD>;
     constexpr bool U = S::kUnroll;
-    __shared__ int s_q1;
     const int pidx = diag ? (nbra - 1 - (int)blockIdx.x) : (int)blockIdx.x;
     const PairRec P = bra[pidx];
-    if (threadIdx.x == 0) s_q1 = first_below(ket, diag ? pidx + 1 : nket, P.Q, thr);
-    __syncthreads();
-    const int q1 = s_q1;
+    const int q1 = diag ? pidx + 1 : nket;  // screened below with the reference's predicate
     for (int q = threadIdx.x; q < q1; q += blockDim.x) {
         const PairRec Q = ket[q];
         if (P.Q * Q.Q < thr) continue;
@@ -361,6 +381,8 @@ __global__ __launch_bounds__(256) void eri_tensor_kernel(
 #define QCG_LA(c) ((c) == 0 ? 0 : (c) <= 2 ? 1 : 2)
 #define QCG_LB(c) ((c) == 0 || (c) == 1 || (c) == 3 ? 0 : ((c) == 2 || (c) == 4) ? 1 : 2)
 
+constexpr size_t kKrowMaxBytes = 32768;
+
 template <int CB, int CK, typename R>
 static hipError_t launch_jk_t(const JkLaunch& a, hipStream_t st) {
     constexpr int LA = QCG_LA(CB), LB = QCG_LB(CB), LC = QCG_LA(CK), LD = QCG_LB(CK);
@@ -373,9 +395,13 @@ static hipError_t launch_jk_t(const JkLaunch& a, hipStream_t st) {
             return hipGetLastError();
         }
     }
-    hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, R>), dim3(a.nitems), dim3(a.block > 0 ? a.block : 64), 0, st, a.items, a.bra, a.ket,
-                       a.diag, (const PrimPair<R>*)a.pp, (const R*)a.rys, a.D, a.Jacc, a.Kacc, a.do_j, a.do_k,
-                       a.ket_off, a.mat_off, a.mol_nbf);
+    // K rows of the bra functions in LDS when they fit (a.nbf is the largest molecule's nbf)
+    using S = EriShape<LA, LB, LC, LD>;
+    const size_t krow_bytes = (size_t)(S::NA + S::NB) * a.nbf * sizeof(double);
+    const int krow = (a.do_k && krow_bytes <= kKrowMaxBytes) ? 1 : 0;
+    hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, R>), dim3(a.nitems), dim3(a.block > 0 ? a.block : 64),
+                       krow ? krow_bytes : 0, st, a.items, a.bra, a.ket, a.diag, (const PrimPair<R>*)a.pp,
+                       (const R*)a.rys, a.D, a.Jacc, a.Kacc, a.do_j, a.do_k, a.ket_off, a.mat_off, a.mol_nbf, krow);
     return hipGetLastError();
 }
 

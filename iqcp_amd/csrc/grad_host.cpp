@@ -11,8 +11,8 @@ using namespace qcg;
 
 namespace {
 
-// Work items of the gradient kernels: every bra pair's surviving kets (exact screening predicate
-// Q_P Q_Q >= threshold on the Q-sorted lists, as for J/K) cut into ranges of kKets.
+// Work items of the gradient kernels: every bra pair's surviving kets (the exact screening
+// predicate Q_P Q_Q >= threshold, per contraction bucket, as for J/K) cut into ranges of kGradKets.
 constexpr int kGradKets = 4;
 
 int eri_grad(qcg_context* c, const double* dD, double hf_frac, double* dgrad, hipStream_t st) {
@@ -22,17 +22,10 @@ int eri_grad(qcg_context* c, const double* dD, double hf_frac, double* dgrad, hi
     for (int cb = kNumPairClasses - 1; cb >= 0; --cb)
         for (int ck = cb; ck >= 0; --ck, ++cls) {
             off[cls] = (int)items.size();
-            const int nb = c->class_off[cb + 1] - c->class_off[cb], nk = c->class_off[ck + 1] - c->class_off[ck];
-            for (int p = 0; p < nb; ++p) {
-                const double Qp = c->pairs[c->class_off[cb] + p].Q;
-                int lo = 0, hi = (cb == ck) ? p + 1 : nk;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (Qp * c->pairs[c->class_off[ck] + mid].Q >= c->schwarz_thr) lo = mid + 1;
-                    else hi = mid;
-                }
-                for (int q0 = 0; q0 < lo; q0 += kGradKets) items.push_back(make_int4(p, q0, std::min(q0 + kGradKets, lo), 0));
-            }
+            for (int p = 0; p < c->ket_cnt_h[cb]; ++p)
+                for_ket_ranges(c, cb, ck, 0, p, c->schwarz_thr, c->schwarz_thr, [&](int bs, int, int e) {
+                    for (int q0 = bs; q0 < e; q0 += kGradKets) items.push_back(make_int4(p, q0, std::min(q0 + kGradKets, e), 0));
+                });
             cnt[cls] = (int)items.size() - off[cls];
         }
     if (items.empty()) return QCG_OK;

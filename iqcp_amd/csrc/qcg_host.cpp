@@ -260,13 +260,24 @@ int qcg_set_basis_batch(qcg_context* c, int nmol, const int* mol_nshell, const i
             c->Qshell[(size_t)all[i].sb * ns + all[i].sa] = Q[i];
         }
     }
+    // contraction buckets: every wavefront of a J/K launch then runs the same primitive loop
+    // trip count; Schwarz order inside each bucket keeps the screened kets a prefix
+    c->bucket_bnd.assign(kNumPairClasses * nmol, std::vector<int>());
     for (int k = 0; k < kNumPairClasses; ++k)
         for (int m = 0; m < nmol; ++m) {
             auto first = all.begin() + c->class_off[k] + c->ket_off_h[k * nmol + m];
             auto last = first + c->ket_cnt_h[k * nmol + m];
-            std::stable_sort(first, last, [](const PairRec& x, const PairRec& y) { return x.Q > y.Q; });
+            std::stable_sort(first, last, [](const PairRec& x, const PairRec& y) {
+                return x.npp != y.npp ? x.npp > y.npp : x.Q > y.Q;
+            });
+            std::vector<int>& bnd = c->bucket_bnd[k * nmol + m];
+            bnd.push_back(0);
             int pos = 0;
-            for (auto it = first; it != last; ++it) it->pos = pos++;
+            for (auto it = first; it != last; ++it) {
+                if (pos > 0 && it->npp != (it - 1)->npp) bnd.push_back(pos);
+                it->pos = pos++;
+            }
+            if (pos > 0) bnd.push_back(pos);
         }
     HIPCHK(c, hipMemcpy(c->d_pairs, all.data(), all.size() * sizeof(PairRec), hipMemcpyHostToDevice));
     c->pairs = all;
@@ -301,27 +312,15 @@ int qcg_screen_counts(qcg_context* c, int64_t* n_kept, int64_t* n_fp32) {
     if (!c) return QCG_INVALID_ARG;
     if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
     int64_t kept = 0, f32 = 0;
-    auto first_below = [&](int off, int qmax, double Qp, double thr) {
-        int lo = 0, hi = qmax;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (Qp * c->pairs[off + mid].Q >= thr) lo = mid + 1;
-            else hi = mid;
-        }
-        return lo;
-    };
+    const double cut = c->fp32_cutoff > c->schwarz_thr ? c->fp32_cutoff : c->schwarz_thr;
     for (int cb = 0; cb < kNumPairClasses; ++cb)
-        for (int ck = 0; ck <= cb; ++ck) {
-            const int nb = c->class_off[cb + 1] - c->class_off[cb], nk = c->class_off[ck + 1] - c->class_off[ck];
-            for (int p = 0; p < nb; ++p) {
-                const double Qp = c->pairs[c->class_off[cb] + p].Q;
-                const int qmax = (cb == ck) ? p + 1 : nk;
-                const int e = first_below(c->class_off[ck], qmax, Qp, c->schwarz_thr);
-                kept += e;
-                if (c->fp32_cutoff > c->schwarz_thr)
-                    f32 += e - std::min(e, first_below(c->class_off[ck], qmax, Qp, c->fp32_cutoff));
-            }
-        }
+        for (int ck = 0; ck <= cb; ++ck)
+            for (int m = 0; m < c->nmol; ++m)
+                for (int p = 0; p < c->ket_cnt_h[cb * c->nmol + m]; ++p)
+                    for_ket_ranges(c, cb, ck, m, p, c->schwarz_thr, cut, [&](int bs, int sp, int e) {
+                        kept += e - bs;
+                        f32 += e - sp;
+                    });
     if (n_kept) *n_kept = kept;
     if (n_fp32) *n_fp32 = f32;
     return QCG_OK;
@@ -335,38 +334,26 @@ static int build_plan(qcg_context* c) {
     if (c->plan_sig == 1 && std::memcmp(key, c->plan_key, sizeof(key)) == 0) return QCG_OK;
     c->items.clear();
     const int nmol = c->nmol;
-    auto first_below = [&](int base, int qmax, double Qp, double thr) {
-        int lo = 0, hi = qmax;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (Qp * c->pairs[base + mid].Q >= thr) lo = mid + 1;
-            else hi = mid;
-        }
-        return lo;
-    };
+    const double cut = mixed ? c->fp32_cutoff : c->schwarz_thr;
     int cls = 0;
     for (int cb = kNumPairClasses - 1; cb >= 0; --cb)
         for (int ck = cb; ck >= 0; --ck, ++cls) {
-            const int ch = std::max(1, c->tile[cls].block > 0 ? c->tile[cls].block * c->tile[cls].reps : c->tile[cls].reps);
+            const qcg_context::Tile& t = c->tile[cls];
+            const int ch = std::max(1, t.block > 0 ? t.block * t.reps : t.reps);
             for (int part = 0; part < 2; ++part) {
                 c->item_off[cls][part] = (int)c->items.size();
                 if (part == 1 && !mixed) {
                     c->item_cnt[cls][part] = 0;
                     continue;
                 }
-                const double lo_thr = (part == 0 && mixed) ? c->fp32_cutoff : c->schwarz_thr;
-                const bool has_hi = part == 1;
                 for (int m = 0; m < nmol; ++m) {
-                    const int nb = c->ket_cnt_h[cb * nmol + m], nk = c->ket_cnt_h[ck * nmol + m];
                     const int pbase = c->ket_off_h[cb * nmol + m];
-                    const int kbase = c->class_off[ck] + c->ket_off_h[ck * nmol + m];
-                    for (int p = 0; p < nb; ++p) {
-                        const double Qp = c->pairs[c->class_off[cb] + pbase + p].Q;
-                        const int qmax = (cb == ck) ? p + 1 : nk;
-                        const int e = first_below(kbase, qmax, Qp, lo_thr);
-                        const int s = has_hi ? std::min(e, first_below(kbase, qmax, Qp, c->fp32_cutoff)) : 0;
-                        for (int q0 = s; q0 < e; q0 += ch) c->items.push_back(make_int4(pbase + p, q0, std::min(q0 + ch, e), 0));
-                    }
+                    for (int p = 0; p < c->ket_cnt_h[cb * nmol + m]; ++p)
+                        for_ket_ranges(c, cb, ck, m, p, c->schwarz_thr, cut, [&](int bs, int sp, int e) {
+                            const int lo = part == 0 ? bs : sp, hi = part == 0 ? sp : e;
+                            for (int q0 = lo; q0 < hi; q0 += ch)
+                                c->items.push_back(make_int4(pbase + p, q0, std::min(q0 + ch, hi), 0));
+                        });
                 }
                 c->item_cnt[cls][part] = (int)c->items.size() - c->item_off[cls][part];
             }
@@ -541,15 +528,6 @@ int qcg_class_stats(qcg_context* c, qcg_class_stat* out, int* n) {
     if (cached) {
         for (int i = 0; i < 21; ++i) out[i] = c->stats[i];
     }
-    auto first_below = [&](int off, int qmax, double Qp, double t) {
-        int lo = 0, hi = qmax;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (Qp * c->pairs[off + mid].Q >= t) lo = mid + 1;
-            else hi = mid;
-        }
-        return lo;
-    };
     int cls = 0;
     for (int cb = kNumPairClasses - 1; cb >= 0; --cb)
         for (int ck = cb; ck >= 0; --ck, ++cls) {
@@ -561,22 +539,25 @@ int qcg_class_stats(qcg_context* c, qcg_class_stat* out, int* n) {
             const int la = cb == 0 ? 0 : cb <= 2 ? 1 : 2, lb = (cb == 0 || cb == 1 || cb == 3) ? 0 : (cb == 2 || cb == 4) ? 1 : 2;
             const int lc = ck == 0 ? 0 : ck <= 2 ? 1 : 2, ld = (ck == 0 || ck == 1 || ck == 3) ? 0 : (ck == 2 || ck == 4) ? 1 : 2;
             s.la = la; s.lb = lb; s.lc = lc; s.ld = ld;
-            const int nb = c->class_off[cb + 1] - c->class_off[cb], nk = c->class_off[ck + 1] - c->class_off[ck];
             int64_t q64 = 0, q32 = 0, p64 = 0, p32 = 0;
+            for (int m = 0; m < c->nmol; ++m) {
+                const int nb = c->ket_cnt_h[cb * c->nmol + m];
+                const int bbase = c->class_off[cb] + c->ket_off_h[cb * c->nmol + m];
+                const int kbase = c->class_off[ck] + c->ket_off_h[ck * c->nmol + m];
 #pragma omp parallel for schedule(dynamic, 16) reduction(+ : q64, q32, p64, p32)
-            for (int p = 0; p < nb; ++p) {
-                const PairRec& P = c->pairs[c->class_off[cb] + p];
-                const int qmax = (cb == ck) ? p + 1 : nk;
-                const int e_all = first_below(c->class_off[ck], qmax, P.Q, thr);
-                const int e_64 = mixed ? std::min(e_all, first_below(c->class_off[ck], qmax, P.Q, cut)) : e_all;
-                for (int q = 0; q < e_all; ++q) {
-                    const PairRec& Q = c->pairs[c->class_off[ck] + q];
-                    int64_t np = 0;
-                    for (int i = 0; i < P.npp; ++i) {
-                        const double ka = c->h_pp[P.pp_off + i].kab;
-                        for (int j = 0; j < Q.npp; ++j) np += (ka * c->h_pp[Q.pp_off + j].kab >= kPrimScreen) ? 1 : 0;
-                    }
-                    if (q < e_64) { ++q64; p64 += np; } else { ++q32; p32 += np; }
+                for (int p = 0; p < nb; ++p) {
+                    const PairRec& P = c->pairs[bbase + p];
+                    for_ket_ranges(c, cb, ck, m, p, thr, cut, [&](int bs, int sp, int e) {
+                        for (int q = bs; q < e; ++q) {
+                            const PairRec& Q = c->pairs[kbase + q];
+                            int64_t np = 0;
+                            for (int i = 0; i < P.npp; ++i) {
+                                const double ka = c->h_pp[P.pp_off + i].kab;
+                                for (int j = 0; j < Q.npp; ++j) np += (ka * c->h_pp[Q.pp_off + j].kab >= kPrimScreen) ? 1 : 0;
+                            }
+                            if (q < sp) { ++q64; p64 += np; } else { ++q32; p32 += np; }
+                        }
+                    });
                 }
             }
             const int nr = (la + lb + lc + ld) / 2 + 1;

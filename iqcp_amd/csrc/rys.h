@@ -15,9 +15,14 @@ constexpr int kRysStride = QCG_RYS_DEG + 1;
 
 QCG_HD constexpr int rys_tri(int n) { return n * (n - 1) / 2; }
 
-// tab: qcg_rys_coef (double) or its float copy, resident in device memory.
+// Number of table entries of the N-point rule (intervals x 2N polynomials x coefficients).
+template <int N>
+constexpr int rys_tab_size() { return qcg_rys_nint[N] * 2 * N * (QCG_RYS_DEG + 1); }
+
+// tabN: the coefficients of the N-point rule, i.e. qcg_rys_coef + qcg_rys_off[N] (or its float
+// copy), in device memory or staged in LDS.
 template <typename R, int N>
-QCG_HD void rys_eval(R T, const R* __restrict__ tab, R* x, R* w) {
+QCG_HD void rys_eval_tab(R T, const R* __restrict__ tabN, R* x, R* w) {
     static_assert(N >= 1 && N <= QCG_RYS_NMAX, "Rys order out of table range");
     if (T >= (R)qcg_rys_tmax[N]) {
         const R it = R(1) / T;
@@ -31,7 +36,7 @@ QCG_HD void rys_eval(R T, const R* __restrict__ tab, R* x, R* w) {
     }
     const int k = (int)(T * (R)(1.0 / QCG_RYS_H));
     const R s = (T - (R)(k * QCG_RYS_H)) * (R)(2.0 / QCG_RYS_H) - R(1);
-    const R* c = tab + qcg_rys_off[N] + k * (2 * N * kRysStride);
+    const R* c = tabN + k * (2 * N * kRysStride);
 #pragma unroll
     for (int i = 0; i < 2 * N; ++i) {
         const R* ci = c + i * kRysStride;
@@ -40,6 +45,12 @@ QCG_HD void rys_eval(R T, const R* __restrict__ tab, R* x, R* w) {
         for (int j = QCG_RYS_DEG - 1; j >= 0; --j) v = v * s + ci[j];
         if (i < N) x[i] = v; else w[i - N] = v;
     }
+}
+
+// tab: qcg_rys_coef (double) or its float copy, resident in device memory.
+template <typename R, int N>
+QCG_HD void rys_eval(R T, const R* __restrict__ tab, R* x, R* w) {
+    rys_eval_tab<R, N>(T, tab + qcg_rys_off[N], x, w);
 }
 
 // Runtime-order variant for host code and batch kernels.

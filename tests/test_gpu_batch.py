@@ -31,11 +31,19 @@ def test_batch_scf_equals_single(use_graph):
 
 
 def test_alkane_batch_64():
-    """The 64-molecule workload: every molecule converges; three checked against single runs."""
+    """The 64-molecule workload: every molecule converges; four checked against single runs.
+
+    Core-Hamiltonian guess + DIIS alone diverges for molecule 56 (C11H24): the same SCF driven by
+    the oracle's Fock builds on the CPU diverges identically, so it is a property of the
+    iteration, not of the GPU path.  The workload therefore uses the reference's level_shift
+    option (scf/mod.rs:1038-1063) at 0.3 Ha, which converges all 64 (molecule 56 in 32
+    iterations to -430.3014749375 Ha on the CPU)."""
     bases = workloads.alkane_batch_631g()
     assert len(bases) == 64 and min(b.nbf for b in bases) == 56 and max(b.nbf for b in bases) == 160
-    res, Ds, ms = BatchEngine(bases).rhf_scf(ScfConfig.tight())
+    cfg = ScfConfig.tight(level_shift=0.3)
+    res, Ds, ms = BatchEngine(bases).rhf_scf(cfg)
     assert all(r["converged"] == 1 for r in res)
-    for i in (0, 17, 63):
-        single = Engine(bases[i]).rhf_scf(ScfConfig.tight())
+    assert abs(res[56]["energy_total"] - (-430.3014749375)) < 1e-8
+    for i in (0, 17, 56, 63):
+        single = Engine(bases[i]).rhf_scf(cfg)
         assert abs(res[i]["energy_total"] - single["energy_total"]) < 1e-9
