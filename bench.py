@@ -27,6 +27,27 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def ncart(l):
+    return (l + 1) * (l + 2) // 2
+
+
+def kernel_name(la, lb, lc, ld, tile, f64):
+    """Name of the J/K kernel a class runs under a tile (block 0 = wave-cooperative kernel)."""
+    nq = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
+    k = "jk_wave_kernel" if (tile[0] == 0 and nq >= 27) else "jk_kernel"
+    return f"{k}<{la}, {lb}, {lc}, {ld}, {'double' if f64 else 'float'}>"
+
+
+def hbm_traffic(name):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc measurement
+    (FETCH_SIZE and WRITE_SIZE, scaled by the same counters on a 1 GiB copy); None if absent."""
+    f = ROOT / "profiles" / "jk_decane_631gs_hbm_traffic.json"
+    if not f.exists():
+        return None
+    k = json.loads(f.read_text()).get("kernels", {}).get(name)
+    return None if k is None else k["bytes_per_launch"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,11 +160,15 @@ def main():
             if ms > 0 and (best is None or ms > best[0]):
                 best = (ms, s, part, dt)
     ms, s, part, dt = best
-    cls = f"({'spd'[s['la']]}{'spd'[s['lb']]}|{'spd'[s['lc']]}{'spd'[s['ld']]})"
+    ci = per_kernel["mixed"].index(s)
+    tile = tiles["mixed"][ci][0 if part == "fp64" else 1]
+    kname = kernel_name(s["la"], s["lb"], s["lc"], s["ld"], tile, part == "fp64")
     achieved = s[f"flops_{part}"] / (ms * 1e-3) / 1e12
-    roofline = {"bound": "valu", "kernel": f"jk_kernel{cls} {dt}", "achieved": round(achieved, 3),
-                "peak": PEAK[dt], "unit": "TFLOP/s", "frac": round(achieved / PEAK[dt], 4), "traffic": None,
-                "kernel_ms": round(ms, 4), "kernel_flops_model": s[f"flops_{part}"]}
+    # FP64 vector ALU bound: MI355X's FP64 peak is 78.6 TFLOP/s for vector and matrix alike
+    roofline = {"bound": "valu", "kernel": kname, "achieved": round(achieved, 3),
+                "peak": PEAK[dt], "unit": "TFLOP/s", "frac": round(achieved / PEAK[dt], 4),
+                "traffic": hbm_traffic(kname), "kernel_ms": round(ms, 4), "kernel_flops_model": s[f"flops_{part}"],
+                "tile": list(tile)}
     tot_fl = sum(x["flops_fp64"] + x["flops_fp32"] for x in per_kernel["mixed"])
     tot_ms = sum(x["ms_fp64"] + x["ms_fp32"] for x in per_kernel["mixed"])
     kept, n32 = eng.screen_counts()

@@ -135,12 +135,13 @@ typedef struct qcg_class_stat {
     double ms_fp64, ms_fp32;
 } qcg_class_stat;
 int qcg_set_profiling(qcg_context* ctx, int on);
-/* Autotuner for the J/K kernels: for every quartet class, time each candidate tile (threads per
- * block x kets per thread, or block 0 = one wavefront per contracted quartet, kets per work item)
- * on the device density dD (iters timed builds per candidate) and keep the fastest.  chosen
- * (optional, 42 ints) receives (block, kets) per class. */
+/* Autotuner for the J/K kernels: for every quartet class and precision part (FP64 quartets, FP32
+ * quartets of the mixed mode), time each candidate tile (threads per block x kets per thread, or
+ * block 0 = one wavefront per contracted quartet, kets per work item) on the device density dD
+ * (iters timed builds per candidate) and keep the fastest.  chosen (optional, 84 ints) receives
+ * (block, kets) per class and part: [class][part][2]. */
 int qcg_autotune(qcg_context* ctx, const double* dD, int iters, int* chosen);
-/* Set / read the J/K tiles directly (42 ints, same layout as qcg_autotune's chosen; classes in the
+/* Set / read the J/K tiles directly (84 ints, same layout as qcg_autotune's chosen; classes in the
  * order (dd|dd), (dd|dp), ..., (ss|ss)).  No counterpart in qc-core: launch configuration only.
  * Block 0 on a class with fewer than 27 components runs the per-lane kernel with 64 threads. */
 int qcg_set_tiles(qcg_context* ctx, const int* tiles);
@@ -197,6 +198,15 @@ int qcg_one_electron_gradient(qcg_context* ctx, const double* D, const double* W
 /* rhf_gradient (scf/gradient.rs:900): W = 2 sum_i<n_occ eps_i C_i C_i^T (:101), then both parts.
  * C: nbf x nbf, columns = MOs (row-major storage, C[mu*nbf + i]). */
 int qcg_rhf_gradient(qcg_context* ctx, const double* D, const double* C, const double* eps, int n_occ, double* grad);
+/* XC part of the nuclear gradient, B3LYP on the context's grid, no grid-weight response: replaces
+ * xc_gradient_analytical (scf/gradient.rs:1687-1910, with evaluate_basis_hessian_on_grid :1938).
+ * grad (natom x 3) is overwritten. */
+int qcg_xc_gradient(qcg_context* ctx, const double* D, double* grad);
+/* B3LYP (closed-shell KS) nuclear gradient at a converged KS state: nuclear repulsion + one-electron
+ * (D, W) + two-electron with exact-exchange fraction 0.20 + XC, i.e. the gradient assembly of
+ * ks_dft_gradient_with_guess (scf/gradient.rs:1214-1341) without D3(BJ).  C: MOs in columns,
+ * row-major; eps ascending; grad (natom x 3) overwritten. */
+int qcg_ks_gradient(qcg_context* ctx, const double* D, const double* C, const double* eps, int n_occ, double* grad);
 
 #ifdef __cplusplus
 }

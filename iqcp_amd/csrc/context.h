@@ -60,7 +60,7 @@ struct qcg_context {
     bool ev_recorded[21][2] = {};
     // J/K work plan: items per (class pair, precision part), rebuilt when options or tiles change
     struct Tile { int block = 64, reps = 1; };
-    Tile tile[21];
+    Tile tile[21][2];  // per quartet class and precision part (0 FP64, 1 FP32)
     int class_nq[21] = {};  // Cartesian components per quartet class
     std::vector<int4> items;
     int item_off[21][2] = {}, item_cnt[21][2] = {};

@@ -159,21 +159,21 @@ QCG_HD void contract_quartet(const PairRec& P, const PairRec& Q,
     for (int i = 0; i < P.npp; ++i) {
         const PrimPair<R> a = pp[P.pp_off + i];
         const R p = a.p;
-        const R hp = R(0.5) / p;
+        const R hp = R(0.5) * qrcp(p);
         for (int j = 0; j < Q.npp; ++j) {
             const PrimPair<R> c = pp[Q.pp_off + j];
             if (a.kab * c.kab < (R)kPrimScreen) continue;
             const R q = c.p;
             const R pq_sum = p + q;
-            const R ipq = R(1) / pq_sum;
+            const R ipq = qrcp(pq_sum);
             const R PQx = a.px - c.px, PQy = a.py - c.py, PQz = a.pz - c.pz;
             const R rho = p * q * ipq;
             const R T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-            const R pref = two_pi_52 / (p * q * sqrt(pq_sum)) * a.ck * c.ck;
+            const R pref = two_pi_52 * qrcp(p * q) * qrsqrt(pq_sum) * a.ck * c.ck;
             R rx[N], rw[N];
             if constexpr (TAB_N) rys_eval_tab<R, N>(T, rys_tab, rx, rw);
             else rys_eval<R, N>(T, rys_tab, rx, rw);
-            const R hq = R(0.5) / q;
+            const R hq = R(0.5) * qrcp(q);
             const R qf = q * ipq, pf = p * ipq;
             const R PAx = a.px - Ax, PAy = a.py - Ay, PAz = a.pz - Az;
             const R QCx = c.px - Cx, QCy = c.py - Cy, QCz = c.pz - Cz;

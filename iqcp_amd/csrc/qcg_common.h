@@ -43,6 +43,24 @@ QCG_HD constexpr int cart_y(int l, int c) {
 }
 QCG_HD constexpr int cart_z(int l, int c) { return l - cart_x(l, c) - cart_y(l, c); }
 
+// Reciprocal and reciprocal square root in the arithmetic type R.  FP64: correctly rounded
+// division / sqrt.  FP32 on the device: the hardware v_rcp_f32 / v_rsq_f32 (about 1 ulp), which is
+// all the FP32 quartets of the mixed mode need (their contributions are below 1e-6 x 1e-7).
+template <typename R>
+QCG_HD inline R qrcp(R x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (sizeof(R) == 4) return __builtin_amdgcn_rcpf(x);
+#endif
+    return R(1) / x;
+}
+template <typename R>
+QCG_HD inline R qrsqrt(R x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (sizeof(R) == 4) return __builtin_amdgcn_rsqf(x);
+#endif
+    return R(1) / sqrt(x);
+}
+
 QCG_HD constexpr double dfact_odd(int n) {  // (2n-1)!! with (-1)!! = 1
     double r = 1.0;
     for (int k = 2 * n - 1; k > 1; k -= 2) r *= k;

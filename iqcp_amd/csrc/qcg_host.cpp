@@ -55,8 +55,10 @@ int qcg_create(qcg_context** out, int device) {
             const int la = cb == 0 ? 0 : cb <= 2 ? 1 : 2, lb = (cb == 0 || cb == 1 || cb == 3) ? 0 : (cb == 2 || cb == 4) ? 1 : 2;
             const int lc = ck == 0 ? 0 : ck <= 2 ? 1 : 2, ld = (ck == 0 || ck == 1 || ck == 3) ? 0 : (ck == 2 || ck == 4) ? 1 : 2;
             const int nq = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
-            c->tile[cls].block = nq >= 81 ? 0 : 64;  // block 0: wave-cooperative kernel
-            c->tile[cls].reps = nq <= 3 ? 4 : nq <= 9 ? 2 : nq >= 81 ? 2 : 1;
+            for (int part = 0; part < 2; ++part) {
+                c->tile[cls][part].block = nq >= 81 ? 0 : 64;  // block 0: wave-cooperative kernel
+                c->tile[cls][part].reps = nq <= 3 ? 4 : nq <= 9 ? 2 : nq >= 81 ? 2 : 1;
+            }
             c->class_nq[cls] = nq;
         }
     *out = c.release();
@@ -338,9 +340,9 @@ static int build_plan(qcg_context* c) {
     int cls = 0;
     for (int cb = kNumPairClasses - 1; cb >= 0; --cb)
         for (int ck = cb; ck >= 0; --ck, ++cls) {
-            const qcg_context::Tile& t = c->tile[cls];
-            const int ch = std::max(1, t.block > 0 ? t.block * t.reps : t.reps);
             for (int part = 0; part < 2; ++part) {
+                const qcg_context::Tile& t = c->tile[cls][part];
+                const int ch = std::max(1, t.block > 0 ? t.block * t.reps : t.reps);
                 c->item_off[cls][part] = (int)c->items.size();
                 if (part == 1 && !mixed) {
                     c->item_cnt[cls][part] = 0;
@@ -422,7 +424,7 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
                 a.mol_nbf = c->d_mol_nbf;
                 a.items = c->d_items + c->item_off[cls][part];
                 a.nitems = c->item_cnt[cls][part];
-                a.block = c->tile[cls].block;
+                a.block = c->tile[cls][part].block;
                 a.pp = part ? (const void*)c->d_ppf : (const void*)c->d_pp;
                 a.rys = part ? (const void*)c->d_rys_f : (const void*)c->d_rys;
                 hipStream_t s = fork ? c->aux[launched % qcg_context::kStreams] : st;
@@ -455,17 +457,19 @@ int qcg_autotune(qcg_context* c, const double* dD, int iters, int* chosen) {
     // (wave-cooperative kernel, classes with at least 27 components only)
     const int cand[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {256, 1}, {64, 8}, {0, 1}, {0, 2}, {0, 4}};
     const int ncand = sizeof(cand) / sizeof(cand[0]);
-    std::vector<double> best(21, 1e300);
-    std::vector<int> pick(21, 0);
+    auto tile_of = [&](int k, int cls) {
+        return (cand[k][0] == 0 && c->class_nq[cls] < 27) ? qcg_context::Tile{64, 1}
+                                                          : qcg_context::Tile{cand[k][0], cand[k][1]};
+    };
+    std::vector<double> best(42, 1e300);
+    std::vector<int> pick(42, 0);
     const bool prof = c->profiling;
     c->profiling = true;
     iters = std::max(1, iters);
     for (int k = 0; k < ncand; ++k) {
-        for (int i = 0; i < 21; ++i)
-            c->tile[i] = (cand[k][0] == 0 && c->class_nq[i] < 27) ? qcg_context::Tile{64, 1}
-                                                                  : qcg_context::Tile{cand[k][0], cand[k][1]};
+        for (int i = 0; i < 21; ++i) c->tile[i][0] = c->tile[i][1] = tile_of(k, i);
         c->tile_sig += 1;
-        std::vector<double> tot(21, 0.0);
+        std::vector<double> tot(42, 0.0);
         for (int it = 0; it < iters + 1; ++it) {
             int rc = run_jk(c, dD, c->d_J, c->d_K, c->stream);
             if (rc) { c->profiling = prof; return rc; }
@@ -476,17 +480,20 @@ int qcg_autotune(qcg_context* c, const double* dD, int iters, int* chosen) {
                     if (c->ev_recorded[i][part]) {
                         float ms = 0.f;
                         HIPCHK(c, hipEventElapsedTime(&ms, c->ev[4 * i + 2 * part], c->ev[4 * i + 2 * part + 1]));
-                        tot[i] += ms;
+                        tot[2 * i + part] += ms;
                     }
         }
-        for (int i = 0; i < 21; ++i)
+        for (int i = 0; i < 42; ++i)
             if (tot[i] < best[i]) { best[i] = tot[i]; pick[i] = k; }
     }
-    for (int i = 0; i < 21; ++i) {
-        c->tile[i] = (cand[pick[i]][0] == 0 && c->class_nq[i] < 27) ? qcg_context::Tile{64, 1}
-                                                                    : qcg_context::Tile{cand[pick[i]][0], cand[pick[i]][1]};
-        if (chosen) { chosen[2 * i] = c->tile[i].block; chosen[2 * i + 1] = c->tile[i].reps; }
-    }
+    for (int i = 0; i < 21; ++i)
+        for (int part = 0; part < 2; ++part) {
+            c->tile[i][part] = tile_of(pick[2 * i + part], i);
+            if (chosen) {
+                chosen[4 * i + 2 * part] = c->tile[i][part].block;
+                chosen[4 * i + 2 * part + 1] = c->tile[i][part].reps;
+            }
+        }
     c->tile_sig += 1;
     c->profiling = prof;
     return QCG_OK;
@@ -494,21 +501,24 @@ int qcg_autotune(qcg_context* c, const double* dD, int iters, int* chosen) {
 
 int qcg_set_tiles(qcg_context* c, const int* tiles) {
     if (!c || !tiles) return QCG_INVALID_ARG;
-    for (int i = 0; i < 21; ++i)
+    for (int i = 0; i < 42; ++i)
         if (tiles[2 * i] < 0 || tiles[2 * i] > 1024 || (tiles[2 * i] % 64) != 0 || tiles[2 * i + 1] < 1 ||
             tiles[2 * i + 1] > 64)
             return fail(c, QCG_INVALID_ARG, "tile: block must be 0 or a multiple of 64 up to 1024, kets 1..64");
-    for (int i = 0; i < 21; ++i) c->tile[i] = {tiles[2 * i], tiles[2 * i + 1]};
+    for (int i = 0; i < 21; ++i)
+        for (int part = 0; part < 2; ++part)
+            c->tile[i][part] = {tiles[4 * i + 2 * part], tiles[4 * i + 2 * part + 1]};
     c->tile_sig += 1;
     return QCG_OK;
 }
 
 int qcg_get_tiles(qcg_context* c, int* tiles) {
     if (!c || !tiles) return QCG_INVALID_ARG;
-    for (int i = 0; i < 21; ++i) {
-        tiles[2 * i] = c->tile[i].block;
-        tiles[2 * i + 1] = c->tile[i].reps;
-    }
+    for (int i = 0; i < 21; ++i)
+        for (int part = 0; part < 2; ++part) {
+            tiles[4 * i + 2 * part] = c->tile[i][part].block;
+            tiles[4 * i + 2 * part + 1] = c->tile[i][part].reps;
+        }
     return QCG_OK;
 }
 

@@ -8,6 +8,7 @@
 // Phi and grad Phi are evaluated once per grid and kept resident in HBM.
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "context.h"
@@ -55,6 +56,8 @@ int upload_grid(qcg_context* c) {
         g.nprim = B.nprim[s];
         g.poff = (int)ex.size();
         g.bf = B.bf_off[s];
+        g.atom = B.shell_atom[s];
+        g.pad = 0;
         double amin = 1e300;
         for (int p = 0; p < B.nprim[s]; ++p) {
             const double a = B.exps[B.prim_off[s] + p];
@@ -99,9 +102,89 @@ int xc_build(qcg_context* c, const double* dD, double* dV, double* dExc, hipStre
     return QCG_OK;
 }
 
+// XC nuclear gradient at the device density dD, ADDED into dgrad (natom x 3) on stream st.
+int xc_gradient(qcg_context* c, const double* dD, double* dgrad, hipStream_t st) {
+    if (c->G <= 0) return fail(c, QCG_NO_GRID, "no grid: call qcg_build_grid or qcg_set_grid");
+    int rc = ensure_blas(c);
+    if (rc) return rc;
+    const HostBasis& B = c->basis;
+    const int nbf = B.nbf;
+    const rocblas_int G = (rocblas_int)c->G;
+    if (rocblas_set_stream(c->blas, st) != rocblas_status_success) return fail(c, QCG_DEVICE, "rocblas_set_stream");
+    const double one = 1.0, zero = 0.0;
+    double *dM = nullptr, *dt = nullptr;
+    HIPCHK(c, c->alloc(&dM, (size_t)G * nbf));
+    HIPCHK(c, c->alloc(&dt, 3 * (size_t)G));
+    rc = QCG_OK;
+    if (rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, nbf, G, nbf, &one, dD, nbf, c->d_phi,
+                      nbf, &zero, c->d_X, nbf) != rocblas_status_success)
+        rc = fail(c, QCG_DEVICE, "rocblas_dgemm (Phi D)");
+    hipError_t e = hipSuccess;
+    if (!rc) e = launch_xc_grad_points((int)G, nbf, c->d_w, c->d_phi, c->d_dphx, c->d_dphy, c->d_dphz, c->d_X,
+                                       c->d_aow, dt, st);
+    if (!rc && e != hipSuccess) rc = fail(c, QCG_DEVICE, std::string("xc_grad_points: ") + hipGetErrorString(e));
+    // M (= A D, row-major G x nbf) as column-major nbf x G: D^T (= D) * A^T
+    if (!rc && rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, nbf, G, nbf, &one, dD, nbf,
+                             c->d_aow, nbf, &zero, dM, nbf) != rocblas_status_success)
+        rc = fail(c, QCG_DEVICE, "rocblas_dgemm (A D)");
+    if (!rc) e = launch_xc_grad_accumulate((int)G, B.nshell, c->d_gshell, c->d_gexp, c->d_gcn, c->d_pts, nbf, c->d_X,
+                                           dM, dt, dgrad, st);
+    if (!rc && e != hipSuccess) rc = fail(c, QCG_DEVICE, std::string("xc_grad_accumulate: ") + hipGetErrorString(e));
+    const hipError_t es = hipStreamSynchronize(st);  // the temporaries are read until here
+    c->release(dM);
+    c->release(dt);
+    if (rc) return rc;
+    HIPCHK(c, es);
+    return QCG_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int qcg_xc_gradient(qcg_context* c, const double* D, double* grad) {
+    if (!c || !D || !grad) return QCG_INVALID_ARG;
+    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    if (c->nmol != 1) return fail(c, QCG_INVALID_ARG, "gradient: one molecule per context");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nbf = c->basis.nbf, n3 = 3 * c->basis.natom;
+    double* dg = nullptr;
+    HIPCHK(c, c->alloc(&dg, n3));
+    HIPCHK(c, hipMemcpyAsync(c->d_D, D, (size_t)nbf * nbf * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_fill(dg, n3, 0.0, c->stream));
+    int rc = xc_gradient(c, c->d_D, dg, c->stream);
+    if (!rc) {
+        HIPCHK(c, hipMemcpyAsync(grad, dg, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    c->release(dg);
+    return rc;
+}
+
+int qcg_ks_gradient(qcg_context* c, const double* D, const double* C, const double* eps, int n_occ, double* grad) {
+    if (!c || !D || !C || !eps || !grad) return QCG_INVALID_ARG;
+    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    if (c->G <= 0) return fail(c, QCG_NO_GRID, "no grid: call qcg_build_grid or qcg_set_grid");
+    const int nbf = c->basis.nbf, n3 = 3 * c->basis.natom;
+    if (n_occ < 0 || n_occ > nbf) return fail(c, QCG_INVALID_ARG, "n_occ");
+    // W = 2 sum_i eps_i C_mu,i C_nu,i  (build_energy_weighted_density, scf/gradient.rs:101)
+    std::vector<double> W((size_t)nbf * nbf, 0.0);
+    for (int mu = 0; mu < nbf; ++mu)
+        for (int nu = 0; nu < nbf; ++nu) {
+            double v = 0.0;
+            for (int i = 0; i < n_occ; ++i) v += eps[i] * C[(size_t)mu * nbf + i] * C[(size_t)nu * nbf + i];
+            W[(size_t)mu * nbf + nu] = 2.0 * v;
+        }
+    std::vector<double> g2(n3), gx(n3);
+    int rc = qcg_eri_gradient(c, D, 0.20, g2.data());  // B3lyp::A = 0.20 exact exchange
+    if (rc) return rc;
+    rc = qcg_xc_gradient(c, D, gx.data());
+    if (rc) return rc;
+    rc = qcg_one_electron_gradient(c, D, W.data(), grad);
+    if (rc) return rc;
+    for (int i = 0; i < n3; ++i) grad[i] += g2[i] + gx[i];
+    return QCG_OK;
+}
 
 int qcg_build_grid(qcg_context* c, int n_radial, int quality, int pruning, int64_t* npts) {
     if (!c) return QCG_INVALID_ARG;

@@ -122,6 +122,143 @@ __global__ void sym_half_kernel(const double* __restrict__ V, double* __restrict
     out[(size_t)i * n + j] = 0.5 * (V[(size_t)i * n + j] + V[(size_t)j * n + i]);
 }
 
+// XC nuclear gradient (xc_gradient_analytical, scf/gradient.rs:1687-1882; B3LYP, no grid-weight
+// response).  Per point: rho, grad rho, B3LYP, then the row
+//   A_g = w vrho Phi_g + 2 w vsigma (grad rho . grad Phi_g)
+// so that (A D)_g,mu = w vrho (chi D)_mu + 2 w vsigma grad rho . (grad chi D)_mu is the reference's
+// aow (:1846-1856), and t_g = 2 w vsigma grad rho for the basis-Hessian part (:1858-1874).
+__global__ __launch_bounds__(256) void xc_grad_point_kernel(int G, int nbf, const double* __restrict__ w,
+                                                            const double* __restrict__ phi,
+                                                            const double* __restrict__ dphx,
+                                                            const double* __restrict__ dphy,
+                                                            const double* __restrict__ dphz,
+                                                            const double* __restrict__ X, double* __restrict__ A,
+                                                            double* __restrict__ t) {
+    const int lane = threadIdx.x & 63;
+    const int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (g >= G) return;
+    const size_t row = (size_t)g * nbf;
+    double r = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+    for (int mu = lane; mu < nbf; mu += 64) {
+        const double x = X[row + mu];
+        r += phi[row + mu] * x;
+        gx += dphx[row + mu] * x;
+        gy += dphy[row + mu] * x;
+        gz += dphz[row + mu] * x;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        r += __shfl_xor(r, off, 64);
+        gx += __shfl_xor(gx, off, 64);
+        gy += __shfl_xor(gy, off, 64);
+        gz += __shfl_xor(gz, off, 64);
+    }
+    const double rho = fmax(r, 0.0);
+    const double grx = 2.0 * gx, gry = 2.0 * gy, grz = 2.0 * gz;
+    double exc, vrho, vsig;
+    xc_b3lyp(rho, grx * grx + gry * gry + grz * grz, exc, vrho, vsig);
+    const double wvr = w[g] * vrho, f2 = 2.0 * w[g] * vsig;
+    for (int mu = lane; mu < nbf; mu += 64)
+        A[row + mu] = wvr * phi[row + mu] + f2 * (grx * dphx[row + mu] + gry * dphy[row + mu] + grz * dphz[row + mu]);
+    if (lane == 0) {
+        t[3 * (size_t)g] = f2 * grx;
+        t[3 * (size_t)g + 1] = f2 * gry;
+        t[3 * (size_t)g + 2] = f2 * grz;
+    }
+}
+
+// One block row per shell (blockIdx.y), threads over grid points: every thread of a block adds to
+// the same atom, so the three sums are wave-reduced before one atomic per wave.
+//   grad[A] += -2 (A D)_g,mu grad chi_mu(g) - 2 (chi D)_g,mu sum_e t_e d2 chi_mu/dr_d dr_e
+// with d2 chi/dr_a dr_b = N [d2ang R0 - 2 (r_b dang_a + r_a dang_b + delta_ab ang) R1
+// + 4 r_a r_b ang R2], R_k = sum_p c_p n_p alpha_p^k e^{-alpha_p r^2} (gradient.rs:1938-2050).
+__global__ __launch_bounds__(256) void xc_grad_kernel(int G, const GridShell* __restrict__ shells,
+                                                      const double* __restrict__ exps, const double* __restrict__ cn,
+                                                      const double* __restrict__ pts, int nbf,
+                                                      const double* __restrict__ X, const double* __restrict__ M,
+                                                      const double* __restrict__ t, double* __restrict__ grad) {
+    const GridShell sh = shells[blockIdx.y];
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (g < G) {
+        const double d[3] = {pts[3 * (size_t)g] - sh.cx, pts[3 * (size_t)g + 1] - sh.cy, pts[3 * (size_t)g + 2] - sh.cz};
+        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (!(sh.amin * r2 > 34.5388)) {
+            double R0 = 0.0, R1 = 0.0, R2 = 0.0;
+            for (int p = 0; p < sh.nprim; ++p) {
+                const double a = exps[sh.poff + p];
+                const double e = cn[sh.poff + p] * exp(-a * r2);
+                R0 += e;
+                R1 += a * e;
+                R2 += a * a * e;
+            }
+            const double tv[3] = {t[3 * (size_t)g], t[3 * (size_t)g + 1], t[3 * (size_t)g + 2]};
+            const size_t row = (size_t)g * nbf + sh.bf;
+            auto pw = [](double v, int k) { return k <= 0 ? (k == 0 ? 1.0 : 0.0) : (k == 1 ? v : v * v); };
+            const int nc = ncart(sh.l);
+            for (int c = 0; c < nc; ++c) {
+                const int p3[3] = {cart_x(sh.l, c), cart_y(sh.l, c), cart_z(sh.l, c)};
+                const double nrm = (sh.l == 2 && (p3[0] == 2 || p3[1] == 2 || p3[2] == 2)) ? 0.57735026918962576451 : 1.0;
+                auto angp = [&](int a0, int a1, int a2) { return pw(d[0], a0) * pw(d[1], a1) * pw(d[2], a2); };
+                const double ang = angp(p3[0], p3[1], p3[2]);
+                double dang[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    int q[3] = {p3[0], p3[1], p3[2]};
+                    q[a] -= 1;
+                    dang[a] = p3[a] > 0 ? p3[a] * angp(q[0], q[1], q[2]) : 0.0;
+                }
+                const double xv = X[row + c], mv = M[row + c];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const double gphi = nrm * (dang[a] * R0 - 2.0 * d[a] * ang * R1);
+                    double hc = 0.0;
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) {
+                        double d2 = 0.0;
+                        int q[3] = {p3[0], p3[1], p3[2]};
+                        if (a == b) {
+                            if (p3[a] >= 2) { q[a] -= 2; d2 = p3[a] * (p3[a] - 1) * angp(q[0], q[1], q[2]); }
+                        } else if (p3[a] > 0 && p3[b] > 0) {
+                            q[a] -= 1; q[b] -= 1;
+                            d2 = p3[a] * p3[b] * angp(q[0], q[1], q[2]);
+                        }
+                        const double h = nrm * (d2 * R0 - 2.0 * (d[b] * dang[a] + d[a] * dang[b] + (a == b ? ang : 0.0)) * R1 +
+                                                4.0 * d[a] * d[b] * ang * R2);
+                        hc += tv[b] * h;
+                    }
+                    acc[a] += -2.0 * mv * gphi - 2.0 * xv * hc;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double v = acc[a];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(&grad[3 * sh.atom + a], v);
+    }
+}
+
+hipError_t launch_xc_grad_points(int G, int nbf, const double* w, const double* phi, const double* dphx,
+                                 const double* dphy, const double* dphz, const double* X, double* A, double* t,
+                                 hipStream_t st) {
+    if (G == 0) return hipSuccess;
+    hipLaunchKernelGGL(xc_grad_point_kernel, dim3((G + 3) / 4), dim3(256), 0, st, G, nbf, w, phi, dphx, dphy, dphz, X,
+                       A, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_xc_grad_accumulate(int G, int nshell, const GridShell* shells, const double* exps, const double* cn,
+                                     const double* pts, int nbf, const double* X, const double* M, const double* t,
+                                     double* grad, hipStream_t st) {
+    if (G == 0 || nshell == 0) return hipSuccess;
+    hipLaunchKernelGGL(xc_grad_kernel, dim3((G + 255) / 256, nshell), dim3(256), 0, st, G, shells, exps, cn, pts, nbf,
+                       X, M, t, grad);
+    return hipGetLastError();
+}
+
 hipError_t launch_basis_on_grid(int G, int nshell, const GridShell* shells, const double* exps, const double* cn,
                                 const double* pts, int nbf, double* phi, double* dphx, double* dphy, double* dphz,
                                 hipStream_t st) {

@@ -38,7 +38,7 @@ EXPORTS = [
     "qcg_build_grid", "qcg_set_grid", "qcg_get_grid", "qcg_xc_eval", "qcg_xc", "qcg_xc_device", "qcg_ks_scf",
     "qcg_eri_gradient", "qcg_one_electron_gradient", "qcg_rhf_gradient",
     "qcg_set_basis_batch", "qcg_batch_build_jk", "qcg_batch_rhf_scf", "qcg_autotune",
-    "qcg_set_tiles", "qcg_get_tiles",
+    "qcg_set_tiles", "qcg_get_tiles", "qcg_xc_gradient", "qcg_ks_gradient",
 ]
 
 
@@ -221,23 +221,30 @@ class Engine:
                                                  ctypes.c_void_p(stream_ptr)))
 
     def autotune(self, dD_ptr: int, iters: int = 2):
-        """Pick the fastest tile per quartet class on a device density; returns [(block, kets/thread)]."""
-        ch = (ctypes.c_int * 42)()
+        """Pick the fastest tile per quartet class and precision part on a device density.
+        Returns [[(block, kets) FP64 part, (block, kets) FP32 part] per class]."""
+        ch = (ctypes.c_int * 84)()
         self._check(self.L.qcg_autotune(self.ctx, ctypes.c_void_p(dD_ptr), int(iters), ch))
-        return [(ch[2 * i], ch[2 * i + 1]) for i in range(21)]
+        return [[(ch[4 * i], ch[4 * i + 1]), (ch[4 * i + 2], ch[4 * i + 3])] for i in range(21)]
 
     def set_tiles(self, tiles):
-        """Set (block, kets) per quartet class; block 0 selects the wave-cooperative kernel."""
+        """Tiles per quartet class: either (block, kets) for both precision parts or a pair
+        [(block, kets) FP64, (block, kets) FP32].  Block 0 selects the wave-cooperative kernel."""
         t = list(tiles)
         if len(t) != 21:
-            raise ValueError("21 (block, kets) tiles expected")
-        arr = (ctypes.c_int * 42)(*[int(v) for bk in t for v in bk])
+            raise ValueError("21 tiles expected")
+        flat = []
+        for x in t:
+            parts = [tuple(x), tuple(x)] if isinstance(x[0], int) else [tuple(x[0]), tuple(x[1])]
+            for bk in parts:
+                flat += [int(bk[0]), int(bk[1])]
+        arr = (ctypes.c_int * 84)(*flat)
         self._check(self.L.qcg_set_tiles(self.ctx, arr))
 
     def get_tiles(self):
-        arr = (ctypes.c_int * 42)()
+        arr = (ctypes.c_int * 84)()
         self._check(self.L.qcg_get_tiles(self.ctx, arr))
-        return [(arr[2 * i], arr[2 * i + 1]) for i in range(21)]
+        return [[(arr[4 * i], arr[4 * i + 1]), (arr[4 * i + 2], arr[4 * i + 3])] for i in range(21)]
 
     def set_profiling(self, on: bool):
         self._check(self.L.qcg_set_profiling(self.ctx, int(bool(on))))
@@ -323,6 +330,23 @@ class Engine:
         eps = np.ascontiguousarray(eps, dtype=np.float64)
         g = np.zeros((self.basis.natom, 3))
         self._check(self.L.qcg_rhf_gradient(self.ctx, _d(D), _d(C), _d(eps), int(n_occ), _d(g)))
+        return g
+
+    def xc_gradient(self, D):
+        """B3LYP XC nuclear gradient on the current grid (xc_gradient_analytical,
+        scf/gradient.rs:1687; no grid-weight response): (natom, 3)."""
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        g = np.zeros((self.basis.natom, 3))
+        self._check(self.L.qcg_xc_gradient(self.ctx, _d(D), _d(g)))
+        return g
+
+    def ks_gradient(self, D, C, eps, n_occ):
+        """B3LYP nuclear gradient (ks_dft_gradient_with_guess, scf/gradient.rs:1214, no D3): (natom, 3)."""
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        C = np.ascontiguousarray(C, dtype=np.float64)
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        g = np.zeros((self.basis.natom, 3))
+        self._check(self.L.qcg_ks_gradient(self.ctx, _d(D), _d(C), _d(eps), int(n_occ), _d(g)))
         return g
 
     # ---- one-electron + SCF

@@ -170,6 +170,27 @@ def basis_on_grid(b, pts):
     return chi, grad
 
 
+def basis_hessian_on_grid(b, pts):
+    """evaluate_basis_hessian_on_grid (scf/gradient.rs:1938): (G, nbf, 6) xx xy xz yy yz zz."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    G = len(pts)
+    hess = np.zeros((G, b.nbf, 6))
+    lib().or_basis_hessian_on_grid(*_basis_args(b), ctypes.c_int64(G), _d(pts), _d(hess))
+    return hess
+
+
+def xc_gradient(b, pts, w, D):
+    """xc_gradient_analytical (scf/gradient.rs:1687), B3LYP, no grid-weight response: (natom, 3)."""
+    chi, grad = basis_on_grid(b, pts)
+    hess = basis_hessian_on_grid(b, pts)
+    g = np.zeros((b.natom, 3))
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    lib().or_xc_gradient(b.nshell, _i(b.shell_l), _i(b.shell_bf_off), _i(np.ascontiguousarray(b.shell_atom, dtype=np.int32)),
+                         b.nbf, ctypes.c_int64(len(w)), _d(w), _d(chi), _d(grad), _d(hess),
+                         _d(np.ascontiguousarray(D, dtype=np.float64)), _d(g))
+    return g
+
+
 def vxc(chi, grad, w, D):
     """build_vxc_gga_matmul (dft/ks_scf.rs:1150) incl. the 512-point block screening: (V, Exc, rho)."""
     G, nbf = chi.shape

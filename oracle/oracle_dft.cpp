@@ -8,6 +8,10 @@
 //   or_b3lyp          dft/functionals.rs:179, :324, :691, :1170, :1625      pinned: libxc vectors
 //   or_vxc            dft/ks_scf.rs:936 (chi D), :965 screening (1e-10 per 512-point block),
 //                     :1028 density, :1150-1296 blocked V_xc and the (V+V^T)/2 symmetrisation
+//   or_basis_hessian_on_grid  scf/gradient.rs:1938-2050 (same screening as the values)
+//   or_xc_gradient    scf/gradient.rs:1687-1910 (xc_gradient_analytical, GGA branch: chi D,
+//                     rho, grad rho, sigma, B3LYP, grad_chi_D, the two accumulation parts)
+//                     pinned: finite differences of E_xc with the grid held fixed (tests)
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -323,6 +327,132 @@ double or_vxc(int nbf, int64_t G, const double* chi, const double* grad, const d
 }  // extern "C"
 
 extern "C" {
+// hess[g*nbf*6 + mu*6 + idx], idx 0 xx, 1 xy, 2 xz, 3 yy, 4 yz, 5 zz (gradient.rs:1938-2050):
+// d2 chi/dr_a dr_b = N e [d2ang - 2 alpha (r_b dang_a + r_a dang_b + delta_ab ang) + 4 alpha^2 r_a r_b ang]
+int or_basis_hessian_on_grid(int nshell, const int* l, const int* nprim, const int* prim_off, const double* exps,
+                             const double* coefs, const double* centers, const int* bf_off, int nbf, int64_t G,
+                             const double* pts, double* hess) {
+    std::memset(hess, 0, sizeof(double) * G * nbf * 6);
+    static const int HA[6] = {0, 0, 0, 1, 1, 2}, HB[6] = {0, 1, 2, 1, 2, 2};
+#pragma omp parallel for schedule(static)
+    for (int64_t g = 0; g < G; ++g)
+        for (int s = 0; s < nshell; ++s) {
+            const double d[3] = {pts[3 * g] - centers[3 * s], pts[3 * g + 1] - centers[3 * s + 1],
+                                 pts[3 * g + 2] - centers[3 * s + 2]};
+            const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            double amin = 1e300;
+            for (int p = 0; p < nprim[s]; ++p) amin = std::fmin(amin, exps[prim_off[s] + p]);
+            if (amin * r2 > 34.5388) continue;
+            int n = 0;
+            for (int i = l[s]; i >= 0; --i)
+                for (int j = l[s] - i; j >= 0; --j, ++n) {
+                    const int pw[3] = {i, j, l[s] - i - j};
+                    auto angp = [&](int a0, int a1, int a2) {
+                        if (a0 < 0 || a1 < 0 || a2 < 0) return 0.0;
+                        return ipow(d[0], a0) * ipow(d[1], a1) * ipow(d[2], a2);
+                    };
+                    const double ang = angp(pw[0], pw[1], pw[2]);
+                    double dang[3];
+                    for (int a = 0; a < 3; ++a) {
+                        int q[3] = {pw[0], pw[1], pw[2]};
+                        q[a] -= 1;
+                        dang[a] = pw[a] > 0 ? pw[a] * angp(q[0], q[1], q[2]) : 0.0;
+                    }
+                    const int mu = bf_off[s] + n;
+                    for (int h = 0; h < 6; ++h) {
+                        const int a = HA[h], b = HB[h];
+                        double d2 = 0.0;
+                        if (a == b) {
+                            if (pw[a] >= 2) {
+                                int q[3] = {pw[0], pw[1], pw[2]};
+                                q[a] -= 2;
+                                d2 = pw[a] * (pw[a] - 1) * angp(q[0], q[1], q[2]);
+                            }
+                        } else if (pw[a] > 0 && pw[b] > 0) {
+                            int q[3] = {pw[0], pw[1], pw[2]};
+                            q[a] -= 1;
+                            q[b] -= 1;
+                            d2 = pw[a] * pw[b] * angp(q[0], q[1], q[2]);
+                        }
+                        double v = 0.0;
+                        for (int p = 0; p < nprim[s]; ++p) {
+                            const double al = exps[prim_off[s] + p];
+                            const double nc = coefs[prim_off[s] + p] * cnorm_grid(al, pw[0], pw[1], pw[2]);
+                            const double e = std::exp(-al * r2);
+                            v += nc * e *
+                                 (d2 - 2.0 * al * (d[b] * dang[a] + d[a] * dang[b] + (a == b ? ang : 0.0)) +
+                                  4.0 * al * al * d[a] * d[b] * ang);
+                        }
+                        hess[(g * nbf + mu) * 6 + h] = v;
+                    }
+                }
+        }
+    return 0;
+}
+
+// xc_gradient_analytical, GGA branch (scf/gradient.rs:1687-1882), B3LYP.  chi/grad/hess from the
+// functions above; shell_atom/bf_off/l give each basis function's atom; grad (natom x 3) is ADDED to.
+int or_xc_gradient(int nshell, const int* l, const int* bf_off, const int* shell_atom, int nbf, int64_t G,
+                   const double* w, const double* chi, const double* gchi, const double* hess, const double* D,
+                   double* grad) {
+    std::vector<int> atom_of(nbf, 0);
+    for (int s = 0; s < nshell; ++s)
+        for (int c = 0; c < (l[s] + 1) * (l[s] + 2) / 2; ++c) atom_of[bf_off[s] + c] = shell_atom[s];
+    int natom = 0;
+    for (int s = 0; s < nshell; ++s) natom = std::max(natom, shell_atom[s] + 1);
+    static const int HIDX[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    std::vector<double> acc((size_t)natom * 3, 0.0);
+    for (int64_t g = 0; g < G; ++g) {
+        const double* c = chi + g * nbf;
+        const double* gc = gchi + g * nbf * 3;
+        std::vector<double> X(nbf, 0.0), GX((size_t)nbf * 3, 0.0);  // chi D, grad_chi_D
+        for (int mu = 0; mu < nbf; ++mu) {
+            double x = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+            for (int nu = 0; nu < nbf; ++nu) {
+                const double dmn = D[(size_t)mu * nbf + nu];
+                x += c[nu] * dmn;
+                gx += dmn * gc[nu * 3];
+                gy += dmn * gc[nu * 3 + 1];
+                gz += dmn * gc[nu * 3 + 2];
+            }
+            X[mu] = x;
+            GX[mu * 3] = gx;
+            GX[mu * 3 + 1] = gy;
+            GX[mu * 3 + 2] = gz;
+        }
+        double r = 0.0, g3[3] = {0, 0, 0};
+        for (int mu = 0; mu < nbf; ++mu) {
+            r += c[mu] * X[mu];
+            for (int d = 0; d < 3; ++d) g3[d] += gc[mu * 3 + d] * X[mu];
+        }
+        const double rho = std::fmax(r, 0.0);
+        const double grho[3] = {2.0 * g3[0], 2.0 * g3[1], 2.0 * g3[2]};
+        const double sigma = grho[0] * grho[0] + grho[1] * grho[1] + grho[2] * grho[2];
+        double exc, vrho, vsig;
+        or_b3lyp(1, &rho, &sigma, &exc, &vrho, &vsig);
+        const double wvr = w[g] * vrho, wvs = w[g] * vsig;
+        if (std::fabs(wvr) < 1e-30 && std::fabs(wvs) < 1e-30) continue;
+        for (int mu = 0; mu < nbf; ++mu) {
+            const int A = atom_of[mu];
+            double aow = wvr * X[mu];
+            if (std::fabs(wvs) > 1e-30)
+                aow += 2.0 * wvs * (grho[0] * GX[mu * 3] + grho[1] * GX[mu * 3 + 1] + grho[2] * GX[mu * 3 + 2]);
+            for (int d = 0; d < 3; ++d) acc[A * 3 + d] += -2.0 * aow * gc[mu * 3 + d];
+            if (std::fabs(wvs) > 1e-30 && std::fabs(X[mu]) > 1e-30) {
+                const double f2 = -4.0 * wvs * X[mu];
+                const double* H = hess + (g * nbf + mu) * 6;
+                for (int d = 0; d < 3; ++d) {
+                    double hc = 0.0;
+                    for (int e = 0; e < 3; ++e) hc += grho[e] * H[HIDX[d][e]];
+                    acc[A * 3 + d] += f2 * hc;
+                }
+            }
+        }
+    }
+    for (int i = 0; i < natom * 3; ++i) grad[i] += acc[i];
+    return 0;
+}
+
 // Single functional component: 0 Slater, 1 VWN5, 2 B88, 3 LYP (vsigma = 0 for the LDA parts).
 int or_xc_component(int which, int64_t n, const double* rho, const double* sigma, double* exc, double* vrho,
                     double* vsigma) {

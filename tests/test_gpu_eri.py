@@ -160,7 +160,7 @@ def test_gpu_jk_tiles_vs_oracle(tile):
     eri, _ = oracle.eri_compressed(b)
     eng = _eng(b)
     eng.set_tiles([tile] * 21)
-    assert eng.get_tiles() == [tile] * 21
+    assert eng.get_tiles() == [[tile, tile]] * 21
     D = random_density(b.nbf, seed=7)
     Jo, Ko = oracle.build_jk(D, eri, b.nbf)
     jk = eng.build_jk(D)

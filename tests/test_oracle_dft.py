@@ -81,3 +81,62 @@ def test_grid_sizes():
     # single atom: no partition, every raw point kept (Mura-Knowles weights are positive)
     pa, wa, _ = oracle.becke_grid(np.array([8]), np.zeros((1, 3)), 75, 0, 0)
     assert len(pa) == 75 * 302
+
+
+# ------------------------------------------------------------------ XC gradient (scf/gradient.rs:1687)
+def _displaced(b, atom, axis, h):
+    """The basis with every shell on `atom` moved by h along `axis` (the grid is NOT moved)."""
+    import copy
+    c = copy.deepcopy(b)
+    sc = np.array(c.shell_center, dtype=float).copy()
+    sc[np.asarray(c.shell_atom) == atom, axis] += h
+    c.shell_center = sc
+    return c
+
+
+def _exc_fixed_grid(b, pts, w, D):
+    chi, g = oracle.basis_on_grid(b, pts)
+    X = chi @ D
+    rho = np.maximum(np.einsum("gm,gm->g", chi, X), 0.0)
+    grho = 2.0 * np.einsum("gmd,gm->gd", g, X)
+    exc, _, _ = oracle.b3lyp(rho, np.einsum("gd,gd->g", grho, grho))
+    return float(np.sum(w * exc * rho))
+
+
+def test_basis_hessian_vs_fd():
+    """d2 chi/dr_a dr_b (gradient.rs:1938) against central differences of the analytic first
+    derivatives (ks_scf.rs:115), for s, p and Cartesian d shells with contracted primitives."""
+    b = bench_molecule("H2O", "6-31g*")
+    rng = np.random.default_rng(3)
+    pts = np.array(b.xyz)[rng.integers(0, b.natom, 40)] + rng.normal(scale=0.8, size=(40, 3))
+    hess = oracle.basis_hessian_on_grid(b, pts)
+    h = 1e-5
+    idx = {(0, 0): 0, (0, 1): 1, (0, 2): 2, (1, 1): 3, (1, 2): 4, (2, 2): 5}
+    for bdir in range(3):
+        dp = pts.copy(); dp[:, bdir] += h
+        dm = pts.copy(); dm[:, bdir] -= h
+        _, gp = oracle.basis_on_grid(b, dp)
+        _, gm = oracle.basis_on_grid(b, dm)
+        fd = (gp - gm) / (2 * h)  # d/dr_b of d chi/dr_a
+        for a in range(3):
+            got = hess[:, :, idx[tuple(sorted((a, bdir)))]]
+            np.testing.assert_allclose(got, fd[:, :, a], atol=2e-7 * max(1.0, np.abs(fd).max()))
+
+
+def test_xc_gradient_vs_fixed_grid_fd():
+    """The analytic XC gradient (no grid-weight response) is the derivative of E_xc with the grid
+    held fixed while the basis functions move with their atoms: central differences, h = 1e-4."""
+    from helpers import random_density
+    b = bench_molecule("H2O", "6-31g*")
+    pts, w, _ = oracle.becke_grid(b.Z, b.xyz, 40, 0, 1)
+    D = random_density(b.nbf, seed=2, nocc=b.n_occ)
+    g = oracle.xc_gradient(b, pts, w, D)
+    h = 1e-4
+    fd = np.zeros_like(g)
+    for a in range(b.natom):
+        for x in range(3):
+            fd[a, x] = (_exc_fixed_grid(_displaced(b, a, x, h), pts, w, D)
+                        - _exc_fixed_grid(_displaced(b, a, x, -h), pts, w, D)) / (2 * h)
+    np.testing.assert_allclose(g, fd, atol=2e-7)
+    # a dropped Hessian term or a wrong sign shows up at the 1e-2 level
+    assert np.abs(fd).max() > 1e-2
