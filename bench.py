@@ -56,6 +56,8 @@ def main():
     ap.add_argument("--fp32-cutoff", type=float, default=1e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-gradient", action="store_true", help="skip the gradient evals/s leg")
+    ap.add_argument("--grad-steps", type=int, default=3)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default tiles instead of autotuning (profiling runs)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -193,6 +195,32 @@ def main():
                "max_abs_F_vs_gpu_fp64": float(np.abs(Fc - F64).max())}
         log(f"cpu baseline: {cpu['value']} builds/s on {nthr} threads")
 
+    # Gradient evals/s: analytic RHF/6-31+G* gradient of n-octane (seeded 0.05 A distortion) at its
+    # converged density and energy-weighted density: ERI adjoint kernels on the GPU, one-electron
+    # and nuclear terms on the host.
+    grad = None
+    if not args.no_gradient:
+        bo = workloads.octane_631pgs()
+        eo = Engine(bo, device=0)
+        so = eo.rhf_scf(ScfConfig.tight())
+        eo.rhf_gradient(so["D"], so["C"], so["eps"], bo.n_occ)  # warm-up
+        ng = max(1, args.grad_steps)
+        t = time.perf_counter()
+        for _ in range(ng):
+            g = eo.rhf_gradient(so["D"], so["C"], so["eps"], bo.n_occ)
+        el_g = time.perf_counter() - t
+        t = time.perf_counter()
+        for _ in range(ng):
+            eo.eri_gradient(so["D"], 1.0)
+        el_e = time.perf_counter() - t
+        grad = {"workload": "RHF/6-31+G* n-octane C8H18 analytic nuclear gradient at the converged density",
+                "nbf": bo.nbf, "nshell": bo.nshell, "evals": ng,
+                "gradient_evals_per_s": round(ng / el_g, 3), "ms_per_gradient": round(1e3 * el_g / ng, 3),
+                "eri_gradient_ms": round(1e3 * el_e / ng, 3),
+                "max_abs_gradient": float(np.abs(g).max()),
+                "translational_sum": float(np.abs(g.sum(axis=0)).max())}
+        log(f"gradient: {grad['ms_per_gradient']} ms/eval (ERI part {grad['eri_gradient_ms']} ms) nbf={bo.nbf}")
+
     out = {
         "metric": "Fock builds/s, mixed vs FP64 on same GPU at |dE|<=1uHa; gradient evals/s",
         "value": round(args.steps / elmx, 3),
@@ -217,6 +245,7 @@ def main():
         "scf_iterations": scf["iterations"],
         "all_kernels_model_tflops": round(tot_fl / (tot_ms * 1e-3) / 1e12, 3) if tot_ms > 0 else None,
         "tiles_mixed": tiles.get("mixed"),
+        "gradient": grad,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
