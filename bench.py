@@ -57,6 +57,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-gradient", action="store_true", help="skip the gradient evals/s leg")
+    ap.add_argument("--no-graph", action="store_true", help="launch every Fock build eagerly")
     ap.add_argument("--grad-steps", type=int, default=3)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default tiles instead of autotuning (profiling runs)")
@@ -107,6 +108,7 @@ def main():
         eng.set_options(prec, args.fp32_cutoff)
         # per-class tile choice for this precision mode
         tiles[prec] = eng.get_tiles() if args.no_autotune else eng.autotune(dD.data_ptr(), 2)
+        eng.set_graph(not args.no_graph)  # one HIP-graph replay per Fock build
         with torch.cuda.stream(stream):
             for _ in range(args.warmup):
                 eng.build_fock_device(dH.data_ptr(), dD.data_ptr(), dF.data_ptr(), 0.5, stream.cuda_stream)
@@ -116,6 +118,7 @@ def main():
                 eng.build_fock_device(dH.data_ptr(), dD.data_ptr(), dF.data_ptr(), 0.5, stream.cuda_stream)
             torch.cuda.synchronize()
             el = time.perf_counter() - t
+        eng.set_graph(False)
         F = dF.cpu().numpy()
         E = 0.5 * float(np.sum(D * (H + F))) + b.e_nuc
         return el, F, E

@@ -120,6 +120,12 @@ int qcg_build_jk_device(qcg_context* ctx, const double* dD, double* dJ, double* 
  * KS, dft/ks_scf.rs:1960: k_factor = a/2). */
 int qcg_build_fock_device(qcg_context* ctx, const double* dH, const double* dD, double k_factor, double* dF,
                           void* stream);
+/* Graph replay for qcg_build_fock_device: on != 0 captures the whole launch sequence of a device
+ * Fock build (fill, class kernels on the auxiliary streams, combine) as one HIP graph the first
+ * time and replays it afterwards; it is captured again when the buffers, stream, k factor,
+ * precision options or tiles change.  Profiling mode always launches eagerly.  No counterpart in
+ * qc-core: launch mode only. */
+int qcg_set_graph(qcg_context* ctx, int on);
 
 /* Per quartet-class statistics of the integral-direct J/K build: integer counts (exact, same
  * screens as the kernels), the algorithmic flop model, and (after qcg_set_profiling(ctx, 1) and

@@ -53,6 +53,17 @@ struct qcg_context {
     double *d_H = nullptr, *d_F = nullptr;
     std::vector<void*> owned;
     std::vector<PrimPair<double>> h_pp;  // host copy of the primitive pairs (flop model counts)
+    // graph replay of the device Fock build (launch sequence captured once per key)
+    bool use_graph = false;
+    hipGraph_t fock_graph = nullptr;
+    hipGraphExec_t fock_exec = nullptr;
+    double fock_key[10] = {};
+    void drop_fock_graph() {
+        if (fock_exec) (void)hipGraphExecDestroy(fock_exec);
+        if (fock_graph) (void)hipGraphDestroy(fock_graph);
+        fock_exec = nullptr;
+        fock_graph = nullptr;
+    }
     // per-class profiling of the J/K build: events around each class launch
     bool profiling = false;
     hipEvent_t ev[2 * 21 * 2 + 2] = {};
@@ -105,6 +116,7 @@ struct qcg_context {
         (void)hipFree(p);
     }
     void free_basis() {
+        drop_fock_graph();
         for (void** p : {(void**)&d_pairs, (void**)&d_pp, (void**)&d_ppf, (void**)&d_D, (void**)&d_J, (void**)&d_K,
                          (void**)&d_Jacc, (void**)&d_Kacc, (void**)&d_H, (void**)&d_F, (void**)&d_mol_nbf,
                          (void**)&d_mat_off, (void**)&d_ket_off, (void**)&d_ket_cnt, (void**)&d_items}) {

@@ -38,14 +38,8 @@ __device__ __forceinline__ void rys_eval_one(R T, const R* __restrict__ tab, int
     const R s = (T - (R)(k * QCG_RYS_H)) * (R)(2.0 / QCG_RYS_H) - R(1);
     const R* cx = tab + qcg_rys_off[N] + k * (2 * N * kRysStride) + r * kRysStride;
     const R* cw = cx + N * kRysStride;
-    R vx = cx[QCG_RYS_DEG], vw = cw[QCG_RYS_DEG];
-#pragma unroll
-    for (int j = QCG_RYS_DEG - 1; j >= 0; --j) {
-        vx = vx * s + cx[j];
-        vw = vw * s + cw[j];
-    }
-    x = vx;
-    w = vw;
+    x = poly13(cx, s);
+    w = poly13(cw, s);
 }
 
 // LDS geometry of one wave for transfer tables with powers up to (TA, TB, TC, TD), N nodes,

@@ -43,22 +43,45 @@ QCG_HD constexpr int cart_y(int l, int c) {
 }
 QCG_HD constexpr int cart_z(int l, int c) { return l - cart_x(l, c) - cart_y(l, c); }
 
-// Reciprocal and reciprocal square root in the arithmetic type R.  FP64: correctly rounded
-// division / sqrt.  FP32 on the device: the hardware v_rcp_f32 / v_rsq_f32 (about 1 ulp), which is
-// all the FP32 quartets of the mixed mode need (their contributions are below 1e-6 x 1e-7).
+// Reciprocal and reciprocal square root in the arithmetic type R, for positive normal arguments
+// (exponent sums, T above the asymptotic switch).  On the device:
+//   FP32: the hardware v_rcp_f32 / v_rsq_f32 (about 1 ulp), all the FP32 quartets of the mixed
+//         mode need (their contributions are below 1e-6 x 1e-7);
+//   FP64: the hardware v_rcp_f64 / v_rsq_f64 estimate refined by two Newton steps
+//         (r <- r + r(1 - x r);  y <- y + y/2 (1 - x y^2)), within about 1 ulp of the correctly
+//         rounded value at a third of the instructions of IEEE division / sqrt.
+// On the host: plain division / sqrt.
 template <typename R>
-QCG_HD inline R qrcp(R x) {
+QCG_HD R qrcp(R x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    if constexpr (sizeof(R) == 4) return __builtin_amdgcn_rcpf(x);
-#endif
+    if constexpr (sizeof(R) == 4) {
+        return __builtin_amdgcn_rcpf(x);
+    } else {
+        R r = __builtin_amdgcn_rcp(x);
+        R e = fma(-x, r, R(1));
+        r = fma(r, e, r);
+        e = fma(-x, r, R(1));
+        return fma(r, e, r);
+    }
+#else
     return R(1) / x;
+#endif
 }
 template <typename R>
-QCG_HD inline R qrsqrt(R x) {
+QCG_HD R qrsqrt(R x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    if constexpr (sizeof(R) == 4) return __builtin_amdgcn_rsqf(x);
-#endif
+    if constexpr (sizeof(R) == 4) {
+        return __builtin_amdgcn_rsqf(x);
+    } else {
+        R y = __builtin_amdgcn_rsq(x);
+        R e = fma(-x * y, y, R(1));
+        y = fma(R(0.5) * y, e, y);
+        e = fma(-x * y, y, R(1));
+        return fma(R(0.5) * y, e, y);
+    }
+#else
     return R(1) / sqrt(x);
+#endif
 }
 
 QCG_HD constexpr double dfact_odd(int n) {  // (2n-1)!! with (-1)!! = 1

@@ -1,6 +1,7 @@
 // Host side of the qcg library: context, basis/pair setup, C ABI (include/qcg.h).
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -80,6 +81,7 @@ void qcg_destroy(qcg_context* c) {
             (void)hipStreamDestroy(c->aux[i]);
         }
     }
+    c->drop_fock_graph();
     if (c->blas) (void)rocblas_destroy_handle(c->blas);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -605,9 +607,43 @@ int qcg_build_fock_device(qcg_context* c, const double* dH, const double* dD, do
     if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    if (c->use_graph && !c->profiling) {
+        // replay the captured launch sequence; capture again when anything it bakes in changed
+        const double key[10] = {(double)(uintptr_t)dH, (double)(uintptr_t)dD, (double)(uintptr_t)dF,
+                                (double)(uintptr_t)st, k_factor, (double)c->tile_sig, (double)c->precision,
+                                c->fp32_cutoff, c->schwarz_thr, 1.0};
+        if (!c->fock_exec || std::memcmp(key, c->fock_key, sizeof(key)) != 0) {
+            c->drop_fock_graph();
+            int rc = prepare_jk(c);
+            if (rc) return rc;
+            HIPCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            rc = run_jk(c, dD, c->d_J, c->d_K, st);
+            hipError_t e = rc ? hipSuccess : launch_fock_combine(dH, c->d_Jacc, c->d_Kacc, dF, c->basis.nbf, k_factor, st);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(st, &g);
+            if (rc || e != hipSuccess || ec != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                (void)hipGetLastError();
+                return rc ? rc : fail(c, QCG_DEVICE, std::string("Fock graph capture: ") +
+                                                         hipGetErrorString(e != hipSuccess ? e : ec));
+            }
+            c->fock_graph = g;
+            HIPCHK(c, hipGraphInstantiate(&c->fock_exec, g, nullptr, nullptr, 0));
+            std::memcpy(c->fock_key, key, sizeof(key));
+        }
+        HIPCHK(c, hipGraphLaunch(c->fock_exec, st));
+        return QCG_OK;
+    }
     int rc = run_jk(c, dD, c->d_J, c->d_K, st);
     if (rc) return rc;
     HIPCHK(c, launch_fock_combine(dH, c->d_Jacc, c->d_Kacc, dF, c->basis.nbf, k_factor, st));
+    return QCG_OK;
+}
+
+int qcg_set_graph(qcg_context* c, int on) {
+    if (!c) return QCG_INVALID_ARG;
+    c->use_graph = on != 0;
+    if (!c->use_graph) c->drop_fock_graph();
     return QCG_OK;
 }
 

@@ -19,6 +19,19 @@ QCG_HD constexpr int rys_tri(int n) { return n * (n - 1) / 2; }
 template <int N>
 constexpr int rys_tab_size() { return qcg_rys_nint[N] * 2 * N * (QCG_RYS_DEG + 1); }
 
+// c[0] + c[1] s + ... + c[13] s^13 in Estrin's scheme: dependency depth 5 instead of Horner's 13
+// (the FP64 FMA latency, not the issue rate, bounds a lone Horner chain per lane).
+template <typename R>
+QCG_HD R poly13(const R* __restrict__ c, R s) {
+    static_assert(QCG_RYS_DEG == 13, "poly13 evaluates the degree-13 fits");
+    const R s2 = s * s, s4 = s2 * s2, s8 = s4 * s4;
+    const R t0 = c[0] + c[1] * s, t1 = c[2] + c[3] * s, t2 = c[4] + c[5] * s, t3 = c[6] + c[7] * s;
+    const R t4 = c[8] + c[9] * s, t5 = c[10] + c[11] * s, t6 = c[12] + c[13] * s;
+    const R u0 = t0 + t1 * s2, u1 = t2 + t3 * s2, u2 = t4 + t5 * s2;
+    const R v0 = u0 + u1 * s4, v1 = u2 + t6 * s4;
+    return v0 + v1 * s8;
+}
+
 // tabN: the coefficients of the N-point rule, i.e. qcg_rys_coef + qcg_rys_off[N] (or its float
 // copy), in device memory or staged in LDS.
 template <typename R, int N>
@@ -39,10 +52,7 @@ QCG_HD void rys_eval_tab(R T, const R* __restrict__ tabN, R* x, R* w) {
     const R* c = tabN + k * (2 * N * kRysStride);
 #pragma unroll
     for (int i = 0; i < 2 * N; ++i) {
-        const R* ci = c + i * kRysStride;
-        R v = ci[QCG_RYS_DEG];
-#pragma unroll
-        for (int j = QCG_RYS_DEG - 1; j >= 0; --j) v = v * s + ci[j];
+        const R v = poly13(c + i * kRysStride, s);
         if (i < N) x[i] = v; else w[i - N] = v;
     }
 }

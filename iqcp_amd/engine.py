@@ -38,7 +38,7 @@ EXPORTS = [
     "qcg_build_grid", "qcg_set_grid", "qcg_get_grid", "qcg_xc_eval", "qcg_xc", "qcg_xc_device", "qcg_ks_scf",
     "qcg_eri_gradient", "qcg_one_electron_gradient", "qcg_rhf_gradient",
     "qcg_set_basis_batch", "qcg_batch_build_jk", "qcg_batch_rhf_scf", "qcg_autotune",
-    "qcg_set_tiles", "qcg_get_tiles", "qcg_xc_gradient", "qcg_ks_gradient",
+    "qcg_set_tiles", "qcg_get_tiles", "qcg_xc_gradient", "qcg_ks_gradient", "qcg_set_graph",
 ]
 
 
@@ -245,6 +245,10 @@ class Engine:
         arr = (ctypes.c_int * 84)()
         self._check(self.L.qcg_get_tiles(self.ctx, arr))
         return [[(arr[4 * i], arr[4 * i + 1]), (arr[4 * i + 2], arr[4 * i + 3])] for i in range(21)]
+
+    def set_graph(self, on: bool):
+        """Replay device Fock builds from a captured HIP graph."""
+        self._check(self.L.qcg_set_graph(self.ctx, int(bool(on))))
 
     def set_profiling(self, on: bool):
         self._check(self.L.qcg_set_profiling(self.ctx, int(bool(on))))
