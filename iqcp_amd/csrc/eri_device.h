@@ -156,27 +156,30 @@ QCG_HD void contract_quartet(const PairRec& P, const PairRec& Q,
     const R Ax = (R)P.ax, Ay = (R)P.ay, Az = (R)P.az;
     const R Cx = (R)Q.ax, Cy = (R)Q.ay, Cz = (R)Q.az;
     const R two_pi_52 = (R)(2.0 * kPi * kPi * 1.7724538509055160273);
-    for (int i = 0; i < P.npp; ++i) {
-        const PrimPair<R> a = pp[P.pp_off + i];
-        const R p = a.p;
-        const R hp = R(0.5) * qrcp(p);
-        for (int j = 0; j < Q.npp; ++j) {
-            const PrimPair<R> c = pp[Q.pp_off + j];
+    // ket primitives outer: in the J/K kernels the bra pair is uniform over the block, so the inner
+    // loop's bra primitive loads are uniform (scalar) and each lane loads its ket primitive once
+    for (int j = 0; j < Q.npp; ++j) {
+        const PrimPair<R> c = pp[Q.pp_off + j];
+        const R q = c.p;
+        const R hq = R(0.5) * qrcp(q);
+        const R QCx = c.px - Cx, QCy = c.py - Cy, QCz = c.pz - Cz;
+        for (int i = 0; i < P.npp; ++i) {
+            const PrimPair<R> a = pp[P.pp_off + i];
             if (a.kab * c.kab < (R)kPrimScreen) continue;
-            const R q = c.p;
-            const R pq_sum = p + q;
-            const R ipq = qrcp(pq_sum);
+            const R p = a.p;
+            const R hp = R(0.5) * qrcp(p);
+            // 1/(p+q) = rs^2 and 1/(pq) = (2 hp)(2 hq): one reciprocal square root per quartet
+            const R rs = qrsqrt(p + q);
+            const R ipq = rs * rs;
             const R PQx = a.px - c.px, PQy = a.py - c.py, PQz = a.pz - c.pz;
             const R rho = p * q * ipq;
             const R T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-            const R pref = two_pi_52 * qrcp(p * q) * qrsqrt(pq_sum) * a.ck * c.ck;
+            const R pref = two_pi_52 * (R(4) * hp * hq) * rs * a.ck * c.ck;
             R rx[N], rw[N];
             if constexpr (TAB_N) rys_eval_tab<R, N>(T, rys_tab, rx, rw);
             else rys_eval<R, N>(T, rys_tab, rx, rw);
-            const R hq = R(0.5) * qrcp(q);
             const R qf = q * ipq, pf = p * ipq;
             const R PAx = a.px - Ax, PAy = a.py - Ay, PAz = a.pz - Az;
-            const R QCx = c.px - Cx, QCy = c.py - Cy, QCz = c.pz - Cz;
 #pragma unroll
             for (int r = 0; r < N; ++r) {
                 const R u = rx[r];

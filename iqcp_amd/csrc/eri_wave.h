@@ -74,15 +74,17 @@ __device__ __forceinline__ void wave_tables(const PairRec& P, const PairRec& Q, 
         const PrimPair<R> a = pp[P.pp_off + i];
         const PrimPair<R> c = pp[Q.pp_off + j];
         const R p = a.p, q = c.p;
-        const R ipq = qrcp(p + q);
+        const R hp = R(0.5) * qrcp(p), hq = R(0.5) * qrcp(q);
+        const R rs = qrsqrt(p + q);
+        const R ipq = rs * rs;
         const R PQx = a.px - c.px, PQy = a.py - c.py, PQz = a.pz - c.pz;
         const R T = p * q * ipq * (PQx * PQx + PQy * PQy + PQz * PQz);
         R x, w;
         rys_eval_one<R, N>(T, rys_tab, r, x, w);
         const R qf = q * ipq, pf = p * ipq;
         const R b00 = R(0.5) * x * ipq;
-        const R b10 = (R(0.5) * qrcp(p)) * (R(1) - x * qf);
-        const R b01 = (R(0.5) * qrcp(q)) * (R(1) - x * pf);
+        const R b10 = hp * (R(1) - x * qf);
+        const R b01 = hq * (R(1) - x * pf);
         R PA, QC, PQ, AB, CD;
         if (axis == 0) {
             PA = a.px - (R)P.ax; QC = c.px - (R)Q.ax; PQ = PQx; AB = (R)P.abx; CD = (R)Q.abx;
@@ -95,7 +97,7 @@ __device__ __forceinline__ void wave_tables(const PairRec& P, const PairRec& Q, 
         vrr2d<NB_, NK_, R>(g, PA - x * qf * PQ, QC + x * pf * PQ, b00, b10, b01);
         R sc = R(1);
         if (axis == 2)
-            sc = (a.kab * c.kab < (R)kPrimScreen) ? R(0) : w * (two_pi_52 * qrcp(p * q) * qrsqrt(p + q) * a.ck * c.ck);
+            sc = (a.kab * c.kab < (R)kPrimScreen) ? R(0) : w * (two_pi_52 * (R(4) * hp * hq) * rs * a.ck * c.ck);
         R* row = tbl + axis * HT * ES + pql * N + r;
         hrr1d_put<TA, TB, TC, TD, R>(g, AB, CD, [&](int h, R v) { row[h * ES] = v * sc; });
     }
