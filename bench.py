@@ -58,6 +58,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-gradient", action="store_true", help="skip the gradient evals/s leg")
     ap.add_argument("--no-graph", action="store_true", help="launch every Fock build eagerly")
+    ap.add_argument("--no-b3lyp", action="store_true", help="skip the (H2O)16 B3LYP J+K+XC leg")
     ap.add_argument("--grad-steps", type=int, default=3)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default tiles instead of autotuning (profiling runs)")
@@ -198,6 +199,43 @@ def main():
                "max_abs_F_vs_gpu_fp64": float(np.abs(Fc - F64).max())}
         log(f"cpu baseline: {cpu['value']} builds/s on {nthr} threads")
 
+    # B3LYP leg: one J+K+XC build of (H2O)16/6-31G* on the literal 75 x 302 unpruned grid at its
+    # converged KS density, all-FP64 and mixed (the XC part is FP64 in both).
+    ks = None
+    if not args.no_b3lyp:
+        bw = workloads.water16_631gs()
+        ew = Engine(bw, device=0)
+        npts = ew.build_grid(75, "standard", False)
+        sw = ew.ks_scf(ScfConfig.tight())
+        dDw = torch.tensor(sw["D"], device=dev)
+        dJw, dKw, dVw = torch.empty_like(dDw), torch.empty_like(dDw), torch.empty_like(dDw)
+        dEw = torch.zeros(1, dtype=torch.float64, device=dev)
+        ks = {"workload": "B3LYP/6-31G* (H2O)16, J + K + V_xc build at the converged KS density",
+              "nbf": bw.nbf, "grid_points": npts, "scf_energy": sw["energy_total"], "scf_iterations": sw["iterations"]}
+        for prec in ("fp64", "mixed"):
+            ew.set_options(prec, args.fp32_cutoff)
+            with torch.cuda.stream(stream):
+                for _ in range(2):
+                    ew.build_jk_device(dDw.data_ptr(), dJw.data_ptr(), dKw.data_ptr(), stream.cuda_stream)
+                    ew.xc_device(dDw.data_ptr(), dVw.data_ptr(), dEw.data_ptr(), stream.cuda_stream)
+                torch.cuda.synchronize()
+                nk = max(1, args.steps // 2)
+                t = time.perf_counter()
+                for _ in range(nk):
+                    ew.build_jk_device(dDw.data_ptr(), dJw.data_ptr(), dKw.data_ptr(), stream.cuda_stream)
+                    ew.xc_device(dDw.data_ptr(), dVw.data_ptr(), dEw.data_ptr(), stream.cuda_stream)
+                torch.cuda.synchronize()
+                el_k = time.perf_counter() - t
+            Dn = sw["D"]
+            Ek = float(np.sum(Dn * (0.5 * dJw.cpu().numpy() - 0.1 * 0.5 * dKw.cpu().numpy()))) + float(dEw.cpu()[0])
+            ks[f"{prec}_ms_per_build"] = round(1e3 * el_k / nk, 3)
+            ks[f"{prec}_two_electron_plus_xc_energy"] = Ek
+        ks["speedup_mixed_vs_fp64"] = round(ks["fp64_ms_per_build"] / ks["mixed_ms_per_build"], 3)
+        ks["dE_mixed_vs_fp64_Ha"] = ks["mixed_two_electron_plus_xc_energy"] - ks["fp64_two_electron_plus_xc_energy"]
+        ew.set_options("fp64")
+        del ew
+        log(f"b3lyp water16: fp64 {ks['fp64_ms_per_build']} ms, mixed {ks['mixed_ms_per_build']} ms per J+K+XC build")
+
     # Gradient evals/s: analytic RHF/6-31+G* gradient of n-octane (seeded 0.05 A distortion) at its
     # converged density and energy-weighted density: ERI adjoint kernels on the GPU, one-electron
     # and nuclear terms on the host.
@@ -248,6 +286,7 @@ def main():
         "scf_iterations": scf["iterations"],
         "all_kernels_model_tflops": round(tot_fl / (tot_ms * 1e-3) / 1e12, 3) if tot_ms > 0 else None,
         "tiles_mixed": tiles.get("mixed"),
+        "b3lyp": ks,
         "gradient": grad,
         "roofline": roofline,
         "cpu_baseline": cpu,
