@@ -49,16 +49,17 @@ int qcg_create(qcg_context** out, int device) {
     for (size_t i = 0; i < nt; ++i) tf[i] = (float)qcg_rys_coef[i];
     if (hipMemcpy(c->d_rys, qcg_rys_coef, nt * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return QCG_DEVICE;
     if (hipMemcpy(c->d_rys_f, tf.data(), nt * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return QCG_DEVICE;
-    // default tiles: several kets per thread for the cheapest classes
     int cls = 0;
     for (int cb = kNumPairClasses - 1; cb >= 0; --cb)
         for (int ck = cb; ck >= 0; --ck, ++cls) {
             const int la = cb == 0 ? 0 : cb <= 2 ? 1 : 2, lb = (cb == 0 || cb == 1 || cb == 3) ? 0 : (cb == 2 || cb == 4) ? 1 : 2;
             const int lc = ck == 0 ? 0 : ck <= 2 ? 1 : 2, ld = (ck == 0 || ck == 1 || ck == 3) ? 0 : (ck == 2 || ck == 4) ? 1 : 2;
             const int nq = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+            // defaults close to what the autotuner picks on alkanes: the wave-cooperative kernel
+            // (block 0) from 27 components on, one ket per thread below
             for (int part = 0; part < 2; ++part) {
-                c->tile[cls][part].block = nq >= 81 ? 0 : 64;  // block 0: wave-cooperative kernel
-                c->tile[cls][part].reps = nq <= 3 ? 4 : nq <= 9 ? 2 : nq >= 81 ? 2 : 1;
+                c->tile[cls][part].block = nq >= 27 ? 0 : (nq <= 3 ? 256 : 64);
+                c->tile[cls][part].reps = nq >= 162 ? 1 : nq >= 27 ? 2 : 1;
             }
             c->class_nq[cls] = nq;
         }

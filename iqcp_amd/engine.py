@@ -405,6 +405,13 @@ class BatchEngine:
         self.nbf = [b.nbf for b in bases]
         self.offsets = np.cumsum([0] + [n * n for n in self.nbf])
 
+    def set_tiles(self, tiles):
+        """Tiles per quartet class, as Engine.set_tiles."""
+        Engine.set_tiles(self, tiles)
+
+    def get_tiles(self):
+        return Engine.get_tiles(self)
+
     def _check(self, rc):
         if rc:
             raise QcgError(rc, self.L.qcg_last_error(self.ctx).decode())

@@ -20,6 +20,21 @@ def test_batch_jk_equals_single():
         np.testing.assert_allclose(K, jk.K, atol=1e-12, rtol=0)
 
 
+@pytest.mark.parametrize("tile", [(0, 1), (256, 2)])
+def test_batch_jk_tiles_equal_single(tile):
+    """Batched J/K through the wave-cooperative kernel (block 0) and a wide per-lane tile: work
+    items of different molecules share the launches."""
+    bases = [bench_molecule(m, b) for m, b in (("NH3", "6-31+g*"), ("H2O", "6-31g*"), ("CH4", "6-31g*"))]
+    be = BatchEngine(bases)
+    be.set_tiles([tile] * 21)
+    Ds = [random_density(b.nbf, seed=20 + i, nocc=b.n_occ) for i, b in enumerate(bases)]
+    Js, Ks = be.build_jk(Ds)
+    for b, D, J, K in zip(bases, Ds, Js, Ks):
+        jk = Engine(b).build_jk(D)
+        np.testing.assert_allclose(J, jk.J, atol=1e-12, rtol=0)
+        np.testing.assert_allclose(K, jk.K, atol=1e-12, rtol=0)
+
+
 @pytest.mark.parametrize("use_graph", [True, False])
 def test_batch_scf_equals_single(use_graph):
     bases = [bench_molecule(m, b) for m, b in (("H2O", "6-31g*"), ("CH4", "6-31g"), ("HF", "6-31g"))]

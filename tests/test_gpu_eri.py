@@ -203,6 +203,28 @@ def test_gpu_fock_graph_replay():
             eng.set_graph(False)
 
 
+def test_gpu_autotune_keeps_parity():
+    """After the autotuner picks a tile per (class, precision part), J/K still match the oracle,
+    and every chosen tile is one of the candidates."""
+    import torch
+    b = bench_molecule("NH3", "6-31+g*")
+    eri, _ = oracle.eri_compressed(b)
+    eng = _eng(b)
+    D = random_density(b.nbf, seed=11)
+    dD = torch.tensor(D, device=torch.device("cuda:0"))
+    cands = {(64, 1), (64, 2), (64, 4), (128, 1), (256, 1), (64, 8), (0, 1), (0, 2), (0, 4)}
+    Jo, Ko = oracle.build_jk(D, eri, b.nbf)
+    for prec in ("fp64", "mixed"):
+        eng.set_options(prec, 1e-6)
+        tiles = eng.autotune(dD.data_ptr(), 1)
+        assert len(tiles) == 21 and all(tuple(t) in cands for pair in tiles for t in pair)
+        assert eng.get_tiles() == [[tuple(t) for t in pair] for pair in tiles]
+        jk = eng.build_jk(D)
+        tol = 1e-11 if prec == "fp64" else 1e-8
+        np.testing.assert_allclose(jk.J, Jo, atol=tol, rtol=0)
+        np.testing.assert_allclose(jk.K, Ko, atol=tol, rtol=0)
+
+
 def test_gpu_set_tiles_rejects_bad_tiles():
     b = bench_molecule("H2O", "sto-3g")
     eng = _eng(b)
