@@ -11,6 +11,7 @@
 //   per node for all (i,j,k,l) powers instead of once per Cartesian component, then
 //   I(abcd) += Ix Iy Iz summed over nodes and primitive quartets.
 #pragma once
+#include <utility>
 #include "qcg_types.h"
 #include "rys.h"
 
@@ -43,6 +44,12 @@ __host__ __device__ constexpr double kCompNorm[3][6] = {
     {1.0, 0, 0, 0, 0, 0},
     {1.0, 1.0, 1.0, 0, 0, 0},
     {0.57735026918962576451, 1.0, 1.0, 0.57735026918962576451, 1.0, 0.57735026918962576451}};
+
+// kCompNorm[l][c] as arithmetic on (l, c): folds to a constant when both are constants after
+// unrolling, and needs no memory access when they are not.
+QCG_HD double cnorm(int l, int c) {
+    return (l == 2 && (c == 0 || c == 3 || c == 5)) ? 0.57735026918962576451 : 1.0;
+}
 
 // Per-component indices into the 1-D transfer tables.
 template <int LA, int LB, int LC, int LD>
@@ -128,13 +135,24 @@ QCG_HD void hrr1d(const R* g, R ab, R cd, R* out) {
     hrr1d_put<LA, LB, LC, LD, R>(g, ab, cd, [&](int h, R v) { out[h] = v; });
 }
 
-// Run body(i) for i in [0, N): fully unrolled when the class is small enough to live in
-// registers, a plain loop otherwise (large classes keep their arrays in scratch).
+// body(std::integral_constant<int, I>{}) for I in [0, N): the index is a constant expression in
+// the body, so table lookups with it fold at compile time instead of becoming indexed register
+// moves or select chains.
+template <typename F, int... Is>
+QCG_HD void static_for_impl(F&& f, std::integer_sequence<int, Is...>) {
+    (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, typename F>
+QCG_HD void static_for(F&& f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// Run body(i) for i in [0, N): fully unrolled (constant i) when the class is small enough to live
+// in registers, a plain loop otherwise (large classes keep their arrays in scratch).
 template <bool UNROLL, int N, typename F>
 QCG_HD void for_n(F&& body) {
     if constexpr (UNROLL) {
-#pragma unroll
-        for (int i = 0; i < N; ++i) body(i);
+        static_for<N>([&](auto ic) { body((int)decltype(ic)::value); });
     } else {
 #pragma unroll 1
         for (int i = 0; i < N; ++i) body(i);
@@ -198,8 +216,18 @@ QCG_HD void contract_quartet(const PairRec& P, const PairRec& Q,
                 const R wp = rw[r] * pref;
 #pragma unroll
                 for (int h = 0; h < S::H4; ++h) iz[h] *= wp;
-                constexpr const CompTab<LA, LB, LC, LD>& ct = kCompTab<LA, LB, LC, LD>;
-                for_n<S::kUnroll, S::NQ>([&](int k) { eri[k] += ix[ct.hx[k]] * iy[ct.hy[k]] * iz[ct.hz[k]]; });
+                if constexpr (S::kUnroll) {
+                    static_for<S::NQ>([&](auto kc) {
+                        constexpr int k = decltype(kc)::value;
+                        constexpr int hx = kCompTab<LA, LB, LC, LD>.hx[k];
+                        constexpr int hy = kCompTab<LA, LB, LC, LD>.hy[k];
+                        constexpr int hz = kCompTab<LA, LB, LC, LD>.hz[k];
+                        eri[k] += ix[hx] * iy[hy] * iz[hz];
+                    });
+                } else {
+                    constexpr const CompTab<LA, LB, LC, LD>& ct = kCompTab<LA, LB, LC, LD>;
+                    for_n<false, S::NQ>([&](int k) { eri[k] += ix[ct.hx[k]] * iy[ct.hy[k]] * iz[ct.hz[k]]; });
+                }
             }
         }
     }

@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void grad_wave_kernel(const int4* __restrict__
             const int d = k % ND, c = (k / ND) % NC, b = (k / NCD) % NB, a = k / (NCD * NB);
             Wl[k] = 4.0 * deg *
                     (Dab[a * NB + b] * Dcd[c * ND + d] - xfac * (Dac[a * NC + c] * Dbd[b * ND + d] + Dad[a * ND + d] * Dbc[b * NC + c])) *
-                    kCompNorm[LA][a] * kCompNorm[LB][b] * kCompNorm[LC][c] * kCompNorm[LD][d];
+                    cnorm(LA, a) * cnorm(LB, b) * cnorm(LC, c) * cnorm(LD, d);
         }
         double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // A xyz, B xyz, C xyz
         const int npq = P.npp * Q.npp;

@@ -74,7 +74,7 @@ __global__ __launch_bounds__(256) void jk_kernel(
     double Dps[S::NAB], jp[S::NAB];
     for_n<U, S::NAB>([&](int ab) {
         const int a = ab / S::NB, b = ab % S::NB;
-        Dps[ab] = D[(size_t)(P.bfa + a) * nbf + P.bfb + b] * kCompNorm[LA][a] * kCompNorm[LB][b];
+        Dps[ab] = D[(size_t)(P.bfa + a) * nbf + P.bfb + b] * cnorm(LA, a) * cnorm(LB, b);
         jp[ab] = 0.0;
     });
     const double deg_p = (P.sa == P.sb) ? 0.5 : 1.0;
@@ -90,20 +90,20 @@ __global__ __launch_bounds__(256) void jk_kernel(
             double Dqs[S::NCD];
             for_n<U, S::NCD>([&](int cd) {
                 const int c = cd / S::ND, d = cd % S::ND;
-                Dqs[cd] = D[(size_t)(Q.bfa + c) * nbf + Q.bfb + d] * kCompNorm[LC][c] * kCompNorm[LD][d];
+                Dqs[cd] = D[(size_t)(Q.bfa + c) * nbf + Q.bfb + d] * cnorm(LC, c) * cnorm(LD, d);
             });
             const double fj = 2.0 * deg;
             for_n<U, S::NAB>([&](int ab) {
                 double s = 0.0;
                 for_n<U, S::NCD>([&](int cd) { s += Dqs[cd] * (double)eri[ab * S::NCD + cd]; });
                 const int a = ab / S::NB, b = ab % S::NB;
-                jp[ab] += fj * kCompNorm[LA][a] * kCompNorm[LB][b] * s;
+                jp[ab] += fj * cnorm(LA, a) * cnorm(LB, b) * s;
             });
             for_n<U, S::NCD>([&](int cd) {
                 double s = 0.0;
                 for_n<U, S::NAB>([&](int ab) { s += Dps[ab] * (double)eri[ab * S::NCD + cd]; });
                 const int c = cd / S::ND, d = cd % S::ND;
-                atomicAdd(&Jacc[(size_t)(Q.bfa + c) * nbf + Q.bfb + d], fj * kCompNorm[LC][c] * kCompNorm[LD][d] * s);
+                atomicAdd(&Jacc[(size_t)(Q.bfa + c) * nbf + Q.bfb + d], fj * cnorm(LC, c) * cnorm(LD, d) * s);
             });
         }
         if (do_k) {
@@ -112,19 +112,19 @@ __global__ __launch_bounds__(256) void jk_kernel(
             double Dbd[S::NB * S::ND], Dad[S::NA * S::ND], Dbc[S::NB * S::NC], Dac[S::NA * S::NC];
             for_n<U, S::NB * S::ND>([&](int i) {
                 const int b = i / S::ND, d = i % S::ND;
-                Dbd[i] = D[(size_t)(P.bfb + b) * nbf + Q.bfb + d] * kCompNorm[LB][b] * kCompNorm[LD][d];
+                Dbd[i] = D[(size_t)(P.bfb + b) * nbf + Q.bfb + d] * cnorm(LB, b) * cnorm(LD, d);
             });
             for_n<U, S::NA * S::ND>([&](int i) {
                 const int a = i / S::ND, d = i % S::ND;
-                Dad[i] = D[(size_t)(P.bfa + a) * nbf + Q.bfb + d] * kCompNorm[LA][a] * kCompNorm[LD][d];
+                Dad[i] = D[(size_t)(P.bfa + a) * nbf + Q.bfb + d] * cnorm(LA, a) * cnorm(LD, d);
             });
             for_n<U, S::NB * S::NC>([&](int i) {
                 const int b = i / S::NC, c = i % S::NC;
-                Dbc[i] = D[(size_t)(P.bfb + b) * nbf + Q.bfa + c] * kCompNorm[LB][b] * kCompNorm[LC][c];
+                Dbc[i] = D[(size_t)(P.bfb + b) * nbf + Q.bfa + c] * cnorm(LB, b) * cnorm(LC, c);
             });
             for_n<U, S::NA * S::NC>([&](int i) {
                 const int a = i / S::NC, c = i % S::NC;
-                Dac[i] = D[(size_t)(P.bfa + a) * nbf + Q.bfa + c] * kCompNorm[LA][a] * kCompNorm[LC][c];
+                Dac[i] = D[(size_t)(P.bfa + a) * nbf + Q.bfa + c] * cnorm(LA, a) * cnorm(LC, c);
             });
 #define VIDX(a, b, c, d) ((((a) * S::NB + (b)) * S::NC + (c)) * S::ND + (d))
             for_n<U, S::NA * S::NC>([&](int ac) {
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void jk_kernel(
                     const int b = bd / S::ND, d = bd % S::ND;
                     s += Dbd[bd] * (double)eri[VIDX(a, b, c, d)];
                 });
-                kadd(false, a, Q.bfa + c, deg * kCompNorm[LA][a] * kCompNorm[LC][c] * s);
+                kadd(false, a, Q.bfa + c, deg * cnorm(LA, a) * cnorm(LC, c) * s);
             });
             for_n<U, S::NB * S::NC>([&](int bc) {
                 const int b = bc / S::NC, c = bc % S::NC;
@@ -143,7 +143,7 @@ __global__ __launch_bounds__(256) void jk_kernel(
                     const int a = ad / S::ND, d = ad % S::ND;
                     s += Dad[ad] * (double)eri[VIDX(a, b, c, d)];
                 });
-                kadd(true, b, Q.bfa + c, deg * kCompNorm[LB][b] * kCompNorm[LC][c] * s);
+                kadd(true, b, Q.bfa + c, deg * cnorm(LB, b) * cnorm(LC, c) * s);
             });
             for_n<U, S::NA * S::ND>([&](int ad) {
                 const int a = ad / S::ND, d = ad % S::ND;
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256) void jk_kernel(
                     const int b = bc / S::NC, c = bc % S::NC;
                     s += Dbc[bc] * (double)eri[VIDX(a, b, c, d)];
                 });
-                kadd(false, a, Q.bfb + d, deg * kCompNorm[LA][a] * kCompNorm[LD][d] * s);
+                kadd(false, a, Q.bfb + d, deg * cnorm(LA, a) * cnorm(LD, d) * s);
             });
             for_n<U, S::NB * S::ND>([&](int bd) {
                 const int b = bd / S::ND, d = bd % S::ND;
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void jk_kernel(
                     const int a = ac / S::NC, c = ac % S::NC;
                     s += Dac[ac] * (double)eri[VIDX(a, b, c, d)];
                 });
-                kadd(true, b, Q.bfb + d, deg * kCompNorm[LB][b] * kCompNorm[LD][d] * s);
+                kadd(true, b, Q.bfb + d, deg * cnorm(LB, b) * cnorm(LD, d) * s);
             });
 #undef VIDX
         }
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(256) void jk_wave_kernel(
             const int k = lane + 64 * j;
             if (k < S::NQ) {
                 const int d = k % ND, c = (k / ND) % NC, b = (k / NCD) % NB, a = k / (NCD * NB);
-                V[k] = (double)acc[j] * (kCompNorm[LA][a] * kCompNorm[LB][b] * kCompNorm[LC][c] * kCompNorm[LD][d] * deg);
+                V[k] = (double)acc[j] * (cnorm(LA, a) * cnorm(LB, b) * cnorm(LC, c) * cnorm(LD, d) * deg);
             }
         }
         if (do_j)
@@ -373,7 +373,7 @@ __global__ __launch_bounds__(256) void eri_tensor_kernel(
             if (diag && q == pidx && ij < kl) continue;
             const int64_t hi = ij > kl ? ij : kl, lo = ij > kl ? kl : ij;
             out[hi * (hi + 1) / 2 + lo] =
-                eri[idx] * kCompNorm[LA][a] * kCompNorm[LB][b] * kCompNorm[LC][c] * kCompNorm[LD][d];
+                eri[idx] * cnorm(LA, a) * cnorm(LB, b) * cnorm(LC, c) * cnorm(LD, d);
         }
     }
 }

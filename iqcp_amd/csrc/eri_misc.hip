@@ -21,7 +21,7 @@ __global__ __launch_bounds__(128) void schwarz_kernel(const PairRec* __restrict_
 #pragma unroll 1
     for (int k = 0; k < S::NQ; ++k) {
         const int d = k % S::NB, c = (k / S::NB) % S::NA, b = (k / S::NAB) % S::NB, a = k / (S::NAB * S::NB);
-        const double nrm = kCompNorm[LA][a] * kCompNorm[LB][b] * kCompNorm[LA][c] * kCompNorm[LB][d];
+        const double nrm = cnorm(LA, a) * cnorm(LB, b) * cnorm(LA, c) * cnorm(LB, d);
         mx = fmax(mx, fabs(eri[k] * nrm));
     }
     Q[i] = sqrt(mx);
