@@ -179,6 +179,143 @@ __global__ __launch_bounds__(256) void grad_wave_kernel(const int4* __restrict__
     }
 }
 
+// ------------------------------------------------------------------ per-lane form (light classes)
+// For classes with at most kGradLaneMaxNQ components one ket quartet per thread is cheaper than a
+// wavefront per quartet (which would leave most lanes idle in phase B).  Every table index below
+// is a compile-time constant (static_for), so the transfer tables stay in registers with no
+// indexed register access.
+constexpr int kGradLaneMaxNQ = 18;
+constexpr int kGradLaneBlock = 128;
+
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void grad_quartet_lane(const PairRec& P, const PairRec& Q,
+                                                  const PrimPair<double>* __restrict__ pp,
+                                                  const double* __restrict__ rys_n, const double* W, double* gq) {
+    using S = EriShape<LA, LB, LC, LD>;
+    constexpr int EA = LA + 1, EB = LB + 1, EC = LC + 1, ED = LD;
+    constexpr int NB_ = EA + EB, NK_ = EC + ED;
+    constexpr int GE = (NB_ + 1) * (NK_ + 1);
+    constexpr int HE = (EA + 1) * (EB + 1) * (EC + 1) * (ED + 1);
+    constexpr int N = (LA + LB + LC + LD + 1) / 2 + 1;
+    constexpr int sI = (EB + 1) * (EC + 1) * (ED + 1), sJ = (EC + 1) * (ED + 1), sK = ED + 1;
+    const double two_pi_52 = 2.0 * kPi * kPi * 1.7724538509055160273;
+    const double AB[3] = {P.abx, P.aby, P.abz}, CD[3] = {Q.abx, Q.aby, Q.abz};
+    for (int j = 0; j < Q.npp; ++j) {
+        const PrimPair<double> c = pp[Q.pp_off + j];
+        const double q = c.p, hq = 0.5 * qrcp(q), al_c = c.pad0;
+        const double QC[3] = {c.px - Q.ax, c.py - Q.ay, c.pz - Q.az};
+        for (int i = 0; i < P.npp; ++i) {
+            const PrimPair<double> a = pp[P.pp_off + i];
+            if (a.kab * c.kab < kPrimScreen) continue;
+            const double p = a.p, hp = 0.5 * qrcp(p), al_a = a.pad0, al_b = a.pad1;
+            const double rs = qrsqrt(p + q), ipq = rs * rs;
+            const double PQ[3] = {a.px - c.px, a.py - c.py, a.pz - c.pz};
+            const double T = p * q * ipq * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+            const double pref = two_pi_52 * (4.0 * hp * hq) * rs * a.ck * c.ck;
+            const double PA[3] = {a.px - P.ax, a.py - P.ay, a.pz - P.az};
+            double rx[N], rw[N];
+            rys_eval_tab<double, N>(T, rys_n, rx, rw);
+            const double qf = q * ipq, pf = p * ipq;
+#pragma unroll
+            for (int r = 0; r < N; ++r) {
+                const double u = rx[r];
+                const double b00 = 0.5 * u * ipq, b10 = hp * (1.0 - u * qf), b01 = hq * (1.0 - u * pf);
+                double I[3][HE];
+#pragma unroll
+                for (int x = 0; x < 3; ++x) {
+                    double g[GE];
+                    vrr2d<NB_, NK_, double>(g, PA[x] - u * qf * PQ[x], QC[x] + u * pf * PQ[x], b00, b10, b01);
+                    hrr1d<EA, EB, EC, ED, double>(g, AB[x], CD[x], I[x]);
+                }
+                double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                static_for<S::NQ>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    constexpr int d = k % S::ND, cc = (k / S::ND) % S::NC, b = (k / S::NCD) % S::NB, aa = k / (S::NCD * S::NB);
+                    constexpr int ia[3] = {cart_x(LA, aa), cart_y(LA, aa), cart_z(LA, aa)};
+                    constexpr int ib[3] = {cart_x(LB, b), cart_y(LB, b), cart_z(LB, b)};
+                    constexpr int ic[3] = {cart_x(LC, cc), cart_y(LC, cc), cart_z(LC, cc)};
+                    constexpr int id[3] = {cart_x(LD, d), cart_y(LD, d), cart_z(LD, d)};
+                    constexpr int h0[3] = {((ia[0] * (EB + 1) + ib[0]) * (EC + 1) + ic[0]) * (ED + 1) + id[0],
+                                           ((ia[1] * (EB + 1) + ib[1]) * (EC + 1) + ic[1]) * (ED + 1) + id[1],
+                                           ((ia[2] * (EB + 1) + ib[2]) * (EC + 1) + ic[2]) * (ED + 1) + id[2]};
+                    const double w = W[k];
+                    const double i0[3] = {I[0][h0[0]], I[1][h0[1]], I[2][h0[2]]};
+                    const double pr[3] = {i0[1] * i0[2], i0[0] * i0[2], i0[0] * i0[1]};
+                    static_for<3>([&](auto xc) {
+                        constexpr int x = decltype(xc)::value;
+                        double dA = 2.0 * al_a * I[x][h0[x] + sI];
+                        if constexpr (ia[x] > 0) dA -= ia[x] * I[x][h0[x] - sI];
+                        double dB = 2.0 * al_b * I[x][h0[x] + sJ];
+                        if constexpr (ib[x] > 0) dB -= ib[x] * I[x][h0[x] - sJ];
+                        double dC = 2.0 * al_c * I[x][h0[x] + sK];
+                        if constexpr (ic[x] > 0) dC -= ic[x] * I[x][h0[x] - sK];
+                        acc[x] += w * dA * pr[x];
+                        acc[3 + x] += w * dB * pr[x];
+                        acc[6 + x] += w * dC * pr[x];
+                    });
+                });
+                const double wp = rw[r] * pref;
+#pragma unroll
+                for (int t = 0; t < 9; ++t) gq[t] += wp * acc[t];
+            }
+        }
+    }
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(kGradLaneBlock) void grad_lane_kernel(
+    const int4* __restrict__ items, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket, int diag,
+    const PrimPair<double>* __restrict__ pp, const double* __restrict__ rys_tab, const double* __restrict__ D,
+    int nbf_, double xfac, int natom, double* __restrict__ grad) {
+    using S = EriShape<LA, LB, LC, LD>;
+    constexpr int N = (LA + LB + LC + LD + 1) / 2 + 1;
+    constexpr int RT = rys_tab_size<N>();
+    __shared__ double rys_s[RT];
+    extern __shared__ double gl[];  // natom x 3: ket-atom sums of the block
+    const int tid = threadIdx.x;
+    for (int i = tid; i < RT; i += blockDim.x) rys_s[i] = rys_tab[qcg_rys_off[N] + i];
+    for (int i = tid; i < 3 * natom; i += blockDim.x) gl[i] = 0.0;
+    __syncthreads();
+    const size_t nbf = (size_t)nbf_;
+    const int4 it = items[blockIdx.x];
+    const PairRec P = bra[it.x];
+    double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0};
+    const int q = it.y + tid;
+    if (q < it.z) {
+        const PairRec Q = ket[q];
+        const double deg = ((P.sa == P.sb) ? 0.5 : 1.0) * ((Q.sa == Q.sb) ? 0.5 : 1.0) * ((diag && q == P.pos) ? 0.5 : 1.0);
+        double W[S::NQ];
+        static_for<S::NQ>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            constexpr int d = k % S::ND, c = (k / S::ND) % S::NC, b = (k / S::NCD) % S::NB, a = k / (S::NCD * S::NB);
+            const size_t i_ = P.bfa + a, j_ = P.bfb + b, k_ = Q.bfa + c, l_ = Q.bfb + d;
+            W[k] = 4.0 * deg *
+                   (D[i_ * nbf + j_] * D[k_ * nbf + l_] - xfac * (D[i_ * nbf + k_] * D[j_ * nbf + l_] + D[i_ * nbf + l_] * D[j_ * nbf + k_])) *
+                   (cnorm(LA, a) * cnorm(LB, b) * cnorm(LC, c) * cnorm(LD, d));
+        });
+        double gq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        grad_quartet_lane<LA, LB, LC, LD>(P, Q, pp, rys_s, W, gq);
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+            gA[x] = gq[x];
+            gB[x] = gq[3 + x];
+            atomicAdd(&gl[3 * Q.atom_a + x], gq[6 + x]);
+            atomicAdd(&gl[3 * Q.atom_b + x], -(gq[x] + gq[3 + x] + gq[6 + x]));  // centre D
+        }
+    }
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        const double ax = wave_sum(gA[x]), bx = wave_sum(gB[x]);
+        if ((tid & 63) == 0) {
+            atomicAdd(&gl[3 * P.atom_a + x], ax);
+            atomicAdd(&gl[3 * P.atom_b + x], bx);
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < 3 * natom; i += blockDim.x)
+        if (gl[i] != 0.0) atomicAdd(&grad[i], gl[i]);
+}
+
 #define QCG_LA(c) ((c) == 0 ? 0 : (c) <= 2 ? 1 : 2)
 #define QCG_LB(c) ((c) == 0 || (c) == 1 || (c) == 3 ? 0 : ((c) == 2 || (c) == 4) ? 1 : 2)
 
@@ -187,8 +324,15 @@ static hipError_t launch_grad_t(const GradLaunch& a, hipStream_t st) {
     constexpr int LA = QCG_LA(CB), LB = QCG_LB(CB), LC = QCG_LA(CK), LD = QCG_LB(CK);
     using G = GradWave<LA, LB, LC, LD>;
     if (a.nitems <= 0) return hipSuccess;
-    hipLaunchKernelGGL((grad_wave_kernel<LA, LB, LC, LD>), dim3((a.nitems + G::WPB - 1) / G::WPB), dim3(64 * G::WPB),
-                       0, st, a.items, a.nitems, a.bra, a.ket, a.diag, a.pp, a.rys, a.D, a.nbf, a.xfac, a.grad);
+    if constexpr (EriShape<LA, LB, LC, LD>::NQ <= kGradLaneMaxNQ) {
+        hipLaunchKernelGGL((grad_lane_kernel<LA, LB, LC, LD>), dim3(a.nitems), dim3(kGradLaneBlock),
+                           3 * (size_t)a.natom * sizeof(double), st, a.items, a.bra, a.ket, a.diag, a.pp, a.rys, a.D,
+                           a.nbf, a.xfac, a.natom, a.grad);
+    } else {
+        hipLaunchKernelGGL((grad_wave_kernel<LA, LB, LC, LD>), dim3((a.nitems + G::WPB - 1) / G::WPB),
+                           dim3(64 * G::WPB), 0, st, a.items, a.nitems, a.bra, a.ket, a.diag, a.pp, a.rys, a.D, a.nbf,
+                           a.xfac, a.grad);
+    }
     return hipGetLastError();
 }
 

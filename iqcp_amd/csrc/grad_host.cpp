@@ -12,8 +12,8 @@ using namespace qcg;
 namespace {
 
 // Work items of the gradient kernels: every bra pair's surviving kets (the exact screening
-// predicate Q_P Q_Q >= threshold, per contraction bucket, as for J/K) cut into ranges of kGradKets.
-constexpr int kGradKets = 4;
+// predicate Q_P Q_Q >= threshold, per contraction bucket, as for J/K) cut into ranges of
+// grad_item_kets(class pair) kets.
 
 int eri_grad(qcg_context* c, const double* dD, double hf_frac, double* dgrad, hipStream_t st) {
     std::vector<int4> items;
@@ -24,7 +24,8 @@ int eri_grad(qcg_context* c, const double* dD, double hf_frac, double* dgrad, hi
             off[cls] = (int)items.size();
             for (int p = 0; p < c->ket_cnt_h[cb]; ++p)
                 for_ket_ranges(c, cb, ck, 0, p, c->schwarz_thr, c->schwarz_thr, [&](int bs, int, int e) {
-                    for (int q0 = bs; q0 < e; q0 += kGradKets) items.push_back(make_int4(p, q0, std::min(q0 + kGradKets, e), 0));
+                    const int ch = grad_item_kets(cb, ck);
+                    for (int q0 = bs; q0 < e; q0 += ch) items.push_back(make_int4(p, q0, std::min(q0 + ch, e), 0));
                 });
             cnt[cls] = (int)items.size() - off[cls];
         }

@@ -180,10 +180,17 @@ void nuclear_repulsion_gradient(const HostBasis& B, double* grad) {
 // overlap_derivative_matrix :188 and hcore_derivative_matrix :355 contracted as in rhf_gradient :930-950).
 // Basis-function derivatives: d/dA_x phi_i = 2 alpha phi_(i+1) - i phi_(i-1); for V the operator
 // centre C gets -(dA + dB) by translational invariance of each three-centre integral.
-void one_electron_gradient(const HostBasis& B, const double* D, const double* W, double* grad) {
+void one_electron_gradient(const HostBasis& B, const double* D, const double* W, double* grad_out) {
     const int nbf = B.nbf;
     const double* rys_tab = qcg_rys_coef;
     constexpr int LD = kMaxL + 4;
+    const int n3 = 3 * B.natom;
+    // shell pairs spread over the host threads; each thread sums into its own gradient copy
+#pragma omp parallel
+    {
+    std::vector<double> gloc(n3, 0.0);
+    double* grad = gloc.data();
+#pragma omp for schedule(dynamic, 1) collapse(2)
     for (int si = 0; si < B.nshell; ++si)
         for (int sj = 0; sj < B.nshell; ++sj) {
             const int la = B.l[si], lb = B.l[sj];
@@ -292,6 +299,9 @@ void one_electron_gradient(const HostBasis& B, const double* D, const double* W,
                         }
                 }
         }
+#pragma omp critical
+    for (int t = 0; t < n3; ++t) grad_out[t] += gloc[t];
+    }
 }
 
 }  // namespace qcg
