@@ -73,6 +73,8 @@ struct qcg_context {
     struct Tile { int block = 64, reps = 1; };
     Tile tile[21][2];  // per quartet class and precision part (0 FP64, 1 FP32)
     int class_nq[21] = {};  // Cartesian components per quartet class
+    // mixed precision: run the FP64 and FP32 items of a class in one launch (jk_*_mp kernels)
+    bool fuse_mixed = true;
     std::vector<int4> items;
     int item_off[21][2] = {}, item_cnt[21][2] = {};
     int4* d_items = nullptr;
@@ -89,12 +91,14 @@ struct qcg_context {
     std::vector<double> grid_xyz, grid_w;
     std::vector<int> grid_atom;
     double *d_pts = nullptr, *d_w = nullptr, *d_phi = nullptr, *d_dphx = nullptr, *d_dphy = nullptr,
-           *d_dphz = nullptr, *d_X = nullptr, *d_aow = nullptr, *d_V = nullptr, *d_exc = nullptr;
+           *d_dphz = nullptr, *d_X = nullptr, *d_aow = nullptr, *d_V = nullptr, *d_exc = nullptr,
+           *d_Vparts = nullptr;  // split-K partial products of V = Phi^T aow
+    int v_chunks = 0;
     GridShell* d_gshell = nullptr;
     double *d_gexp = nullptr, *d_gcn = nullptr;
     void free_grid() {
         for (void** p : {(void**)&d_pts, (void**)&d_w, (void**)&d_phi, (void**)&d_dphx, (void**)&d_dphy,
-                         (void**)&d_dphz, (void**)&d_X, (void**)&d_aow, (void**)&d_V, (void**)&d_exc,
+                         (void**)&d_dphz, (void**)&d_X, (void**)&d_aow, (void**)&d_V, (void**)&d_exc, (void**)&d_Vparts,
                          (void**)&d_gshell, (void**)&d_gexp, (void**)&d_gcn}) {
             release(*p);
             *p = nullptr;

@@ -184,7 +184,7 @@ __global__ __launch_bounds__(256) void grad_wave_kernel(const int4* __restrict__
 // wavefront per quartet (which would leave most lanes idle in phase B).  Every table index below
 // is a compile-time constant (static_for), so the transfer tables stay in registers with no
 // indexed register access.
-constexpr int kGradLaneMaxNQ = 18;
+constexpr int kGradLaneMaxNQ = 27;
 constexpr int kGradLaneBlock = 128;
 
 template <int LA, int LB, int LC, int LD>

@@ -32,28 +32,34 @@
 
 namespace qcg {
 
+// The Rys coefficients of order N for a block: staged in LDS (lds, at least rys_tab_size<N>()
+// elements) when N <= 3, where the per-lane kernels gather them at lane-dependent intervals once
+// per primitive quartet, else read from device memory.  The caller's __syncthreads() follows.
+template <typename R, int N>
+__device__ __forceinline__ const R* rys_stage(const R* __restrict__ rys_tab, R* lds) {
+    if constexpr (N <= 3) {
+        for (int i = threadIdx.x; i < rys_tab_size<N>(); i += blockDim.x) lds[i] = rys_tab[qcg_rys_off[N] + i];
+        return lds;
+    } else {
+        return rys_tab + qcg_rys_off[N];
+    }
+}
+template <int N>
+constexpr int rys_lds_doubles() { return N <= 3 ? rys_tab_size<N>() : 1; }
+
 // One work item = (bra pair, contiguous range of its surviving kets); the host builds the items
-// with the exact screening predicate, so no quartet is tested twice or skipped.
+// with the exact screening predicate, so no quartet is tested twice or skipped.  rys_n: the
+// coefficients of this class's Rys order (rys_stage); kl: krow != 0: the four K terms of the item
+// (all in the rows of the bra functions a and b) are summed in LDS rows kl[a][*], kl[NA + b][*]
+// and flushed once per block.
 template <int LA, int LB, int LC, int LD, typename R>
-__global__ __launch_bounds__(256) void jk_kernel(
-    const int4* __restrict__ items, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket, int diag,
-    const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_tab, const double* __restrict__ D,
+__device__ __forceinline__ void jk_item(
+    const int4 it, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket, int diag,
+    const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_n, const double* __restrict__ D,
     double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
-    const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf, int krow) {
+    const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf, int krow, double* kl) {
     using S = EriShape<LA, LB, LC, LD>;
     constexpr bool U = S::kUnroll;
-    // krow != 0: the four K terms of the item (all in the rows of the bra functions a and b) are
-    // summed in LDS rows kl[a][*], kl[NA + b][*] and flushed once per block
-    extern __shared__ double kl[];
-    // the class's Rys coefficients staged in LDS (node/weight fits are gathered per primitive
-    // quartet at lane-dependent intervals: LDS serves those gathers far faster than L1)
-    constexpr int NR = S::NROOTS;
-    constexpr bool kRysLds = NR <= 3;
-    constexpr int RT = kRysLds ? rys_tab_size<NR>() : 1;
-    __shared__ R rys_s[RT];
-    if constexpr (kRysLds)
-        for (int i = threadIdx.x; i < RT; i += blockDim.x) rys_s[i] = rys_tab[qcg_rys_off[NR] + i];
-    const int4 it = items[blockIdx.x];
     const PairRec P = bra[it.x];
     const int pidx = P.pos;
     ket += ket_off[P.mol];
@@ -83,8 +89,7 @@ __global__ __launch_bounds__(256) void jk_kernel(
         const PairRec Q = ket[q];
         R eri[S::NQ];
         for_n<U, S::NQ>([&](int i) { eri[i] = R(0); });
-        if constexpr (kRysLds) contract_quartet<LA, LB, LC, LD, R, true>(P, Q, pp, rys_s, eri);
-        else contract_quartet<LA, LB, LC, LD, R>(P, Q, pp, rys_tab, eri);
+        contract_quartet<LA, LB, LC, LD, R, true>(P, Q, pp, rys_n, eri);
         const double deg = deg_p * ((Q.sa == Q.sb) ? 0.5 : 1.0) * ((diag && q == pidx) ? 0.5 : 1.0);
         if (do_j) {
             double Dqs[S::NCD];
@@ -191,6 +196,43 @@ __global__ __launch_bounds__(256) void jk_kernel(
     }
 }
 
+template <int LA, int LB, int LC, int LD, typename R>
+__global__ __launch_bounds__(256) void jk_kernel(
+    const int4* __restrict__ items, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket, int diag,
+    const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_tab, const double* __restrict__ D,
+    double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
+    const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf, int krow) {
+    constexpr int NR = EriShape<LA, LB, LC, LD>::NROOTS;
+    extern __shared__ double kl[];
+    __shared__ double rys_raw[rys_lds_doubles<NR>()];
+    const R* rys_n = rys_stage<R, NR>(rys_tab, reinterpret_cast<R*>(rys_raw));
+    jk_item<LA, LB, LC, LD, R>(items[blockIdx.x], bra, ket, diag, pp, rys_n, D, Jacc, Kacc, do_j, do_k, ket_off,
+                               mat_off, mol_nbf, krow, kl);
+}
+
+// Mixed precision in one launch: items [0, n64) are FP64 quartets, the rest FP32 (one branch per
+// block); saves the second launch per class and its tail.
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void jk_kernel_mp(
+    const int4* __restrict__ items, int n64, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket,
+    int diag, const PrimPair<double>* __restrict__ pp, const PrimPair<float>* __restrict__ ppf,
+    const double* __restrict__ rys_tab, const float* __restrict__ rys_tabf, const double* __restrict__ D,
+    double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
+    const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf, int krow) {
+    constexpr int NR = EriShape<LA, LB, LC, LD>::NROOTS;
+    extern __shared__ double kl[];
+    __shared__ double rys_raw[rys_lds_doubles<NR>()];
+    if ((int)blockIdx.x < n64) {
+        const double* rys_n = rys_stage<double, NR>(rys_tab, rys_raw);
+        jk_item<LA, LB, LC, LD, double>(items[blockIdx.x], bra, ket, diag, pp, rys_n, D, Jacc, Kacc, do_j, do_k,
+                                        ket_off, mat_off, mol_nbf, krow, kl);
+    } else {
+        const float* rys_n = rys_stage<float, NR>(rys_tabf, reinterpret_cast<float*>(rys_raw));
+        jk_item<LA, LB, LC, LD, float>(items[blockIdx.x], bra, ket, diag, ppf, rys_n, D, Jacc, Kacc, do_j, do_k,
+                                       ket_off, mat_off, mol_nbf, krow, kl);
+    }
+}
+
 // Wave-cooperative J/K for the classes with many components (eri_wave.h): four wavefronts per
 // block, each takes one work item (bra pair, ket range) and walks its kets one quartet at a time.
 // Digestion of a contracted quartet (phase C): the quartet v (norms and symmetry factor applied)
@@ -209,9 +251,10 @@ struct JkWave {
     static constexpr bool kEnabled = S::NQ >= 27;
 };
 
+// One work item on one wavefront; V: the wave's LDS region (JkWave<..., R>::WAVE_D doubles).
 template <int LA, int LB, int LC, int LD, typename R>
-__global__ __launch_bounds__(256) void jk_wave_kernel(
-    const int4* __restrict__ items, int nitems, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket,
+__device__ __forceinline__ void jk_wave_item(
+    const int4 it, int lane, double* V, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket,
     int diag, const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_tab, const double* __restrict__ D,
     double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
     const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf) {
@@ -220,11 +263,6 @@ __global__ __launch_bounds__(256) void jk_wave_kernel(
     using TB = typename W::T;
     constexpr int N = S::NROOTS, HT = TB::HT, ES = TB::ES, KPL = W::KPL;
     constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND, NCD = S::NCD;
-    __shared__ double smem[4 * W::WAVE_D];
-    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int item = blockIdx.x * 4 + wid;
-    if (item >= nitems) return;  // no block-wide barriers below: waves are independent
-    double* V = smem + wid * W::WAVE_D;
     R* tbl = reinterpret_cast<R*>(V);
     double* Dab = V + W::VD;
     double* Dcd = Dab + S::NAB;
@@ -233,7 +271,6 @@ __global__ __launch_bounds__(256) void jk_wave_kernel(
     double* Dbc = Dad + NA * ND;
     double* Dac = Dbc + NB * NC;
 
-    const int4 it = items[item];
     const PairRec P = bra[it.x];
     const int pidx = P.pos;
     ket += ket_off[P.mol];
@@ -340,6 +377,43 @@ __global__ __launch_bounds__(256) void jk_wave_kernel(
     if (do_j && lane < S::NAB && jp != 0.0) atomicAdd(&Jacc[(P.bfa + lane / NB) * nbf + P.bfb + lane % NB], jp);
 }
 
+template <int LA, int LB, int LC, int LD, typename R>
+__global__ __launch_bounds__(256) void jk_wave_kernel(
+    const int4* __restrict__ items, int nitems, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket,
+    int diag, const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_tab, const double* __restrict__ D,
+    double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
+    const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf) {
+    constexpr int WD = JkWave<LA, LB, LC, LD, R>::WAVE_D;
+    __shared__ double smem[4 * WD];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int item = blockIdx.x * 4 + wid;
+    if (item >= nitems) return;  // no block-wide barriers in the item: waves are independent
+    jk_wave_item<LA, LB, LC, LD, R>(items[item], lane, smem + wid * WD, bra, ket, diag, pp, rys_tab, D, Jacc, Kacc,
+                                    do_j, do_k, ket_off, mat_off, mol_nbf);
+}
+
+// Mixed precision in one launch: items [0, n64) FP64, the rest FP32 (one branch per wavefront).
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void jk_wave_kernel_mp(
+    const int4* __restrict__ items, int nitems, int n64, const PairRec* __restrict__ bra,
+    const PairRec* __restrict__ ket, int diag, const PrimPair<double>* __restrict__ pp,
+    const PrimPair<float>* __restrict__ ppf, const double* __restrict__ rys_tab, const float* __restrict__ rys_tabf,
+    const double* __restrict__ D, double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k,
+    const int* __restrict__ ket_off, const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf) {
+    constexpr int WD64 = JkWave<LA, LB, LC, LD, double>::WAVE_D, WD32 = JkWave<LA, LB, LC, LD, float>::WAVE_D;
+    constexpr int WD = WD64 > WD32 ? WD64 : WD32;
+    __shared__ double smem[4 * WD];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int item = blockIdx.x * 4 + wid;
+    if (item >= nitems) return;
+    if (item < n64)
+        jk_wave_item<LA, LB, LC, LD, double>(items[item], lane, smem + wid * WD, bra, ket, diag, pp, rys_tab, D, Jacc,
+                                             Kacc, do_j, do_k, ket_off, mat_off, mol_nbf);
+    else
+        jk_wave_item<LA, LB, LC, LD, float>(items[item], lane, smem + wid * WD, bra, ket, diag, ppf, rys_tabf, D, Jacc,
+                                            Kacc, do_j, do_k, ket_off, mat_off, mol_nbf);
+}
+
 __device__ __forceinline__ int64_t pair_index64(int64_t i, int64_t j) {
     return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
 }
@@ -383,11 +457,19 @@ __global__ __launch_bounds__(256) void eri_tensor_kernel(
 
 constexpr size_t kKrowMaxBytes = 32768;
 
+// Classes that have a per-lane kernel: fewer than 27 Cartesian components (the wave-cooperative
+// kernel is faster from there on, and the per-lane form of the larger classes is register-starved).
+template <typename S>
+inline constexpr bool kLaneForm = S::NQ < 27;
+
 template <int CB, int CK, typename R>
 static hipError_t launch_jk_t(const JkLaunch& a, hipStream_t st) {
     constexpr int LA = QCG_LA(CB), LB = QCG_LB(CB), LC = QCG_LA(CK), LD = QCG_LB(CK);
+    using S = EriShape<LA, LB, LC, LD>;
     if (a.nitems <= 0) return hipSuccess;
-    if (a.block == 0) {  // wave-cooperative mode: four items per block
+    // wave-cooperative mode (four items per block); classes with 27 or more components always
+    // take it (their per-lane form is register-starved or lives in scratch)
+    if (a.block == 0 || !kLaneForm<S>) {
         if constexpr (JkWave<LA, LB, LC, LD, R>::kEnabled) {
             hipLaunchKernelGGL((jk_wave_kernel<LA, LB, LC, LD, R>), dim3((a.nitems + 3) / 4), dim3(256), 0, st,
                                a.items, a.nitems, a.bra, a.ket, a.diag, (const PrimPair<R>*)a.pp, (const R*)a.rys,
@@ -395,14 +477,49 @@ static hipError_t launch_jk_t(const JkLaunch& a, hipStream_t st) {
             return hipGetLastError();
         }
     }
-    // K rows of the bra functions in LDS when they fit (a.nbf is the largest molecule's nbf)
+    if constexpr (kLaneForm<S>) {
+        // K rows of the bra functions in LDS when they fit (a.nbf is the largest molecule's nbf)
+        const size_t krow_bytes = (size_t)(S::NA + S::NB) * a.nbf * sizeof(double);
+        const int krow = (a.do_k && krow_bytes <= kKrowMaxBytes) ? 1 : 0;
+        hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, R>), dim3(a.nitems), dim3(a.block > 0 ? a.block : 64),
+                           krow ? krow_bytes : 0, st, a.items, a.bra, a.ket, a.diag, (const PrimPair<R>*)a.pp,
+                           (const R*)a.rys, a.D, a.Jacc, a.Kacc, a.do_j, a.do_k, a.ket_off, a.mat_off, a.mol_nbf,
+                           krow);
+        return hipGetLastError();
+    } else {
+        return hipErrorInvalidValue;  // not reached: large classes take the wave-cooperative form above
+    }
+}
+
+// Mixed precision, both parts in one launch: a.items holds a.n64 FP64 items then the FP32 items,
+// all cut for the same tile (a.block); a.pp/a.rys are the FP64 data, a.ppf/a.rysf the FP32 copies.
+template <int CB, int CK>
+static hipError_t launch_jk_mp_t(const JkLaunch& a, hipStream_t st) {
+    constexpr int LA = QCG_LA(CB), LB = QCG_LB(CB), LC = QCG_LA(CK), LD = QCG_LB(CK);
     using S = EriShape<LA, LB, LC, LD>;
-    const size_t krow_bytes = (size_t)(S::NA + S::NB) * a.nbf * sizeof(double);
-    const int krow = (a.do_k && krow_bytes <= kKrowMaxBytes) ? 1 : 0;
-    hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, R>), dim3(a.nitems), dim3(a.block > 0 ? a.block : 64),
-                       krow ? krow_bytes : 0, st, a.items, a.bra, a.ket, a.diag, (const PrimPair<R>*)a.pp,
-                       (const R*)a.rys, a.D, a.Jacc, a.Kacc, a.do_j, a.do_k, a.ket_off, a.mat_off, a.mol_nbf, krow);
-    return hipGetLastError();
+    if (a.nitems <= 0) return hipSuccess;
+    // classes with 27 or more components always take the wave-cooperative form
+    if (a.block == 0 || !kLaneForm<S>) {
+        if constexpr (JkWave<LA, LB, LC, LD, double>::kEnabled) {
+            hipLaunchKernelGGL((jk_wave_kernel_mp<LA, LB, LC, LD>), dim3((a.nitems + 3) / 4), dim3(256), 0, st,
+                               a.items, a.nitems, a.n64, a.bra, a.ket, a.diag, (const PrimPair<double>*)a.pp,
+                               (const PrimPair<float>*)a.ppf, (const double*)a.rys, (const float*)a.rysf, a.D, a.Jacc,
+                               a.Kacc, a.do_j, a.do_k, a.ket_off, a.mat_off, a.mol_nbf);
+            return hipGetLastError();
+        }
+    }
+    if constexpr (kLaneForm<S>) {
+        const size_t krow_bytes = (size_t)(S::NA + S::NB) * a.nbf * sizeof(double);
+        const int krow = (a.do_k && krow_bytes <= kKrowMaxBytes) ? 1 : 0;
+        hipLaunchKernelGGL((jk_kernel_mp<LA, LB, LC, LD>), dim3(a.nitems), dim3(a.block > 0 ? a.block : 64),
+                           krow ? krow_bytes : 0, st, a.items, a.n64, a.bra, a.ket, a.diag,
+                           (const PrimPair<double>*)a.pp, (const PrimPair<float>*)a.ppf, (const double*)a.rys,
+                           (const float*)a.rysf, a.D, a.Jacc, a.Kacc, a.do_j, a.do_k, a.ket_off, a.mat_off, a.mol_nbf,
+                           krow);
+        return hipGetLastError();
+    } else {
+        return hipErrorInvalidValue;  // not reached: large classes take the wave-cooperative form above
+    }
 }
 
 template <int CB, int CK>
@@ -420,7 +537,7 @@ static hipError_t launch_tensor_t(const JkLaunch& a, double* out, hipStream_t st
 hipError_t QCG_CAT(launch_jk_cb, QCG_CB)(int ck, bool fp32, const JkLaunch& a, hipStream_t st) {
     switch (ck) {
 #define QCG_K(CK) \
-    case CK: if constexpr (CK <= QCG_CB) return fp32 ? launch_jk_t<QCG_CB, CK, float>(a, st) : launch_jk_t<QCG_CB, CK, double>(a, st); break;
+    case CK: if constexpr (CK <= QCG_CB) return a.n64 >= 0 ? launch_jk_mp_t<QCG_CB, CK>(a, st) : fp32 ? launch_jk_t<QCG_CB, CK, float>(a, st) : launch_jk_t<QCG_CB, CK, double>(a, st); break;
         QCG_K(0) QCG_K(1) QCG_K(2) QCG_K(3) QCG_K(4) QCG_K(5)
 #undef QCG_K
     }

@@ -31,6 +31,11 @@ struct JkLaunch {
     const int4* items;
     int nitems;
     int block;
+    // mixed precision in one launch (n64 >= 0): the first n64 items are FP64, the rest FP32 with
+    // the FP32 primitive pairs ppf and Rys table rysf; n64 < 0: one precision, given by the caller
+    int n64 = -1;
+    const void* ppf = nullptr;
+    const void* rysf = nullptr;
 };
 
 hipError_t launch_jk_cb0(int ck, bool fp32, const JkLaunch& a, hipStream_t st);
@@ -93,7 +98,7 @@ struct GradLaunch {
 // per item for the light classes (per-lane kernel), a few kets per wavefront otherwise.
 inline int grad_item_kets(int cb, int ck) {
     auto nc = [](int c) { return c == 0 ? 1 : c == 1 ? 3 : c == 2 ? 9 : c == 3 ? 6 : c == 4 ? 18 : 36; };
-    return nc(cb) * nc(ck) <= 18 ? 128 : 4;
+    return nc(cb) * nc(ck) <= 27 ? 128 : 4;
 }
 
 hipError_t launch_grad_cb0(int ck, const GradLaunch& a, hipStream_t st);

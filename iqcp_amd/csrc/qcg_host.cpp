@@ -344,7 +344,8 @@ static int build_plan(qcg_context* c) {
     for (int cb = kNumPairClasses - 1; cb >= 0; --cb)
         for (int ck = cb; ck >= 0; --ck, ++cls) {
             for (int part = 0; part < 2; ++part) {
-                const qcg_context::Tile& t = c->tile[cls][part];
+                // fused mixed launches run both parts with the FP64 part's tile
+                const qcg_context::Tile& t = c->tile[cls][(mixed && c->fuse_mixed) ? 0 : part];
                 const int ch = std::max(1, t.block > 0 ? t.block * t.reps : t.reps);
                 c->item_off[cls][part] = (int)c->items.size();
                 if (part == 1 && !mixed) {
@@ -411,6 +412,10 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
         for (int ck = cb; ck >= 0; --ck, ++cls) {
             for (int part = 0; part < 2; ++part) {
                 if (part == 1 && !mixed) continue;
+                // mixed precision: both parts of the class in one launch (the plan cut the FP32
+                // items with the FP64 part's tile and put them right after the FP64 items)
+                if (part == 1 && c->fuse_mixed) continue;
+                const bool fused = mixed && c->fuse_mixed;
                 JkLaunch a{};
                 a.bra = c->d_pairs + c->class_off[cb];
                 a.ket = c->d_pairs + c->class_off[ck];
@@ -430,6 +435,12 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
                 a.block = c->tile[cls][part].block;
                 a.pp = part ? (const void*)c->d_ppf : (const void*)c->d_pp;
                 a.rys = part ? (const void*)c->d_rys_f : (const void*)c->d_rys;
+                if (fused) {
+                    a.nitems = c->item_cnt[cls][0] + c->item_cnt[cls][1];
+                    a.n64 = c->item_cnt[cls][0];
+                    a.ppf = c->d_ppf;
+                    a.rysf = c->d_rys_f;
+                }
                 hipStream_t s = fork ? c->aux[launched % qcg_context::kStreams] : st;
                 if (a.nitems > 0) ++launched;
                 if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4 * cls + 2 * part], s));
@@ -460,9 +471,11 @@ int qcg_autotune(qcg_context* c, const double* dD, int iters, int* chosen) {
     // (wave-cooperative kernel, classes with at least 27 components only)
     const int cand[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {256, 1}, {64, 8}, {0, 1}, {0, 2}, {0, 4}};
     const int ncand = sizeof(cand) / sizeof(cand[0]);
+    // classes with fewer than 27 components have only the per-lane kernel, the others only the
+    // wave-cooperative one: map each candidate onto what the class has
     auto tile_of = [&](int k, int cls) {
-        return (cand[k][0] == 0 && c->class_nq[cls] < 27) ? qcg_context::Tile{64, 1}
-                                                          : qcg_context::Tile{cand[k][0], cand[k][1]};
+        if (c->class_nq[cls] < 27) return cand[k][0] == 0 ? qcg_context::Tile{64, 1} : qcg_context::Tile{cand[k][0], cand[k][1]};
+        return cand[k][0] == 0 ? qcg_context::Tile{0, cand[k][1]} : qcg_context::Tile{0, std::min(8, cand[k][1] * 2)};
     };
     std::vector<double> best(42, 1e300);
     std::vector<int> pick(42, 0);

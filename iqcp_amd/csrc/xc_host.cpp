@@ -25,6 +25,8 @@ int ensure_blas(qcg_context* c) {
     return QCG_OK;
 }
 
+constexpr int64_t kVChunk = 16384;
+
 int upload_grid(qcg_context* c) {
     const HostBasis& B = c->basis;
     c->free_grid();
@@ -41,6 +43,11 @@ int upload_grid(qcg_context* c) {
     HIPCHK(c, c->alloc(&c->d_X, gn));
     HIPCHK(c, c->alloc(&c->d_aow, gn));
     HIPCHK(c, c->alloc(&c->d_V, (size_t)B.nbf * B.nbf));
+    // V = Phi^T aow has a long inner dimension (the grid): split it into chunks of about
+    // kVChunk points, one strided-batched GEMM over the chunks plus a sum, so the GEMM has
+    // chunks x tiles workgroups instead of a handful of tiles with a 10^6-long K loop
+    c->v_chunks = (int)std::max<int64_t>(1, G / kVChunk);
+    HIPCHK(c, c->alloc(&c->d_Vparts, (size_t)c->v_chunks * B.nbf * B.nbf));
     HIPCHK(c, c->alloc(&c->d_exc, 1));
     HIPCHK(c, hipMemcpy(c->d_pts, c->grid_xyz.data(), 3 * (size_t)G * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_w, c->grid_w.data(), (size_t)G * sizeof(double), hipMemcpyHostToDevice));
@@ -94,10 +101,21 @@ int xc_build(qcg_context* c, const double* dD, double* dV, double* dExc, hipStre
     HIPCHK(c, launch_fill(dExc, 1, 0.0, st));
     HIPCHK(c, launch_xc_points((int)G, nbf, c->d_w, c->d_phi, c->d_dphx, c->d_dphy, c->d_dphz, c->d_X, c->d_aow, dExc,
                                nullptr, st));
-    // V (column-major nbf x nbf) = Phi^T (nbf x G) * aow^T^T
-    if (rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose, nbf, nbf, G, &one, c->d_phi, nbf,
-                      c->d_aow, nbf, &zero, c->d_V, nbf) != rocblas_status_success)
-        return fail(c, QCG_DEVICE, "rocblas_dgemm (Phi^T aow)");
+    // V (column-major nbf x nbf) = Phi^T (nbf x G) * aow^T^T, split over grid chunks: chunk k
+    // covers points [k Gc, (k+1) Gc), the last one also the remainder
+    const int nch = c->v_chunks;
+    const rocblas_int Gc = G / nch, Glast = G - (nch - 1) * Gc;
+    const rocblas_stride sg = (rocblas_stride)Gc * nbf, sv = (rocblas_stride)nbf * nbf;
+    if (nch > 1 &&
+        rocblas_dgemm_strided_batched(c->blas, rocblas_operation_none, rocblas_operation_transpose, nbf, nbf, Gc, &one,
+                                      c->d_phi, nbf, sg, c->d_aow, nbf, sg, &zero, c->d_Vparts, nbf, sv,
+                                      nch - 1) != rocblas_status_success)
+        return fail(c, QCG_DEVICE, "rocblas_dgemm_strided_batched (Phi^T aow)");
+    if (rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose, nbf, nbf, Glast, &one,
+                      c->d_phi + (size_t)(nch - 1) * sg, nbf, c->d_aow + (size_t)(nch - 1) * sg, nbf, &zero,
+                      c->d_Vparts + (size_t)(nch - 1) * sv, nbf) != rocblas_status_success)
+        return fail(c, QCG_DEVICE, "rocblas_dgemm (Phi^T aow, last chunk)");
+    HIPCHK(c, launch_sum_parts(c->d_Vparts, nch, (size_t)nbf * nbf, c->d_V, st));
     HIPCHK(c, launch_sym_half(c->d_V, dV, nbf, st));
     return QCG_OK;
 }

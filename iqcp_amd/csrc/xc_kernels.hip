@@ -116,6 +116,19 @@ __global__ void xc_eval_kernel(int n, const double* __restrict__ rho, const doub
     xc_b3lyp(rho[i], sigma[i], exc[i], vrho[i], vsigma[i]);
 }
 
+__global__ void sum_parts_kernel(const double* __restrict__ parts, int nparts, size_t n, double* __restrict__ out) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int k = 0; k < nparts; ++k) s += parts[(size_t)k * n + i];
+    out[i] = s;
+}
+
+hipError_t launch_sum_parts(const double* parts, int nparts, size_t n, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(sum_parts_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, parts, nparts, n, out);
+    return hipGetLastError();
+}
+
 __global__ void sym_half_kernel(const double* __restrict__ V, double* __restrict__ out, int n) {
     const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;

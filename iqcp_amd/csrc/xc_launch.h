@@ -19,6 +19,8 @@ hipError_t launch_xc_points(int G, int nbf, const double* w, const double* phi, 
 hipError_t launch_xc_eval(int n, const double* rho, const double* sigma, double* exc, double* vrho, double* vsigma,
                           hipStream_t st);
 hipError_t launch_sym_half(const double* V, double* out, int n, hipStream_t st);
+// out[i] = sum_k parts[k n + i]
+hipError_t launch_sum_parts(const double* parts, int nparts, size_t n, double* out, hipStream_t st);
 // XC nuclear gradient: per-point A rows and t = 2 w vsigma grad rho, then the per-(shell, point)
 // accumulation into grad (natom x 3) from X = Phi D and M = A D.
 hipError_t launch_xc_grad_points(int G, int nbf, const double* w, const double* phi, const double* dphx,

@@ -212,7 +212,7 @@ def test_gpu_autotune_keeps_parity():
     eng = _eng(b)
     D = random_density(b.nbf, seed=11)
     dD = torch.tensor(D, device=torch.device("cuda:0"))
-    cands = {(64, 1), (64, 2), (64, 4), (128, 1), (256, 1), (64, 8), (0, 1), (0, 2), (0, 4)}
+    cands = {(64, 1), (64, 2), (64, 4), (128, 1), (256, 1), (64, 8), (0, 1), (0, 2), (0, 4), (0, 8)}
     Jo, Ko = oracle.build_jk(D, eri, b.nbf)
     for prec in ("fp64", "mixed"):
         eng.set_options(prec, 1e-6)
