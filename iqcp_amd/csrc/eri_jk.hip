@@ -457,19 +457,22 @@ __global__ __launch_bounds__(256) void eri_tensor_kernel(
 
 constexpr size_t kKrowMaxBytes = 32768;
 
-// Classes that have a per-lane kernel: fewer than 27 Cartesian components (the wave-cooperative
-// kernel is faster from there on, and the per-lane form of the larger classes is register-starved).
-template <typename S>
-inline constexpr bool kLaneForm = S::NQ < 27;
+// Classes that have a per-lane kernel: up to 54 Cartesian components (above that the
+// wave-cooperative kernel always wins and the per-lane form is register-starved).  One exception:
+// hipcc of ROCm 7.2 stops with "Illegal instruction detected: Operand has incorrect register
+// class" on the FP32 per-lane (dd|ss) kernel; that class runs FP32 quartets in the wave form (the
+// fused mixed kernel, which holds both precisions, compiles and keeps the per-lane form).
+template <typename S, typename R = double>
+inline constexpr bool kLaneForm =
+    S::NQ <= 54 && !(sizeof(R) == 4 && S::NQ == 36 && S::NA == 6 && S::NB == 6);
 
 template <int CB, int CK, typename R>
 static hipError_t launch_jk_t(const JkLaunch& a, hipStream_t st) {
     constexpr int LA = QCG_LA(CB), LB = QCG_LB(CB), LC = QCG_LA(CK), LD = QCG_LB(CK);
     using S = EriShape<LA, LB, LC, LD>;
     if (a.nitems <= 0) return hipSuccess;
-    // wave-cooperative mode (four items per block); classes with 27 or more components always
-    // take it (their per-lane form is register-starved or lives in scratch)
-    if (a.block == 0 || !kLaneForm<S>) {
+    // wave-cooperative mode (four items per block); classes without a per-lane form always take it
+    if (a.block == 0 || !kLaneForm<S, R>) {
         if constexpr (JkWave<LA, LB, LC, LD, R>::kEnabled) {
             hipLaunchKernelGGL((jk_wave_kernel<LA, LB, LC, LD, R>), dim3((a.nitems + 3) / 4), dim3(256), 0, st,
                                a.items, a.nitems, a.bra, a.ket, a.diag, (const PrimPair<R>*)a.pp, (const R*)a.rys,
@@ -477,7 +480,7 @@ static hipError_t launch_jk_t(const JkLaunch& a, hipStream_t st) {
             return hipGetLastError();
         }
     }
-    if constexpr (kLaneForm<S>) {
+    if constexpr (kLaneForm<S, R>) {
         // K rows of the bra functions in LDS when they fit (a.nbf is the largest molecule's nbf)
         const size_t krow_bytes = (size_t)(S::NA + S::NB) * a.nbf * sizeof(double);
         const int krow = (a.do_k && krow_bytes <= kKrowMaxBytes) ? 1 : 0;
@@ -498,7 +501,7 @@ static hipError_t launch_jk_mp_t(const JkLaunch& a, hipStream_t st) {
     constexpr int LA = QCG_LA(CB), LB = QCG_LB(CB), LC = QCG_LA(CK), LD = QCG_LB(CK);
     using S = EriShape<LA, LB, LC, LD>;
     if (a.nitems <= 0) return hipSuccess;
-    // classes with 27 or more components always take the wave-cooperative form
+    // classes without a per-lane form always take the wave-cooperative one
     if (a.block == 0 || !kLaneForm<S>) {
         if constexpr (JkWave<LA, LB, LC, LD, double>::kEnabled) {
             hipLaunchKernelGGL((jk_wave_kernel_mp<LA, LB, LC, LD>), dim3((a.nitems + 3) / 4), dim3(256), 0, st,

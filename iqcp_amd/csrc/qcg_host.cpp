@@ -471,11 +471,13 @@ int qcg_autotune(qcg_context* c, const double* dD, int iters, int* chosen) {
     // (wave-cooperative kernel, classes with at least 27 components only)
     const int cand[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {256, 1}, {64, 8}, {0, 1}, {0, 2}, {0, 4}};
     const int ncand = sizeof(cand) / sizeof(cand[0]);
-    // classes with fewer than 27 components have only the per-lane kernel, the others only the
+    // classes with fewer than 27 components have only the per-lane kernel, those above 54 only the
     // wave-cooperative one: map each candidate onto what the class has
     auto tile_of = [&](int k, int cls) {
-        if (c->class_nq[cls] < 27) return cand[k][0] == 0 ? qcg_context::Tile{64, 1} : qcg_context::Tile{cand[k][0], cand[k][1]};
-        return cand[k][0] == 0 ? qcg_context::Tile{0, cand[k][1]} : qcg_context::Tile{0, std::min(8, cand[k][1] * 2)};
+        const int nq = c->class_nq[cls];
+        if (nq < 27) return cand[k][0] == 0 ? qcg_context::Tile{64, 1} : qcg_context::Tile{cand[k][0], cand[k][1]};
+        if (nq > 54) return cand[k][0] == 0 ? qcg_context::Tile{0, cand[k][1]} : qcg_context::Tile{0, std::min(8, cand[k][1] * 2)};
+        return qcg_context::Tile{cand[k][0], cand[k][1]};
     };
     std::vector<double> best(42, 1e300);
     std::vector<int> pick(42, 0);
