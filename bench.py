@@ -59,6 +59,7 @@ def main():
     ap.add_argument("--no-gradient", action="store_true", help="skip the gradient evals/s leg")
     ap.add_argument("--no-graph", action="store_true", help="launch every Fock build eagerly")
     ap.add_argument("--no-b3lyp", action="store_true", help="skip the (H2O)16 B3LYP J+K+XC leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-alkane batched SCF leg")
     ap.add_argument("--grad-steps", type=int, default=3)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default tiles instead of autotuning (profiling runs)")
@@ -236,6 +237,27 @@ def main():
         del ew
         log(f"b3lyp water16: fp64 {ks['fp64_ms_per_build']} ms, mixed {ks['mixed_ms_per_build']} ms per J+K+XC build")
 
+    # Batch leg: 64 n-alkanes (C4..C12, RHF/6-31G), all molecules iterating SCF together, one
+    # HIP-graph replay of the batched J/K build per iteration.
+    batch = None
+    if not args.no_batch:
+        from iqcp_amd.engine import BatchEngine
+        bb = workloads.alkane_batch_631g()
+        be = BatchEngine(bb, device=0)
+        t = time.perf_counter()
+        res_b, _, jk_ms = be.rhf_scf(ScfConfig.tight(level_shift=0.1))
+        el_b = time.perf_counter() - t
+        iters = [r["iterations"] for r in res_b]
+        batch = {"workload": "64 n-alkanes C4H10..C12H26 RHF/6-31G, lockstep batched SCF (level shift 0.1 Ha)",
+                 "molecules": len(bb), "nbf_min": min(b.nbf for b in bb), "nbf_max": max(b.nbf for b in bb),
+                 "converged": sum(r["converged"] for r in res_b), "batch_iterations": max(iters),
+                 "wall_s": round(el_b, 3), "jk_ms_total": round(jk_ms, 2),
+                 "jk_ms_per_iteration": round(jk_ms / max(iters), 3),
+                 "molecule_scf_per_s": round(len(bb) / el_b, 3)}
+        del be
+        log(f"batch: {batch['molecules']} molecules, {batch['batch_iterations']} iterations, {batch['wall_s']} s, "
+            f"{batch['jk_ms_per_iteration']} ms per batched J/K build")
+
     # Gradient evals/s: analytic RHF/6-31+G* gradient of n-octane (seeded 0.05 A distortion) at its
     # converged density and energy-weighted density: ERI adjoint kernels on the GPU, one-electron
     # and nuclear terms on the host.
@@ -287,6 +309,7 @@ def main():
         "all_kernels_model_tflops": round(tot_fl / (tot_ms * 1e-3) / 1e12, 3) if tot_ms > 0 else None,
         "tiles_mixed": tiles.get("mixed"),
         "b3lyp": ks,
+        "batch": batch,
         "gradient": grad,
         "roofline": roofline,
         "cpu_baseline": cpu,
