@@ -56,10 +56,10 @@ int qcg_create(qcg_context** out, int device) {
             const int lc = ck == 0 ? 0 : ck <= 2 ? 1 : 2, ld = (ck == 0 || ck == 1 || ck == 3) ? 0 : (ck == 2 || ck == 4) ? 1 : 2;
             const int nq = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
             // defaults close to what the autotuner picks on alkanes: the wave-cooperative kernel
-            // (block 0) from 27 components on, one ket per thread below
+            // (block 0) above 54 components, the per-lane kernel up to 54
             for (int part = 0; part < 2; ++part) {
-                c->tile[cls][part].block = nq >= 27 ? 0 : (nq <= 3 ? 256 : 64);
-                c->tile[cls][part].reps = nq >= 162 ? 1 : nq >= 27 ? 2 : 1;
+                c->tile[cls][part].block = nq > 54 ? 0 : (nq <= 3 ? 256 : 64);
+                c->tile[cls][part].reps = nq >= 162 ? 1 : nq > 54 ? 2 : nq >= 18 ? 4 : 1;
             }
             c->class_nq[cls] = nq;
         }
