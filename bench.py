@@ -41,7 +41,7 @@ def kernel_name(la, lb, lc, ld, tile, f64):
 def hbm_traffic(name):
     """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc measurement
     (FETCH_SIZE and WRITE_SIZE, scaled by the same counters on a 1 GiB copy); None if absent."""
-    f = ROOT / "profiles" / "jk_decane_631gs_hbm_traffic.json"
+    f = ROOT / "profiles" / "jk_kernels_decane_631gs_hbm_traffic.json"
     if not f.exists():
         return None
     k = json.loads(f.read_text()).get("kernels", {}).get(name)
