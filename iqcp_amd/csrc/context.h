@@ -75,9 +75,11 @@ struct qcg_context {
     int class_nq[21] = {};  // Cartesian components per quartet class
     // mixed precision: run the FP64 and FP32 items of a class in one launch (jk_*_mp kernels)
     bool fuse_mixed = true;
-    std::vector<int4> items;
+    std::vector<int4> items;  // J/K work items over the ket segments segs (cut_items)
+    std::vector<int2> segs;
     int item_off[21][2] = {}, item_cnt[21][2] = {};
     int4* d_items = nullptr;
+    int2* d_segs = nullptr;
     double plan_key[4] = {-1.0, -1.0, -1.0, -1.0};
     int plan_sig = -1, tile_sig = 0;
     static constexpr int kStreams = 4;
@@ -123,7 +125,8 @@ struct qcg_context {
         drop_fock_graph();
         for (void** p : {(void**)&d_pairs, (void**)&d_pp, (void**)&d_ppf, (void**)&d_D, (void**)&d_J, (void**)&d_K,
                          (void**)&d_Jacc, (void**)&d_Kacc, (void**)&d_H, (void**)&d_F, (void**)&d_mol_nbf,
-                         (void**)&d_mat_off, (void**)&d_ket_off, (void**)&d_ket_cnt, (void**)&d_items}) {
+                         (void**)&d_mat_off, (void**)&d_ket_off, (void**)&d_ket_cnt, (void**)&d_items,
+                         (void**)&d_segs}) {
             release(*p);
             *p = nullptr;
         }

@@ -45,7 +45,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(256) void grad_wave_kernel(const int4* __restrict__ items, int nitems,
+__global__ __launch_bounds__(256) void grad_wave_kernel(const int4* __restrict__ items, const int2* __restrict__ segs, int nitems,
                                                         const PairRec* __restrict__ bra, const PairRec* __restrict__ ket,
                                                         int diag, const PrimPair<double>* __restrict__ pp,
                                                         const double* __restrict__ rys_tab,
@@ -76,11 +76,13 @@ __global__ __launch_bounds__(256) void grad_wave_kernel(const int4* __restrict__
     const int4 it = items[item];
     const PairRec P = bra[it.x];
     const int pidx = P.pos;
+    KetCursor cur{segs, it.y, it.z};
     for (int o = lane; o < S::NAB; o += 64) Dab[o] = D[(P.bfa + o / NB) * nbf + P.bfb + o % NB];
     const double deg_p = (P.sa == P.sb) ? 0.5 : 1.0;
     double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0};
 
-    for (int q = it.y; q < it.z; ++q) {
+    for (int j = 0; j < it.w; ++j, ++cur.r) {
+        const int q = cur.at();
         const PairRec Q = ket[q];
         const double deg = deg_p * ((Q.sa == Q.sb) ? 0.5 : 1.0) * ((diag && q == pidx) ? 0.5 : 1.0);
         wave_sync();  // previous quartet's readers of the weights and density blocks are done
@@ -264,7 +266,7 @@ __device__ __forceinline__ void grad_quartet_lane(const PairRec& P, const PairRe
 
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(kGradLaneBlock) void grad_lane_kernel(
-    const int4* __restrict__ items, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket, int diag,
+    const int4* __restrict__ items, const int2* __restrict__ segs, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket, int diag,
     const PrimPair<double>* __restrict__ pp, const double* __restrict__ rys_tab, const double* __restrict__ D,
     int nbf_, double xfac, int natom, double* __restrict__ grad) {
     using S = EriShape<LA, LB, LC, LD>;
@@ -280,8 +282,9 @@ __global__ __launch_bounds__(kGradLaneBlock) void grad_lane_kernel(
     const int4 it = items[blockIdx.x];
     const PairRec P = bra[it.x];
     double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0};
-    const int q = it.y + tid;
-    if (q < it.z) {
+    if (tid < it.w) {
+        KetCursor cur{segs, it.y, it.z + tid};
+        const int q = cur.at();
         const PairRec Q = ket[q];
         const double deg = ((P.sa == P.sb) ? 0.5 : 1.0) * ((Q.sa == Q.sb) ? 0.5 : 1.0) * ((diag && q == P.pos) ? 0.5 : 1.0);
         double W[S::NQ];
@@ -326,11 +329,11 @@ static hipError_t launch_grad_t(const GradLaunch& a, hipStream_t st) {
     if (a.nitems <= 0) return hipSuccess;
     if constexpr (EriShape<LA, LB, LC, LD>::NQ <= kGradLaneMaxNQ) {
         hipLaunchKernelGGL((grad_lane_kernel<LA, LB, LC, LD>), dim3(a.nitems), dim3(kGradLaneBlock),
-                           3 * (size_t)a.natom * sizeof(double), st, a.items, a.bra, a.ket, a.diag, a.pp, a.rys, a.D,
+                           3 * (size_t)a.natom * sizeof(double), st, a.items, a.segs, a.bra, a.ket, a.diag, a.pp, a.rys, a.D,
                            a.nbf, a.xfac, a.natom, a.grad);
     } else {
         hipLaunchKernelGGL((grad_wave_kernel<LA, LB, LC, LD>), dim3((a.nitems + G::WPB - 1) / G::WPB),
-                           dim3(64 * G::WPB), 0, st, a.items, a.nitems, a.bra, a.ket, a.diag, a.pp, a.rys, a.D, a.nbf,
+                           dim3(64 * G::WPB), 0, st, a.items, a.segs, a.nitems, a.bra, a.ket, a.diag, a.pp, a.rys, a.D, a.nbf,
                            a.xfac, a.grad);
     }
     return hipGetLastError();

@@ -47,14 +47,15 @@ __device__ __forceinline__ const R* rys_stage(const R* __restrict__ rys_tab, R* 
 template <int N>
 constexpr int rys_lds_doubles() { return N <= 3 ? rys_tab_size<N>() : 1; }
 
-// One work item = (bra pair, contiguous range of its surviving kets); the host builds the items
-// with the exact screening predicate, so no quartet is tested twice or skipped.  rys_n: the
+// One work item = (bra pair, run of its surviving kets over the ket segments, KetCursor); the
+// host builds the items with the exact screening predicate, so no quartet is tested twice or
+// skipped.  rys_n: the
 // coefficients of this class's Rys order (rys_stage); kl: krow != 0: the four K terms of the item
 // (all in the rows of the bra functions a and b) are summed in LDS rows kl[a][*], kl[NA + b][*]
 // and flushed once per block.
 template <int LA, int LB, int LC, int LD, typename R>
 __device__ __forceinline__ void jk_item(
-    const int4 it, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket, int diag,
+    const int4 it, const int2* __restrict__ segs, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket, int diag,
     const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_n, const double* __restrict__ D,
     double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
     const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf, int krow, double* kl) {
@@ -71,9 +72,14 @@ __device__ __forceinline__ void jk_item(
     const int nrow = krow ? (S::NA + S::NB) * nbf : 0;
     for (int i = tid; i < nrow; i += blockDim.x) kl[i] = 0.0;
     __syncthreads();
+    // LDS rows take a workgroup-scope atomic, HBM an agent-scope one: two distinct operations, so
+    // the compiler never merges them into one flat atomic on a selected pointer
     auto kadd = [&](bool row_b, int r, int col, double v) {
-        if (krow) atomicAdd(&kl[(row_b ? S::NA + r : r) * nbf + col], v);
-        else atomicAdd(&Kacc[(size_t)((row_b ? P.bfb : P.bfa) + r) * nbf + col], v);
+        if (krow)
+            __hip_atomic_fetch_add(&kl[(row_b ? S::NA + r : r) * nbf + col], v, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+            atomicAdd(&Kacc[(size_t)((row_b ? P.bfb : P.bfa) + r) * nbf + col], v);
     };
 
     // bra density block scaled by the bra component norms
@@ -85,7 +91,9 @@ __device__ __forceinline__ void jk_item(
     });
     const double deg_p = (P.sa == P.sb) ? 0.5 : 1.0;
 
-    for (int q = it.y + tid; q < it.z; q += blockDim.x) {
+    KetCursor cur{segs, it.y, it.z + tid};
+    for (int j = tid; j < it.w; j += blockDim.x, cur.r += blockDim.x) {
+        const int q = cur.at();
         const PairRec Q = ket[q];
         R eri[S::NQ];
         for_n<U, S::NQ>([&](int i) { eri[i] = R(0); });
@@ -198,7 +206,7 @@ __device__ __forceinline__ void jk_item(
 
 template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(256) void jk_kernel(
-    const int4* __restrict__ items, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket, int diag,
+    const int4* __restrict__ items, const int2* __restrict__ segs, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket, int diag,
     const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_tab, const double* __restrict__ D,
     double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
     const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf, int krow) {
@@ -206,7 +214,7 @@ __global__ __launch_bounds__(256) void jk_kernel(
     extern __shared__ double kl[];
     __shared__ double rys_raw[rys_lds_doubles<NR>()];
     const R* rys_n = rys_stage<R, NR>(rys_tab, reinterpret_cast<R*>(rys_raw));
-    jk_item<LA, LB, LC, LD, R>(items[blockIdx.x], bra, ket, diag, pp, rys_n, D, Jacc, Kacc, do_j, do_k, ket_off,
+    jk_item<LA, LB, LC, LD, R>(items[blockIdx.x], segs, bra, ket, diag, pp, rys_n, D, Jacc, Kacc, do_j, do_k, ket_off,
                                mat_off, mol_nbf, krow, kl);
 }
 
@@ -214,7 +222,7 @@ __global__ __launch_bounds__(256) void jk_kernel(
 // block); saves the second launch per class and its tail.
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(256) void jk_kernel_mp(
-    const int4* __restrict__ items, int n64, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket,
+    const int4* __restrict__ items, const int2* __restrict__ segs, int n64, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket,
     int diag, const PrimPair<double>* __restrict__ pp, const PrimPair<float>* __restrict__ ppf,
     const double* __restrict__ rys_tab, const float* __restrict__ rys_tabf, const double* __restrict__ D,
     double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
@@ -224,11 +232,11 @@ __global__ __launch_bounds__(256) void jk_kernel_mp(
     __shared__ double rys_raw[rys_lds_doubles<NR>()];
     if ((int)blockIdx.x < n64) {
         const double* rys_n = rys_stage<double, NR>(rys_tab, rys_raw);
-        jk_item<LA, LB, LC, LD, double>(items[blockIdx.x], bra, ket, diag, pp, rys_n, D, Jacc, Kacc, do_j, do_k,
+        jk_item<LA, LB, LC, LD, double>(items[blockIdx.x], segs, bra, ket, diag, pp, rys_n, D, Jacc, Kacc, do_j, do_k,
                                         ket_off, mat_off, mol_nbf, krow, kl);
     } else {
         const float* rys_n = rys_stage<float, NR>(rys_tabf, reinterpret_cast<float*>(rys_raw));
-        jk_item<LA, LB, LC, LD, float>(items[blockIdx.x], bra, ket, diag, ppf, rys_n, D, Jacc, Kacc, do_j, do_k,
+        jk_item<LA, LB, LC, LD, float>(items[blockIdx.x], segs, bra, ket, diag, ppf, rys_n, D, Jacc, Kacc, do_j, do_k,
                                        ket_off, mat_off, mol_nbf, krow, kl);
     }
 }
@@ -254,7 +262,7 @@ struct JkWave {
 // One work item on one wavefront; V: the wave's LDS region (JkWave<..., R>::WAVE_D doubles).
 template <int LA, int LB, int LC, int LD, typename R>
 __device__ __forceinline__ void jk_wave_item(
-    const int4 it, int lane, double* V, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket,
+    const int4 it, const int2* __restrict__ segs, int lane, double* V, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket,
     int diag, const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_tab, const double* __restrict__ D,
     double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
     const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf) {
@@ -293,7 +301,9 @@ __device__ __forceinline__ void jk_wave_item(
     double jp = 0.0;  // J_ab of lane ab over the item's kets
     const double deg_p = (P.sa == P.sb) ? 0.5 : 1.0;
 
-    for (int q = it.y; q < it.z; ++q) {
+    KetCursor cur{segs, it.y, it.z};
+    for (int j = 0; j < it.w; ++j, ++cur.r) {
+        const int q = cur.at();
         const PairRec Q = ket[q];
         R acc[KPL];
 #pragma unroll
@@ -379,7 +389,7 @@ __device__ __forceinline__ void jk_wave_item(
 
 template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(256) void jk_wave_kernel(
-    const int4* __restrict__ items, int nitems, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket,
+    const int4* __restrict__ items, const int2* __restrict__ segs, int nitems, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket,
     int diag, const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_tab, const double* __restrict__ D,
     double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
     const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf) {
@@ -388,14 +398,14 @@ __global__ __launch_bounds__(256) void jk_wave_kernel(
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int item = blockIdx.x * 4 + wid;
     if (item >= nitems) return;  // no block-wide barriers in the item: waves are independent
-    jk_wave_item<LA, LB, LC, LD, R>(items[item], lane, smem + wid * WD, bra, ket, diag, pp, rys_tab, D, Jacc, Kacc,
+    jk_wave_item<LA, LB, LC, LD, R>(items[item], segs, lane, smem + wid * WD, bra, ket, diag, pp, rys_tab, D, Jacc, Kacc,
                                     do_j, do_k, ket_off, mat_off, mol_nbf);
 }
 
 // Mixed precision in one launch: items [0, n64) FP64, the rest FP32 (one branch per wavefront).
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(256) void jk_wave_kernel_mp(
-    const int4* __restrict__ items, int nitems, int n64, const PairRec* __restrict__ bra,
+    const int4* __restrict__ items, const int2* __restrict__ segs, int nitems, int n64, const PairRec* __restrict__ bra,
     const PairRec* __restrict__ ket, int diag, const PrimPair<double>* __restrict__ pp,
     const PrimPair<float>* __restrict__ ppf, const double* __restrict__ rys_tab, const float* __restrict__ rys_tabf,
     const double* __restrict__ D, double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k,
@@ -407,10 +417,10 @@ __global__ __launch_bounds__(256) void jk_wave_kernel_mp(
     const int item = blockIdx.x * 4 + wid;
     if (item >= nitems) return;
     if (item < n64)
-        jk_wave_item<LA, LB, LC, LD, double>(items[item], lane, smem + wid * WD, bra, ket, diag, pp, rys_tab, D, Jacc,
+        jk_wave_item<LA, LB, LC, LD, double>(items[item], segs, lane, smem + wid * WD, bra, ket, diag, pp, rys_tab, D, Jacc,
                                              Kacc, do_j, do_k, ket_off, mat_off, mol_nbf);
     else
-        jk_wave_item<LA, LB, LC, LD, float>(items[item], lane, smem + wid * WD, bra, ket, diag, ppf, rys_tabf, D, Jacc,
+        jk_wave_item<LA, LB, LC, LD, float>(items[item], segs, lane, smem + wid * WD, bra, ket, diag, ppf, rys_tabf, D, Jacc,
                                             Kacc, do_j, do_k, ket_off, mat_off, mol_nbf);
 }
 
@@ -458,13 +468,9 @@ __global__ __launch_bounds__(256) void eri_tensor_kernel(
 constexpr size_t kKrowMaxBytes = 32768;
 
 // Classes that have a per-lane kernel: up to 54 Cartesian components (above that the
-// wave-cooperative kernel always wins and the per-lane form is register-starved).  One exception:
-// hipcc of ROCm 7.2 stops with "Illegal instruction detected: Operand has incorrect register
-// class" on the FP32 per-lane (dd|ss) kernel; that class runs FP32 quartets in the wave form (the
-// fused mixed kernel, which holds both precisions, compiles and keeps the per-lane form).
-template <typename S, typename R = double>
-inline constexpr bool kLaneForm =
-    S::NQ <= 54 && !(sizeof(R) == 4 && S::NQ == 36 && S::NA == 6 && S::NB == 6);
+// wave-cooperative kernel always wins and the per-lane form is register-starved).
+template <typename S>
+inline constexpr bool kLaneForm = S::NQ <= 54;
 
 template <int CB, int CK, typename R>
 static hipError_t launch_jk_t(const JkLaunch& a, hipStream_t st) {
@@ -472,20 +478,20 @@ static hipError_t launch_jk_t(const JkLaunch& a, hipStream_t st) {
     using S = EriShape<LA, LB, LC, LD>;
     if (a.nitems <= 0) return hipSuccess;
     // wave-cooperative mode (four items per block); classes without a per-lane form always take it
-    if (a.block == 0 || !kLaneForm<S, R>) {
+    if (a.block == 0 || !kLaneForm<S>) {
         if constexpr (JkWave<LA, LB, LC, LD, R>::kEnabled) {
             hipLaunchKernelGGL((jk_wave_kernel<LA, LB, LC, LD, R>), dim3((a.nitems + 3) / 4), dim3(256), 0, st,
-                               a.items, a.nitems, a.bra, a.ket, a.diag, (const PrimPair<R>*)a.pp, (const R*)a.rys,
+                               a.items, a.segs, a.nitems, a.bra, a.ket, a.diag, (const PrimPair<R>*)a.pp, (const R*)a.rys,
                                a.D, a.Jacc, a.Kacc, a.do_j, a.do_k, a.ket_off, a.mat_off, a.mol_nbf);
             return hipGetLastError();
         }
     }
-    if constexpr (kLaneForm<S, R>) {
+    if constexpr (kLaneForm<S>) {
         // K rows of the bra functions in LDS when they fit (a.nbf is the largest molecule's nbf)
         const size_t krow_bytes = (size_t)(S::NA + S::NB) * a.nbf * sizeof(double);
         const int krow = (a.do_k && krow_bytes <= kKrowMaxBytes) ? 1 : 0;
         hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, R>), dim3(a.nitems), dim3(a.block > 0 ? a.block : 64),
-                           krow ? krow_bytes : 0, st, a.items, a.bra, a.ket, a.diag, (const PrimPair<R>*)a.pp,
+                           krow ? krow_bytes : 0, st, a.items, a.segs, a.bra, a.ket, a.diag, (const PrimPair<R>*)a.pp,
                            (const R*)a.rys, a.D, a.Jacc, a.Kacc, a.do_j, a.do_k, a.ket_off, a.mat_off, a.mol_nbf,
                            krow);
         return hipGetLastError();
@@ -505,7 +511,7 @@ static hipError_t launch_jk_mp_t(const JkLaunch& a, hipStream_t st) {
     if (a.block == 0 || !kLaneForm<S>) {
         if constexpr (JkWave<LA, LB, LC, LD, double>::kEnabled) {
             hipLaunchKernelGGL((jk_wave_kernel_mp<LA, LB, LC, LD>), dim3((a.nitems + 3) / 4), dim3(256), 0, st,
-                               a.items, a.nitems, a.n64, a.bra, a.ket, a.diag, (const PrimPair<double>*)a.pp,
+                               a.items, a.segs, a.nitems, a.n64, a.bra, a.ket, a.diag, (const PrimPair<double>*)a.pp,
                                (const PrimPair<float>*)a.ppf, (const double*)a.rys, (const float*)a.rysf, a.D, a.Jacc,
                                a.Kacc, a.do_j, a.do_k, a.ket_off, a.mat_off, a.mol_nbf);
             return hipGetLastError();
@@ -515,7 +521,7 @@ static hipError_t launch_jk_mp_t(const JkLaunch& a, hipStream_t st) {
         const size_t krow_bytes = (size_t)(S::NA + S::NB) * a.nbf * sizeof(double);
         const int krow = (a.do_k && krow_bytes <= kKrowMaxBytes) ? 1 : 0;
         hipLaunchKernelGGL((jk_kernel_mp<LA, LB, LC, LD>), dim3(a.nitems), dim3(a.block > 0 ? a.block : 64),
-                           krow ? krow_bytes : 0, st, a.items, a.n64, a.bra, a.ket, a.diag,
+                           krow ? krow_bytes : 0, st, a.items, a.segs, a.n64, a.bra, a.ket, a.diag,
                            (const PrimPair<double>*)a.pp, (const PrimPair<float>*)a.ppf, (const double*)a.rys,
                            (const float*)a.rysf, a.D, a.Jacc, a.Kacc, a.do_j, a.do_k, a.ket_off, a.mat_off, a.mol_nbf,
                            krow);

@@ -1,8 +1,45 @@
 // Host-side launch interface of the two-electron kernels.
 #pragma once
+#include <algorithm>
+#include <vector>
+
 #include "qcg_types.h"
 
 namespace qcg {
+
+// Work items over ket segments.  The surviving kets of one bra pair are one contiguous range per
+// contraction bucket; the host lists those ranges as segments (first ket, count) and cuts their
+// concatenation into items (bra index, first segment, offset into it, ket count), so a block's
+// lanes stay busy across bucket boundaries instead of idling at the end of every bucket.
+//
+// cut_items: items of at most ch kets over segments [g0, segs.size()) of bra `bra`.
+inline void cut_items(std::vector<int4>& items, const std::vector<int2>& segs, int g0, int bra, int ch) {
+    int total = 0;
+    for (size_t g = (size_t)g0; g < segs.size(); ++g) total += segs[g].y;
+    int g = g0, r = 0;
+    for (int off = 0; off < total; off += ch) {
+        const int n = std::min(ch, total - off);
+        items.push_back(make_int4(bra, g, r, n));
+        r += n;
+        while (g < (int)segs.size() && r >= segs[g].y) r -= segs[g++].y;
+    }
+}
+
+// Device cursor over an item's kets: position r counts kets from the start of segment g; at()
+// returns the ket index and may be called only for the item's own kets (j < item count), which
+// keeps the walk inside the bra's segments.
+struct KetCursor {
+    const int2* __restrict__ seg;
+    int g, r;
+    __device__ __forceinline__ int at() {
+        int2 s = seg[g];
+        while (r >= s.y) {
+            r -= s.y;
+            s = seg[++g];
+        }
+        return s.x + r;
+    }
+};
 
 struct JkLaunch {
     const PairRec* bra;
@@ -26,9 +63,11 @@ struct JkLaunch {
     const int* ket_cnt;
     const int64_t* mat_off;
     const int* mol_nbf;
-    // J/K work items: (index in the bra class list, ket begin, ket end, 0), ket indices relative to
-    // the molecule's list; one block of `block` threads per item
+    // J/K work items (index in the bra class list, first segment, offset into it, ket count) over
+    // the ket segments segs[g] = (first ket, count), ket indices relative to the molecule's list
+    // (see KetCursor); one block of `block` threads per item
     const int4* items;
+    const int2* segs;
     int nitems;
     int block;
     // mixed precision in one launch (n64 >= 0): the first n64 items are FP64, the rest FP32 with
@@ -89,8 +128,10 @@ struct GradLaunch {
     double thr;
     int natom;
     double* grad;  // 3 * natom, accumulated
-    // work items (index in the bra class list, ket begin, ket end, 0): one wavefront each
+    // work items (index in the bra class list, first segment, offset into it, ket count) over the
+    // ket segments segs (see KetCursor): one wavefront, or one block of lanes, each
     const int4* items;
+    const int2* segs;
     int nitems;
 };
 

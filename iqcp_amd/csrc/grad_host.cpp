@@ -12,27 +12,32 @@ using namespace qcg;
 namespace {
 
 // Work items of the gradient kernels: every bra pair's surviving kets (the exact screening
-// predicate Q_P Q_Q >= threshold, per contraction bucket, as for J/K) cut into ranges of
-// grad_item_kets(class pair) kets.
+// predicate Q_P Q_Q >= threshold, one segment per contraction bucket, as for J/K) cut into items
+// of grad_item_kets(class pair) kets.
 
 int eri_grad(qcg_context* c, const double* dD, double hf_frac, double* dgrad, hipStream_t st) {
     std::vector<int4> items;
+    std::vector<int2> segs;
     int off[21], cnt[21];
     int cls = 0;
     for (int cb = kNumPairClasses - 1; cb >= 0; --cb)
         for (int ck = cb; ck >= 0; --ck, ++cls) {
             off[cls] = (int)items.size();
-            for (int p = 0; p < c->ket_cnt_h[cb]; ++p)
-                for_ket_ranges(c, cb, ck, 0, p, c->schwarz_thr, c->schwarz_thr, [&](int bs, int, int e) {
-                    const int ch = grad_item_kets(cb, ck);
-                    for (int q0 = bs; q0 < e; q0 += ch) items.push_back(make_int4(p, q0, std::min(q0 + ch, e), 0));
-                });
+            for (int p = 0; p < c->ket_cnt_h[cb]; ++p) {
+                const int g0 = (int)segs.size();
+                for_ket_ranges(c, cb, ck, 0, p, c->schwarz_thr, c->schwarz_thr,
+                               [&](int bs, int, int e) { segs.push_back(make_int2(bs, e - bs)); });
+                cut_items(items, segs, g0, p, grad_item_kets(cb, ck));
+            }
             cnt[cls] = (int)items.size() - off[cls];
         }
     if (items.empty()) return QCG_OK;
     int4* ditems = nullptr;
+    int2* dsegs = nullptr;
     HIPCHK(c, c->alloc(&ditems, items.size()));
+    HIPCHK(c, c->alloc(&dsegs, segs.size()));
     HIPCHK(c, hipMemcpyAsync(ditems, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(dsegs, segs.data(), segs.size() * sizeof(int2), hipMemcpyHostToDevice, st));
     cls = 0;
     int rc = QCG_OK;
     for (int cb = kNumPairClasses - 1; cb >= 0 && rc == QCG_OK; --cb)
@@ -52,6 +57,7 @@ int eri_grad(qcg_context* c, const double* dD, double hf_frac, double* dgrad, hi
             a.natom = c->basis.natom;
             a.grad = dgrad;
             a.items = ditems + off[cls];
+            a.segs = dsegs;
             a.nitems = cnt[cls];
             const hipError_t e = launch_grad(cb, ck, a, st);
             if (e != hipSuccess) {
@@ -62,6 +68,7 @@ int eri_grad(qcg_context* c, const double* dD, double hf_frac, double* dgrad, hi
     // the item buffer is released only after the kernels that read it have finished
     const hipError_t es = hipStreamSynchronize(st);
     c->release(ditems);
+    c->release(dsegs);
     if (rc) return rc;
     HIPCHK(c, es);
     return QCG_OK;

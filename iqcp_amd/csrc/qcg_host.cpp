@@ -338,6 +338,7 @@ static int build_plan(qcg_context* c) {
     const double key[4] = {(double)c->precision, c->fp32_cutoff, c->schwarz_thr, (double)c->tile_sig};
     if (c->plan_sig == 1 && std::memcmp(key, c->plan_key, sizeof(key)) == 0) return QCG_OK;
     c->items.clear();
+    c->segs.clear();
     const int nmol = c->nmol;
     const double cut = mixed ? c->fp32_cutoff : c->schwarz_thr;
     int cls = 0;
@@ -354,12 +355,14 @@ static int build_plan(qcg_context* c) {
                 }
                 for (int m = 0; m < nmol; ++m) {
                     const int pbase = c->ket_off_h[cb * nmol + m];
-                    for (int p = 0; p < c->ket_cnt_h[cb * nmol + m]; ++p)
+                    for (int p = 0; p < c->ket_cnt_h[cb * nmol + m]; ++p) {
+                        const int g0 = (int)c->segs.size();
                         for_ket_ranges(c, cb, ck, m, p, c->schwarz_thr, cut, [&](int bs, int sp, int e) {
                             const int lo = part == 0 ? bs : sp, hi = part == 0 ? sp : e;
-                            for (int q0 = lo; q0 < hi; q0 += ch)
-                                c->items.push_back(make_int4(pbase + p, q0, std::min(q0 + ch, hi), 0));
+                            if (hi > lo) c->segs.push_back(make_int2(lo, hi - lo));
                         });
+                        cut_items(c->items, c->segs, g0, pbase + p, ch);
+                    }
                 }
                 c->item_cnt[cls][part] = (int)c->items.size() - c->item_off[cls][part];
             }
@@ -368,6 +371,10 @@ static int build_plan(qcg_context* c) {
     c->d_items = nullptr;
     HIPCHK(c, c->alloc(&c->d_items, c->items.size()));
     HIPCHK(c, hipMemcpy(c->d_items, c->items.data(), c->items.size() * sizeof(int4), hipMemcpyHostToDevice));
+    c->release(c->d_segs);
+    c->d_segs = nullptr;
+    HIPCHK(c, c->alloc(&c->d_segs, c->segs.size()));
+    HIPCHK(c, hipMemcpy(c->d_segs, c->segs.data(), c->segs.size() * sizeof(int2), hipMemcpyHostToDevice));
     std::memcpy(c->plan_key, key, sizeof(key));
     c->plan_sig = 1;
     return QCG_OK;
@@ -431,6 +438,7 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
                 a.mat_off = c->d_mat_off;
                 a.mol_nbf = c->d_mol_nbf;
                 a.items = c->d_items + c->item_off[cls][part];
+                a.segs = c->d_segs;
                 a.nitems = c->item_cnt[cls][part];
                 a.block = c->tile[cls][part].block;
                 a.pp = part ? (const void*)c->d_ppf : (const void*)c->d_pp;
