@@ -87,7 +87,10 @@ int qcg_set_basis_batch(qcg_context* ctx, int nmol, const int* mol_nshell, const
                         const double* centers, const int* shell_atom, const int* Z, const double* xyz);
 
 /* precision: QCG_PREC_*; fp32_cutoff: Schwarz-product cutoff below which quartets run in FP32
- * (default 1e-6); schwarz_threshold: SCHWARZ_THRESHOLD (integrals/eri/mod.rs:1258, default 1e-12). */
+ * (default 1e-6); schwarz_threshold: SCHWARZ_THRESHOLD (integrals/eri/mod.rs:1258, default 1e-12).
+ * QCG_PREC_MIXED also runs the XC grid products in FP32 (qcg_set_xc_mode); the SCF drivers
+ * (qcg_rhf_scf, qcg_ks_scf) switch to FP64 builds once the density change drops below 1e-5 and
+ * test convergence on FP64 builds only, so converged energies are those of the FP64 operator. */
 int qcg_set_options(qcg_context* ctx, int precision, double fp32_cutoff, double schwarz_threshold);
 int qcg_nbf(const qcg_context* ctx, int* nbf);
 /* Molecular charge (electron count = sum Z - charge), PresetSystem.nelec (scf/mod.rs:284-310). */
@@ -154,6 +157,11 @@ int qcg_autotune(qcg_context* ctx, const double* dD, int iters, int* chosen);
  * kets for block 0, block x kets otherwise). */
 int qcg_set_tiles(qcg_context* ctx, const int* tiles);
 int qcg_get_tiles(qcg_context* ctx, int* tiles);
+/* Primitive split of the J/K and gradient work items: a bra pair with more than `split` primitive
+ * pairs is processed as ranges of at most `split` of them (the integral is linear in the bra
+ * primitives), so heavy contracted quartets do not serialise on one lane.  split = 0 turns it off.
+ * No counterpart in qc-core: launch configuration only; results agree to rounding. */
+int qcg_set_prim_split(qcg_context* ctx, int split);
 /* out must hold 21 entries; *n receives the number written. */
 int qcg_class_stats(qcg_context* ctx, qcg_class_stat* out, int* n);
 
@@ -173,6 +181,14 @@ int qcg_xc_eval(qcg_context* ctx, int64_t n, const double* rho, const double* si
 int qcg_xc(qcg_context* ctx, const double* D, double* Vxc, double* Exc);
 /* Device-pointer variant; *dExc is written on the device. */
 int qcg_xc_device(qcg_context* ctx, const double* dD, double* dVxc, double* dExc, void* stream);
+/* XC build kernels.  The hand-written MFMA kernels form X = Phi D block by block and reduce it into
+ * rho on the fly, then V = Phi^T aow by tiles; the library form runs DGEMMs around a point kernel.
+ * mode 0 (default): MFMA kernels with FP32 products (FP32 accumulation, rho, the functional and
+ * E_xc in FP64, plus the first-order FP64 correction of E_xc for rounding D to FP32) for the mixed
+ * precision option, library form in FP64 otherwise; mode 1: library form for both options;
+ * mode 2: MFMA kernels for both (FP64 products for FP64).  No counterpart in qc-core: kernel
+ * selection only. */
+int qcg_set_xc_mode(qcg_context* ctx, int mode);
 /* ks_scf (dft/ks_scf.rs:1567) for B3LYP on the current grid: F = H + J - (a/2) K + V_xc. */
 int qcg_ks_scf(qcg_context* ctx, const qcg_scf_config* cfg, const double* D_init, qcg_scf_result* res, double* D,
                double* C, double* eps, double* F);

@@ -75,6 +75,8 @@ struct qcg_context {
     int class_nq[21] = {};  // Cartesian components per quartet class
     // mixed precision: run the FP64 and FP32 items of a class in one launch (jk_*_mp kernels)
     bool fuse_mixed = true;
+    // bra pairs with more primitive pairs than this are cut into primitive ranges (cut_items_split)
+    int prim_split = 0;
     std::vector<int4> items;  // J/K work items over the ket segments segs (cut_items)
     std::vector<int2> segs;
     int item_off[21][2] = {}, item_cnt[21][2] = {};
@@ -96,12 +98,20 @@ struct qcg_context {
            *d_dphz = nullptr, *d_X = nullptr, *d_aow = nullptr, *d_V = nullptr, *d_exc = nullptr,
            *d_Vparts = nullptr;  // split-K partial products of V = Phi^T aow
     int v_chunks = 0;
+    // XC build (qcg_set_xc_mode): 0 = FP32 MFMA kernels for the mixed option and rocBLAS DGEMMs
+    // around point_kernel for FP64; 1 = rocBLAS DGEMMs for both; 2 = MFMA kernels for both
+    int xc_mode = 0;
+    // FP32 copies of Phi and grad Phi plus FP32 work arrays for the mixed-precision XC build,
+    // made on its first use for the current grid
+    float *d_phi32 = nullptr, *d_dphx32 = nullptr, *d_dphy32 = nullptr, *d_dphz32 = nullptr, *d_aow32 = nullptr,
+          *d_D32 = nullptr;
     GridShell* d_gshell = nullptr;
     double *d_gexp = nullptr, *d_gcn = nullptr;
     void free_grid() {
         for (void** p : {(void**)&d_pts, (void**)&d_w, (void**)&d_phi, (void**)&d_dphx, (void**)&d_dphy,
                          (void**)&d_dphz, (void**)&d_X, (void**)&d_aow, (void**)&d_V, (void**)&d_exc, (void**)&d_Vparts,
-                         (void**)&d_gshell, (void**)&d_gexp, (void**)&d_gcn}) {
+                         (void**)&d_gshell, (void**)&d_gexp, (void**)&d_gcn, (void**)&d_phi32, (void**)&d_dphx32,
+                         (void**)&d_dphy32, (void**)&d_dphz32, (void**)&d_aow32, (void**)&d_D32}) {
             release(*p);
             *p = nullptr;
         }

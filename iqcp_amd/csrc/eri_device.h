@@ -179,20 +179,20 @@ QCG_HD void contract_quartet(const PairRec& P, const PairRec& Q,
     for (int j = 0; j < Q.npp; ++j) {
         const PrimPair<R> c = pp[Q.pp_off + j];
         const R q = c.p;
-        const R hq = R(0.5) * qrcp(q);
+        const R hq = c.hp;
         const R QCx = c.px - Cx, QCy = c.py - Cy, QCz = c.pz - Cz;
         for (int i = 0; i < P.npp; ++i) {
             const PrimPair<R> a = pp[P.pp_off + i];
             if (a.kab * c.kab < (R)kPrimScreen) continue;
             const R p = a.p;
-            const R hp = R(0.5) * qrcp(p);
-            // 1/(p+q) = rs^2 and 1/(pq) = (2 hp)(2 hq): one reciprocal square root per quartet
+            const R hp = a.hp;
+            // 1/(p+q) = rs^2: one reciprocal square root per primitive quartet
             const R rs = qrsqrt(p + q);
             const R ipq = rs * rs;
             const R PQx = a.px - c.px, PQy = a.py - c.py, PQz = a.pz - c.pz;
             const R rho = p * q * ipq;
             const R T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-            const R pref = two_pi_52 * (R(4) * hp * hq) * rs * a.ck * c.ck;
+            const R pref = two_pi_52 * rs * a.ckp * c.ckp;
             R rx[N], rw[N];
             if constexpr (TAB_N) rys_eval_tab<R, N>(T, rys_tab, rx, rw);
             else rys_eval<R, N>(T, rys_tab, rx, rw);

@@ -74,14 +74,14 @@ __global__ __launch_bounds__(256) void grad_wave_kernel(const int4* __restrict__
     const size_t nbf = (size_t)nbf_;
 
     const int4 it = items[item];
-    const PairRec P = bra[it.x];
+    const PairRec P = item_bra(bra, it);
     const int pidx = P.pos;
     KetCursor cur{segs, it.y, it.z};
     for (int o = lane; o < S::NAB; o += 64) Dab[o] = D[(P.bfa + o / NB) * nbf + P.bfb + o % NB];
     const double deg_p = (P.sa == P.sb) ? 0.5 : 1.0;
     double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0};
 
-    for (int j = 0; j < it.w; ++j, ++cur.r) {
+    for (int j = 0; j < item_kets(it); ++j, ++cur.r) {
         const int q = cur.at();
         const PairRec Q = ket[q];
         const double deg = deg_p * ((Q.sa == Q.sb) ? 0.5 : 1.0) * ((diag && q == pidx) ? 0.5 : 1.0);
@@ -106,9 +106,10 @@ __global__ __launch_bounds__(256) void grad_wave_kernel(const int4* __restrict__
             wave_tables<LA + 1, LB + 1, LC + 1, LD, N, ES, double>(P, Q, pp, rys_tab, pq0, npc, tbl, lane);
             for (int u = lane; u < npc; u += 64) {
                 const int pq = pq0 + u, i = pq / Q.npp, j = pq - i * Q.npp;
-                alph[3 * u] = pp[P.pp_off + i].pad0;
-                alph[3 * u + 1] = pp[P.pp_off + i].pad1;
-                alph[3 * u + 2] = pp[Q.pp_off + j].pad0;
+                const PrimPair<double>& a = pp[P.pp_off + i];
+                alph[3 * u] = a.ea;
+                alph[3 * u + 1] = a.p - a.ea;
+                alph[3 * u + 2] = pp[Q.pp_off + j].ea;
             }
             wave_sync();
             const int ne = npc * N;
@@ -204,16 +205,16 @@ __device__ __forceinline__ void grad_quartet_lane(const PairRec& P, const PairRe
     const double AB[3] = {P.abx, P.aby, P.abz}, CD[3] = {Q.abx, Q.aby, Q.abz};
     for (int j = 0; j < Q.npp; ++j) {
         const PrimPair<double> c = pp[Q.pp_off + j];
-        const double q = c.p, hq = 0.5 * qrcp(q), al_c = c.pad0;
+        const double q = c.p, hq = c.hp, al_c = c.ea;
         const double QC[3] = {c.px - Q.ax, c.py - Q.ay, c.pz - Q.az};
         for (int i = 0; i < P.npp; ++i) {
             const PrimPair<double> a = pp[P.pp_off + i];
             if (a.kab * c.kab < kPrimScreen) continue;
-            const double p = a.p, hp = 0.5 * qrcp(p), al_a = a.pad0, al_b = a.pad1;
+            const double p = a.p, hp = a.hp, al_a = a.ea, al_b = a.p - a.ea;
             const double rs = qrsqrt(p + q), ipq = rs * rs;
             const double PQ[3] = {a.px - c.px, a.py - c.py, a.pz - c.pz};
             const double T = p * q * ipq * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
-            const double pref = two_pi_52 * (4.0 * hp * hq) * rs * a.ck * c.ck;
+            const double pref = two_pi_52 * rs * a.ckp * c.ckp;
             const double PA[3] = {a.px - P.ax, a.py - P.ay, a.pz - P.az};
             double rx[N], rw[N];
             rys_eval_tab<double, N>(T, rys_n, rx, rw);
@@ -280,9 +281,9 @@ __global__ __launch_bounds__(kGradLaneBlock) void grad_lane_kernel(
     __syncthreads();
     const size_t nbf = (size_t)nbf_;
     const int4 it = items[blockIdx.x];
-    const PairRec P = bra[it.x];
+    const PairRec P = item_bra(bra, it);
     double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0};
-    if (tid < it.w) {
+    if (tid < item_kets(it)) {
         KetCursor cur{segs, it.y, it.z + tid};
         const int q = cur.at();
         const PairRec Q = ket[q];

@@ -61,7 +61,7 @@ __device__ __forceinline__ void jk_item(
     const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf, int krow, double* kl) {
     using S = EriShape<LA, LB, LC, LD>;
     constexpr bool U = S::kUnroll;
-    const PairRec P = bra[it.x];
+    const PairRec P = item_bra(bra, it);
     const int pidx = P.pos;
     ket += ket_off[P.mol];
     D += mat_off[P.mol];
@@ -92,7 +92,7 @@ __device__ __forceinline__ void jk_item(
     const double deg_p = (P.sa == P.sb) ? 0.5 : 1.0;
 
     KetCursor cur{segs, it.y, it.z + tid};
-    for (int j = tid; j < it.w; j += blockDim.x, cur.r += blockDim.x) {
+    for (int j = tid; j < item_kets(it); j += blockDim.x, cur.r += blockDim.x) {
         const int q = cur.at();
         const PairRec Q = ket[q];
         R eri[S::NQ];
@@ -279,7 +279,7 @@ __device__ __forceinline__ void jk_wave_item(
     double* Dbc = Dad + NA * ND;
     double* Dac = Dbc + NB * NC;
 
-    const PairRec P = bra[it.x];
+    const PairRec P = item_bra(bra, it);
     const int pidx = P.pos;
     ket += ket_off[P.mol];
     D += mat_off[P.mol];
@@ -302,7 +302,7 @@ __device__ __forceinline__ void jk_wave_item(
     const double deg_p = (P.sa == P.sb) ? 0.5 : 1.0;
 
     KetCursor cur{segs, it.y, it.z};
-    for (int j = 0; j < it.w; ++j, ++cur.r) {
+    for (int j = 0; j < item_kets(it); ++j, ++cur.r) {
         const int q = cur.at();
         const PairRec Q = ket[q];
         R acc[KPL];

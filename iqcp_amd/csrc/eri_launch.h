@@ -9,20 +9,59 @@ namespace qcg {
 
 // Work items over ket segments.  The surviving kets of one bra pair are one contiguous range per
 // contraction bucket; the host lists those ranges as segments (first ket, count) and cuts their
-// concatenation into items (bra index, first segment, offset into it, ket count), so a block's
-// lanes stay busy across bucket boundaries instead of idling at the end of every bucket.
+// concatenation into items (bra index, first segment, offset into it, w), so a block's lanes stay
+// busy across bucket boundaries instead of idling at the end of every bucket.
 //
-// cut_items: items of at most ch kets over segments [g0, segs.size()) of bra `bra`.
-inline void cut_items(std::vector<int4>& items, const std::vector<int2>& segs, int g0, int bra, int ch) {
+// w packs the ket count (bits 0-15) and an optional range of the bra's primitive pairs: first
+// (bits 16-23) and count (bits 24-31, 0 = all).  A bra pair with many primitive pairs is cut into
+// such ranges so that no lane runs the whole npp_bra x npp_ket primitive loop of a heavy quartet
+// alone; the integral is linear in the bra primitives, so the partial quartets are digested
+// (J and K) like whole ones.
+constexpr int kItemMaxKets = 0xffff;
+constexpr int kItemMaxPrim = 0xff;
+
+// items of at most ch kets over segments [g0, segs.size()) of bra `bra`, primitive range
+// (pbeg, pcnt) (pcnt = 0: all of them)
+inline void cut_items(std::vector<int4>& items, const std::vector<int2>& segs, int g0, int bra, int ch,
+                      int pbeg = 0, int pcnt = 0) {
     int total = 0;
     for (size_t g = (size_t)g0; g < segs.size(); ++g) total += segs[g].y;
+    const int wbits = (pbeg << 16) | (pcnt << 24);
     int g = g0, r = 0;
     for (int off = 0; off < total; off += ch) {
         const int n = std::min(ch, total - off);
-        items.push_back(make_int4(bra, g, r, n));
+        items.push_back(make_int4(bra, g, r, n | wbits));
         r += n;
         while (g < (int)segs.size() && r >= segs[g].y) r -= segs[g++].y;
     }
+}
+
+// cut_items for every primitive range of a bra pair with npp primitive pairs: ranges of at most
+// split pairs (split <= 0 or npp <= split: one range, the whole pair), sizes balanced
+inline void cut_items_split(std::vector<int4>& items, const std::vector<int2>& segs, int g0, int bra, int ch,
+                            int npp, int split) {
+    if (split <= 0 || npp <= split || npp > kItemMaxPrim) {
+        cut_items(items, segs, g0, bra, ch);
+        return;
+    }
+    const int nr = (npp + split - 1) / split;
+    for (int k = 0; k < nr; ++k) {
+        const int b = k * npp / nr, e = (k + 1) * npp / nr;
+        cut_items(items, segs, g0, bra, ch, b, e - b);
+    }
+}
+
+inline QCG_HD int item_kets(const int4& it) { return it.w & kItemMaxKets; }
+
+// the bra pair of an item, restricted to the item's primitive range
+inline QCG_HD PairRec item_bra(const PairRec* __restrict__ bra, const int4& it) {
+    PairRec P = bra[it.x];
+    const int pcnt = (it.w >> 24) & kItemMaxPrim;
+    if (pcnt) {
+        P.pp_off += (it.w >> 16) & kItemMaxPrim;
+        P.npp = pcnt;
+    }
+    return P;
 }
 
 // Device cursor over an item's kets: position r counts kets from the start of segment g; at()
@@ -120,7 +159,7 @@ struct GradLaunch {
     const PairRec* ket;
     int nket;
     int diag;
-    const PrimPair<double>* pp;  // pad0/pad1 hold the two primitive exponents
+    const PrimPair<double>* pp;  // ea and p - ea are the two primitive exponents
     const double* rys;
     const double* D;
     int nbf;

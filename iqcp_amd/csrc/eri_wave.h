@@ -28,8 +28,8 @@ __device__ __forceinline__ void wave_sync() {
 template <typename R, int N>
 __device__ __forceinline__ void rys_eval_one(R T, const R* __restrict__ tab, int r, R& x, R& w) {
     if (T >= (R)qcg_rys_tmax[N]) {
-        const R it = qrcp(T);
         const R rs = qrsqrt(T);
+        const R it = rs * rs;
         x = (R)qcg_rys_hx[rys_tri(N) + r] * it;
         w = (R)qcg_rys_hw[rys_tri(N) + r] * rs;
         return;
@@ -74,7 +74,7 @@ __device__ __forceinline__ void wave_tables(const PairRec& P, const PairRec& Q, 
         const PrimPair<R> a = pp[P.pp_off + i];
         const PrimPair<R> c = pp[Q.pp_off + j];
         const R p = a.p, q = c.p;
-        const R hp = R(0.5) * qrcp(p), hq = R(0.5) * qrcp(q);
+        const R hp = a.hp, hq = c.hp;
         const R rs = qrsqrt(p + q);
         const R ipq = rs * rs;
         const R PQx = a.px - c.px, PQy = a.py - c.py, PQz = a.pz - c.pz;
@@ -97,7 +97,7 @@ __device__ __forceinline__ void wave_tables(const PairRec& P, const PairRec& Q, 
         vrr2d<NB_, NK_, R>(g, PA - x * qf * PQ, QC + x * pf * PQ, b00, b10, b01);
         R sc = R(1);
         if (axis == 2)
-            sc = (a.kab * c.kab < (R)kPrimScreen) ? R(0) : w * (two_pi_52 * (R(4) * hp * hq) * rs * a.ck * c.ck);
+            sc = (a.kab * c.kab < (R)kPrimScreen) ? R(0) : w * (two_pi_52 * rs * a.ckp * c.ckp);
         R* row = tbl + axis * HT * ES + pql * N + r;
         hrr1d_put<TA, TB, TC, TD, R>(g, AB, CD, [&](int h, R v) { row[h * ES] = v * sc; });
     }

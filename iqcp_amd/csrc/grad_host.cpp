@@ -27,7 +27,8 @@ int eri_grad(qcg_context* c, const double* dD, double hf_frac, double* dgrad, hi
                 const int g0 = (int)segs.size();
                 for_ket_ranges(c, cb, ck, 0, p, c->schwarz_thr, c->schwarz_thr,
                                [&](int bs, int, int e) { segs.push_back(make_int2(bs, e - bs)); });
-                cut_items(items, segs, g0, p, grad_item_kets(cb, ck));
+                cut_items_split(items, segs, g0, p, grad_item_kets(cb, ck), c->pairs[c->class_off[cb] + p].npp,
+                                c->prim_split);
             }
             cnt[cls] = (int)items.size() - off[cls];
         }

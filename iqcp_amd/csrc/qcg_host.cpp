@@ -208,10 +208,10 @@ int qcg_set_basis_batch(qcg_context* c, int nmol, const int* mol_nshell, const i
                         q.py = (a * A[1] + b * Bc[1]) / q.p;
                         q.pz = (a * A[2] + b * Bc[2]) / q.p;
                         q.kab = kab;
-                        q.pad0 = a;  // exponents, used by the gradient kernel
-                        q.pad1 = b;
-                        q.ck = kab * B.coefs[B.prim_off[sa] + i] * B.coefs[B.prim_off[sb] + j] *
-                               radial_norm(a, B.l[sa]) * radial_norm(b, B.l[sb]);
+                        q.ea = a;
+                        q.hp = 0.5 / q.p;
+                        q.ckp = kab * B.coefs[B.prim_off[sa] + i] * B.coefs[B.prim_off[sb] + j] *
+                                radial_norm(a, B.l[sa]) * radial_norm(b, B.l[sb]) / q.p;
                         pp.push_back(q);
                     }
                 r.npp = (int)pp.size() - r.pp_off;
@@ -235,8 +235,8 @@ int qcg_set_basis_batch(qcg_context* c, int nmol, const int* mol_nshell, const i
     std::vector<PrimPair<float>> ppf(pp.size());
     for (size_t i = 0; i < pp.size(); ++i) {
         ppf[i].p = (float)pp[i].p; ppf[i].px = (float)pp[i].px; ppf[i].py = (float)pp[i].py;
-        ppf[i].pz = (float)pp[i].pz; ppf[i].kab = (float)pp[i].kab; ppf[i].ck = (float)pp[i].ck;
-        ppf[i].pad0 = (float)pp[i].pad0; ppf[i].pad1 = (float)pp[i].pad1;
+        ppf[i].pz = (float)pp[i].pz; ppf[i].kab = (float)pp[i].kab; ppf[i].ckp = (float)pp[i].ckp;
+        ppf[i].hp = (float)pp[i].hp; ppf[i].ea = (float)pp[i].ea;
     }
     c->h_pp = pp;
     HIPCHK(c, c->alloc(&c->d_pp, pp.size()));
@@ -347,7 +347,7 @@ static int build_plan(qcg_context* c) {
             for (int part = 0; part < 2; ++part) {
                 // fused mixed launches run both parts with the FP64 part's tile
                 const qcg_context::Tile& t = c->tile[cls][(mixed && c->fuse_mixed) ? 0 : part];
-                const int ch = std::max(1, t.block > 0 ? t.block * t.reps : t.reps);
+                const int ch = std::min(kItemMaxKets, std::max(1, t.block > 0 ? t.block * t.reps : t.reps));
                 c->item_off[cls][part] = (int)c->items.size();
                 if (part == 1 && !mixed) {
                     c->item_cnt[cls][part] = 0;
@@ -361,7 +361,8 @@ static int build_plan(qcg_context* c) {
                             const int lo = part == 0 ? bs : sp, hi = part == 0 ? sp : e;
                             if (hi > lo) c->segs.push_back(make_int2(lo, hi - lo));
                         });
-                        cut_items(c->items, c->segs, g0, pbase + p, ch);
+                        cut_items_split(c->items, c->segs, g0, pbase + p, ch,
+                                        c->pairs[c->class_off[cb] + pbase + p].npp, c->prim_split);
                     }
                 }
                 c->item_cnt[cls][part] = (int)c->items.size() - c->item_off[cls][part];
@@ -664,6 +665,14 @@ int qcg_build_fock_device(qcg_context* c, const double* dH, const double* dD, do
     return QCG_OK;
 }
 
+int qcg_set_prim_split(qcg_context* c, int split) {
+    if (!c) return QCG_INVALID_ARG;
+    if (split < 0) return fail(c, QCG_INVALID_ARG, "prim split must be >= 0");
+    c->prim_split = split;
+    ++c->tile_sig;  // the work items change: plan and captured graph are rebuilt
+    return QCG_OK;
+}
+
 int qcg_set_graph(qcg_context* c, int on) {
     if (!c) return QCG_INVALID_ARG;
     c->use_graph = on != 0;
@@ -799,6 +808,11 @@ int rhf_fock_cb(void* u, const double* D, double* F, double* e_elec) {
     *e_elec = 0.5 * e;
     return QCG_OK;
 }
+// mixed precision: the last SCF iterations run FP64 builds (scf_driver's refine step)
+int refine_fp64(void* u) {
+    static_cast<RhfUser*>(u)->c->precision = QCG_PREC_FP64;
+    return QCG_OK;
+}
 }  // namespace
 
 int qcg_rhf_scf(qcg_context* c, const qcg_scf_config* cfg, const double* D_init, qcg_scf_result* res, double* D,
@@ -815,8 +829,10 @@ int qcg_rhf_scf(qcg_context* c, const qcg_scf_config* cfg, const double* D_init,
     for (size_t i = 0; i < n2; ++i) H[i] = T[i] + V[i];
     RhfUser u{c, H.data()};
     qcg_scf_result r{};
+    const int prec = c->precision;
     int rc = scf_driver(B.nbf, nelec / 2, S.data(), H.data(), nuclear_repulsion(B), *cfg, rhf_fock_cb, &u, D_init, &r,
-                        D, C, eps, F);
+                        D, C, eps, F, prec == QCG_PREC_MIXED ? refine_fp64 : nullptr);
+    c->precision = prec;
     if (res) *res = r;
     if (rc == QCG_NOT_CONVERGED) return fail(c, rc, "ScfError::NotConverged");
     if (rc == QCG_NUMERICAL) return fail(c, rc, "ScfError::SingularOverlap");

@@ -4,17 +4,20 @@
 
 namespace qcg {
 
-// One primitive pair (i, j) of a shell pair, array-of-structs (48 B + pad) so that a lane
-// reading "its" pair touches one or two cache lines.
+// One primitive pair (i, j) of a shell pair, array-of-structs (8 values, one 64 B line in FP64) so
+// that a lane reading "its" pair touches one cache line.
 //   p     = a_i + a_j
 //   P     = (a_i A + a_j B) / p
 //   kab   = exp(-a_i a_j / p |A-B|^2)   (raw, used by the 1e-15 primitive screens)
-//   ck    = kab * c_i c_j * n(a_i, la) n(a_j, lb), n(a, l) = (2a/pi)^(3/4) (4a)^(l/2)
+//   ckp   = kab * c_i c_j * n(a_i, la) n(a_j, lb) / p, n(a, l) = (2a/pi)^(3/4) (4a)^(l/2), so that
+//           the quartet prefactor 2 pi^(5/2) / (p q sqrt(p+q)) K_ab K_cd is 2 pi^(5/2) ckp_ab ckp_cd / sqrt(p+q)
+//   hp    = 1 / (2p)
+//   ea    = a_i (the gradient kernels take a_j as p - ea)
 // The component-dependent part of the reference normalisation, 1/sqrt((2i-1)!!(2j-1)!!(2k-1)!!)
 // (integrals/overlap.rs:343-377), is applied after contraction.
 template <typename R>
 struct alignas(8 * sizeof(R)) PrimPair {
-    R p, px, py, pz, kab, ck, pad0, pad1;
+    R p, px, py, pz, kab, ckp, hp, ea;
 };
 
 // A shell pair with l(sa) >= l(sb) (the two shells are swapped if needed; integrals are

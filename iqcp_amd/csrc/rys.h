@@ -38,8 +38,8 @@ template <typename R, int N>
 QCG_HD void rys_eval_tab(R T, const R* __restrict__ tabN, R* x, R* w) {
     static_assert(N >= 1 && N <= QCG_RYS_NMAX, "Rys order out of table range");
     if (T >= (R)qcg_rys_tmax[N]) {
-        const R it = qrcp(T);
         const R rs = qrsqrt(T);
+        const R it = rs * rs;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             x[i] = (R)qcg_rys_hx[rys_tri(N) + i] * it;

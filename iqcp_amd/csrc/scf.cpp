@@ -8,6 +8,7 @@
 // convergence on |dE|, RMS dD and the orbital-gradient norm (:1514-1520).
 // ScfRun holds one molecule's state; scf_driver runs it against a Fock callback, and the batch
 // driver (qcg_batch_rhf_scf) steps many of them in lockstep around one batched Fock build.
+// A mixed-precision run starts on reduced-precision builds and finishes on FP64 ones (refine).
 #include "scf.h"
 
 #include <cmath>
@@ -96,7 +97,11 @@ int ScfRun::init(int nbf_, int nocc_, const double* S_, const double* H_, double
 }
 
 // F and E_elec for the current D.  Returns true when this molecule is finished.
-bool ScfRun::accept_fock(const double* F_, double e_elec) {
+bool ScfRun::criteria_met() const {
+    return last_dE < cfg.energy_tol && last_rms < cfg.density_tol && last_g < std::sqrt(cfg.energy_tol);
+}
+
+bool ScfRun::accept_fock(const double* F_, double e_elec, bool may_converge) {
     const size_t n2 = (size_t)nbf * nbf;
     F.assign(F_, F_ + n2);
     const double e_new = e_elec + e_nuc;
@@ -119,7 +124,8 @@ bool ScfRun::accept_fock(const double* F_, double e_elec) {
         const int nvir = nbf - nocc;
         const double gnorm = nvir > 0 ? std::sqrt(4.0 * g2) / std::sqrt((double)nvir * nocc) : 0.0;
         e_tot = e_new;
-        if (last_dE < cfg.energy_tol && last_rms < cfg.density_tol && gnorm < std::sqrt(cfg.energy_tol)) {
+        last_g = gnorm;
+        if (may_converge && criteria_met()) {
             converged = done = true;
             if (std::fabs(cfg.level_shift) > 1e-10) diagonalize(F);
             return true;
@@ -184,16 +190,22 @@ void ScfRun::fill_result(ScfResult* res) const {
 
 int scf_driver(int nbf, int nocc, const double* S, const double* H, double e_nuc, const ScfConfig& cfg,
                FockFn fock, void* user, const double* D_init, ScfResult* res, double* D_out, double* C_out,
-               double* eps_out, double* F_out) {
+               double* eps_out, double* F_out, RefineFn refine) {
     ScfRun run;
     int rc = run.init(nbf, nocc, S, H, e_nuc, cfg, D_init);
     if (rc) return rc;
     std::vector<double> F((size_t)nbf * nbf);
+    bool exact = refine == nullptr;
     for (;;) {
         double e_elec = 0.0;
         rc = fock(user, run.D.data(), F.data(), &e_elec);
         if (rc) return rc;
-        if (run.accept_fock(F.data(), e_elec)) break;
+        if (run.accept_fock(F.data(), e_elec, exact)) break;
+        if (!exact && run.iter > 0 && (run.last_rms < kRefineRms || run.criteria_met())) {
+            rc = refine(user);
+            if (rc) return rc;
+            exact = true;
+        }
         run.step();
     }
     run.fill_result(res);

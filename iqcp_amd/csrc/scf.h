@@ -20,13 +20,15 @@ struct ScfRun {
     };
     int nbf = 0, nocc = 0, iter = 0;
     ScfConfig cfg{};
-    double e_nuc = 0.0, e_tot = 0.0, last_dE = 0.0, last_rms = 0.0;
+    double e_nuc = 0.0, e_tot = 0.0, last_dE = 0.0, last_rms = 0.0, last_g = 0.0;
     bool done = false, converged = false, have_last = false;
     std::vector<double> S, H, X, D, Dold, F, F_last, C, eps;
     Diis diis;
     int init(int nbf, int nocc, const double* S, const double* H, double e_nuc, const ScfConfig& cfg,
              const double* D_init);
-    bool accept_fock(const double* F, double e_elec);
+    // may_converge = false: the build was reduced precision, so convergence is not declared
+    bool accept_fock(const double* F, double e_elec, bool may_converge = true);
+    bool criteria_met() const;
     void step();
     void diagonalize(const std::vector<double>& F);
     void fill_result(ScfResult* res) const;
@@ -34,9 +36,19 @@ struct ScfRun {
 
 // Fock callback: D (row-major, symmetric) -> F and the electronic energy for that D.
 typedef int (*FockFn)(void* user, const double* D, double* F, double* e_elec);
+// Switches the Fock callback from reduced to full (FP64) precision for the rest of the run.
+typedef int (*RefineFn)(void* user);
 
+// Density change below which a reduced-precision run switches to FP64 builds (mixed precision:
+// the FP32 parts leave noise of about 1e-7 in F, so the tight criteria are only tested on FP64
+// builds; the converged state and energy are those of the FP64 operator).
+constexpr double kRefineRms = 1e-5;
+
+// refine != nullptr: the callback starts in reduced precision; the driver calls refine once, when
+// the density change drops below kRefineRms or the criteria are met, and declares convergence
+// only on FP64 builds.
 int scf_driver(int nbf, int nocc, const double* S, const double* H, double e_nuc, const ScfConfig& cfg,
                FockFn fock, void* user, const double* D_init, ScfResult* res, double* D_out, double* C_out,
-               double* eps_out, double* F_out);
+               double* eps_out, double* F_out, RefineFn refine = nullptr);
 
 }  // namespace qcg

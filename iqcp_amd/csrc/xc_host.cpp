@@ -86,8 +86,50 @@ int upload_grid(qcg_context* c) {
     return QCG_OK;
 }
 
+// FP32 copies of Phi and grad Phi (and FP32 work arrays) for the mixed-precision XC build
+int ensure_xc32(qcg_context* c, hipStream_t st) {
+    if (c->d_phi32) return QCG_OK;
+    const size_t gn = (size_t)c->G * c->basis.nbf, nn = (size_t)c->basis.nbf * c->basis.nbf;
+    HIPCHK(c, c->alloc(&c->d_phi32, gn));
+    HIPCHK(c, c->alloc(&c->d_dphx32, gn));
+    HIPCHK(c, c->alloc(&c->d_dphy32, gn));
+    HIPCHK(c, c->alloc(&c->d_dphz32, gn));
+    HIPCHK(c, c->alloc(&c->d_aow32, gn));
+    HIPCHK(c, c->alloc(&c->d_D32, nn));
+    HIPCHK(c, launch_to_f32(c->d_phi, c->d_phi32, gn, st));
+    HIPCHK(c, launch_to_f32(c->d_dphx, c->d_dphx32, gn, st));
+    HIPCHK(c, launch_to_f32(c->d_dphy, c->d_dphy32, gn, st));
+    HIPCHK(c, launch_to_f32(c->d_dphz, c->d_dphz32, gn, st));
+    return QCG_OK;
+}
+
+// V_xc and E_xc with the MFMA kernels (xc_mfma.hip): FP32 products in mixed precision, FP64 otherwise
+int xc_build_mfma(qcg_context* c, const double* dD, double* dV, double* dExc, hipStream_t st) {
+    const int nbf = c->basis.nbf, G = (int)c->G;
+    const int nch = xc_mfma_chunks(G, c->v_chunks);
+    HIPCHK(c, launch_fill(dExc, 1, 0.0, st));
+    if (c->precision == QCG_PREC_MIXED) {
+        int rc = ensure_xc32(c, st);
+        if (rc) return rc;
+        HIPCHK(c, launch_to_f32(dD, c->d_D32, (size_t)nbf * nbf, st));
+        HIPCHK(c, launch_xc_mfma(G, nbf, c->v_chunks, c->d_w, c->d_phi32, c->d_dphx32, c->d_dphy32, c->d_dphz32,
+                                 c->d_D32, c->d_aow32, dExc, c->d_Vparts, st));
+    } else {
+        HIPCHK(c, launch_xc_mfma(G, nbf, c->v_chunks, c->d_w, c->d_phi, c->d_dphx, c->d_dphy, c->d_dphz, dD, c->d_aow,
+                                 dExc, c->d_Vparts, st));
+    }
+    HIPCHK(c, launch_sum_parts(c->d_Vparts, nch, (size_t)nbf * nbf, c->d_V, st));
+    HIPCHK(c, launch_sym_half(c->d_V, dV, nbf, st));
+    if (c->precision == QCG_PREC_MIXED)
+        HIPCHK(c, launch_exc_d32_correction(dV, dD, c->d_D32, (size_t)nbf * nbf, dExc, st));
+    return QCG_OK;
+}
+
 int xc_build(qcg_context* c, const double* dD, double* dV, double* dExc, hipStream_t st) {
     if (c->G <= 0) return fail(c, QCG_NO_GRID, "no grid: call qcg_build_grid or qcg_set_grid");
+    // mode 0: FP32 MFMA kernels for the mixed option, library DGEMMs for FP64 (faster than the
+    // FP64 MFMA kernels, which mode 2 selects); mode 1: library DGEMMs for both
+    if ((c->xc_mode == 0 && c->precision == QCG_PREC_MIXED) || c->xc_mode == 2) return xc_build_mfma(c, dD, dV, dExc, st);
     int rc = ensure_blas(c);
     if (rc) return rc;
     const int nbf = c->basis.nbf;
@@ -257,6 +299,13 @@ int qcg_xc_eval(qcg_context* c, int64_t n, const double* rho, const double* sigm
     return QCG_OK;
 }
 
+int qcg_set_xc_mode(qcg_context* c, int mode) {
+    if (!c) return QCG_INVALID_ARG;
+    if (mode < 0 || mode > 2) return fail(c, QCG_INVALID_ARG, "xc mode must be 0, 1 or 2");
+    c->xc_mode = mode;
+    return QCG_OK;
+}
+
 int qcg_xc_device(qcg_context* c, const double* dD, double* dV, double* dExc, void* stream) {
     if (!c || !dD || !dV || !dExc) return QCG_INVALID_ARG;
     if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
@@ -322,6 +371,12 @@ int ks_fock_cb(void* u, const double* D, double* F, double* e_elec) {
     return QCG_OK;
 }
 
+// mixed precision: the last SCF iterations run FP64 builds (scf_driver's refine step)
+int ks_refine_fp64(void* u) {
+    static_cast<KsUser*>(u)->c->precision = QCG_PREC_FP64;
+    return QCG_OK;
+}
+
 }  // namespace
 
 extern "C" int qcg_ks_scf(qcg_context* c, const qcg_scf_config* cfg, const double* D_init, qcg_scf_result* res,
@@ -339,8 +394,10 @@ extern "C" int qcg_ks_scf(qcg_context* c, const qcg_scf_config* cfg, const doubl
     for (size_t i = 0; i < n2; ++i) H[i] = T[i] + V[i];
     KsUser u{c, H.data(), 0.20, 0, 0, 0, 0};
     qcg_scf_result r{};
+    const int prec = c->precision;
     int rc = scf_driver(B.nbf, nelec / 2, S.data(), H.data(), nuclear_repulsion(B), *cfg, ks_fock_cb, &u, D_init, &r,
-                        D, C, eps, F);
+                        D, C, eps, F, prec == QCG_PREC_MIXED ? ks_refine_fp64 : nullptr);
+    c->precision = prec;
     r.energy_xc = u.e_xc;
     r.energy_j = u.e_j;
     r.energy_k = u.e_k;

@@ -39,6 +39,7 @@ EXPORTS = [
     "qcg_eri_gradient", "qcg_one_electron_gradient", "qcg_rhf_gradient",
     "qcg_set_basis_batch", "qcg_batch_build_jk", "qcg_batch_rhf_scf", "qcg_autotune",
     "qcg_set_tiles", "qcg_get_tiles", "qcg_xc_gradient", "qcg_ks_gradient", "qcg_set_graph",
+    "qcg_set_prim_split", "qcg_set_xc_mode",
 ]
 
 
@@ -249,6 +250,15 @@ class Engine:
     def set_graph(self, on: bool):
         """Replay device Fock builds from a captured HIP graph."""
         self._check(self.L.qcg_set_graph(self.ctx, int(bool(on))))
+
+    def set_xc_mode(self, mode: int):
+        """XC kernels: 0 = FP32 MFMA kernels for mixed, library DGEMMs for FP64 (default);
+        1 = library DGEMMs for both; 2 = MFMA kernels for both."""
+        self._check(self.L.qcg_set_xc_mode(self.ctx, int(mode)))
+
+    def set_prim_split(self, split: int):
+        """Cut bra pairs with more than `split` primitive pairs into primitive ranges (0 = off)."""
+        self._check(self.L.qcg_set_prim_split(self.ctx, int(split)))
 
     def set_profiling(self, on: bool):
         self._check(self.L.qcg_set_profiling(self.ctx, int(bool(on))))

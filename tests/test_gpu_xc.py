@@ -69,3 +69,36 @@ def test_gpu_b3lyp_energy_vs_iqcp(mol, bas):
     out = eng.ks_scf(ScfConfig.tight())
     assert out["converged"] == 1
     assert abs(out["energy_total"] - iqcp_energy(mol, bas, "B3LYP")) < 1e-7, out["energy_total"]
+
+
+@pytest.mark.parametrize("mol,bas", [("H2O", "6-31g*"), ("NH3", "6-31+g*")])
+def test_gpu_vxc_kernel_modes(mol, bas):
+    """The MFMA XC kernels (FP64 products, and FP32 products for the mixed option) and the
+    library-GEMM form against the oracle's V_xc / E_xc on one density."""
+    b = bench_molecule(mol, bas)
+    eng = Engine(b)
+    eng.build_grid(75, "standard", True)
+    pts, w, _ = eng.get_grid()
+    D = random_density(b.nbf, seed=3, nocc=b.n_occ) * 0.5
+    chi, grad = oracle.basis_on_grid(b, pts)
+    Vo, exco, _ = oracle.vxc(chi, grad, w, D)
+    for prec, mode, tol_v, tol_e in [("fp64", 1, 2e-9, 1e-9), ("fp64", 2, 2e-9, 1e-9), ("mixed", 0, 5e-5, 1e-7)]:
+        eng.set_options(prec)
+        eng.set_xc_mode(mode)
+        V, exc = eng.xc(D)
+        assert abs(exc - exco) < tol_e * max(1.0, abs(exco)), (prec, mode, exc - exco)
+        np.testing.assert_allclose(V, Vo, atol=tol_v, rtol=0, err_msg=f"{prec} mode {mode}")
+        np.testing.assert_allclose(V, V.T, atol=1e-14)
+
+
+@pytest.mark.parametrize("mol,bas", [("H2O", "6-31g*"), ("HF", "6-31+g*")])
+def test_gpu_b3lyp_energy_mixed_vs_iqcp(mol, bas):
+    """Mixed precision (FP32 quartets below 1e-6, FP32 MFMA grid products) stays within 1 uHa of
+    IQCP's B3LYP energy."""
+    b = bench_molecule(mol, bas)
+    eng = Engine(b)
+    eng.set_options("mixed", 1e-6)
+    eng.build_grid(75, "standard", True)
+    out = eng.ks_scf(ScfConfig.tight())
+    assert out["converged"] == 1
+    assert abs(out["energy_total"] - iqcp_energy(mol, bas, "B3LYP")) < 1e-6, out["energy_total"]
