@@ -77,6 +77,10 @@ struct qcg_context {
     bool fuse_mixed = true;
     // bra pairs with more primitive pairs than this are cut into primitive ranges (cut_items_split)
     int prim_split = 0;
+    // launch schedule of the class kernels: measured time per class (qcg_autotune, 0 = unknown);
+    // with all known, classes launch longest first onto the least-loaded auxiliary stream
+    double class_ms[21] = {};
+    int launch_order[21] = {}, launch_stream[21] = {};
     std::vector<int4> items;  // J/K work items over the ket segments segs (cut_items)
     std::vector<int2> segs;
     int item_off[21][2] = {}, item_cnt[21][2] = {};

@@ -415,49 +415,71 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
         for (int i = 0; i < qcg_context::kStreams; ++i) HIPCHK(c, hipStreamWaitEvent(c->aux[i], c->ev_fork, 0));
     }
     std::memset(c->ev_recorded, 0, sizeof(c->ev_recorded));
-    int cls = 0, launched = 0;
-    for (int cb = kNumPairClasses - 1; cb >= 0; --cb)
-        for (int ck = cb; ck >= 0; --ck, ++cls) {
-            for (int part = 0; part < 2; ++part) {
-                if (part == 1 && !mixed) continue;
-                // mixed precision: both parts of the class in one launch (the plan cut the FP32
-                // items with the FP64 part's tile and put them right after the FP64 items)
-                if (part == 1 && c->fuse_mixed) continue;
-                const bool fused = mixed && c->fuse_mixed;
-                JkLaunch a{};
-                a.bra = c->d_pairs + c->class_off[cb];
-                a.ket = c->d_pairs + c->class_off[ck];
-                a.diag = cb == ck;
-                a.D = dD;
-                a.nbf = nbf;
-                a.Jacc = c->d_Jacc;
-                a.Kacc = c->d_Kacc;
-                a.do_j = do_j;
-                a.do_k = do_k;
-                a.ket_off = c->d_ket_off + ck * c->nmol;
-                a.ket_cnt = c->d_ket_cnt + ck * c->nmol;
-                a.mat_off = c->d_mat_off;
-                a.mol_nbf = c->d_mol_nbf;
-                a.items = c->d_items + c->item_off[cls][part];
-                a.segs = c->d_segs;
-                a.nitems = c->item_cnt[cls][part];
-                a.block = c->tile[cls][part].block;
-                a.pp = part ? (const void*)c->d_ppf : (const void*)c->d_pp;
-                a.rys = part ? (const void*)c->d_rys_f : (const void*)c->d_rys;
-                if (fused) {
-                    a.nitems = c->item_cnt[cls][0] + c->item_cnt[cls][1];
-                    a.n64 = c->item_cnt[cls][0];
-                    a.ppf = c->d_ppf;
-                    a.rysf = c->d_rys_f;
-                }
-                hipStream_t s = fork ? c->aux[launched % qcg_context::kStreams] : st;
-                if (a.nitems > 0) ++launched;
-                if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4 * cls + 2 * part], s));
-                HIPCHK(c, launch_jk(cb, ck, part == 1, a, s));
-                if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4 * cls + 2 * part + 1], s));
-                c->ev_recorded[cls][part] = c->profiling;
-            }
+    // class index -> (bra class, ket class), in the order (dd|dd), (dd|dp), ..., (ss|ss)
+    int cls_cb[21], cls_ck[21];
+    for (int cb = kNumPairClasses - 1, i = 0; cb >= 0; --cb)
+        for (int ck = cb; ck >= 0; --ck, ++i) {
+            cls_cb[i] = cb;
+            cls_ck[i] = ck;
         }
+    bool timed = true;
+    for (int i = 0; i < 21; ++i) timed = timed && c->class_ms[i] > 0.0;
+    if (timed) {
+        // longest processing time first onto the least-loaded stream
+        int* ord = c->launch_order;
+        for (int i = 0; i < 21; ++i) ord[i] = i;
+        std::stable_sort(ord, ord + 21, [&](int x, int y) { return c->class_ms[x] > c->class_ms[y]; });
+        double load[qcg_context::kStreams] = {};
+        for (int k = 0; k < 21; ++k) {
+            const int s = (int)(std::min_element(load, load + qcg_context::kStreams) - load);
+            c->launch_stream[ord[k]] = s;
+            load[s] += c->class_ms[ord[k]];
+        }
+    }
+    int launched = 0;
+    for (int k = 0; k < 21; ++k) {
+        const int cls = timed ? c->launch_order[k] : k;
+        const int cb = cls_cb[cls], ck = cls_ck[cls];
+        for (int part = 0; part < 2; ++part) {
+            if (part == 1 && !mixed) continue;
+            // mixed precision: both parts of the class in one launch (the plan cut the FP32
+            // items with the FP64 part's tile and put them right after the FP64 items)
+            if (part == 1 && c->fuse_mixed) continue;
+            const bool fused = mixed && c->fuse_mixed;
+            JkLaunch a{};
+            a.bra = c->d_pairs + c->class_off[cb];
+            a.ket = c->d_pairs + c->class_off[ck];
+            a.diag = cb == ck;
+            a.D = dD;
+            a.nbf = nbf;
+            a.Jacc = c->d_Jacc;
+            a.Kacc = c->d_Kacc;
+            a.do_j = do_j;
+            a.do_k = do_k;
+            a.ket_off = c->d_ket_off + ck * c->nmol;
+            a.ket_cnt = c->d_ket_cnt + ck * c->nmol;
+            a.mat_off = c->d_mat_off;
+            a.mol_nbf = c->d_mol_nbf;
+            a.items = c->d_items + c->item_off[cls][part];
+            a.segs = c->d_segs;
+            a.nitems = c->item_cnt[cls][part];
+            a.block = c->tile[cls][part].block;
+            a.pp = part ? (const void*)c->d_ppf : (const void*)c->d_pp;
+            a.rys = part ? (const void*)c->d_rys_f : (const void*)c->d_rys;
+            if (fused) {
+                a.nitems = c->item_cnt[cls][0] + c->item_cnt[cls][1];
+                a.n64 = c->item_cnt[cls][0];
+                a.ppf = c->d_ppf;
+                a.rysf = c->d_rys_f;
+            }
+            hipStream_t s = !fork ? st : c->aux[timed ? c->launch_stream[cls] : launched % qcg_context::kStreams];
+            if (a.nitems > 0) ++launched;
+            if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4 * cls + 2 * part], s));
+            HIPCHK(c, launch_jk(cb, ck, part == 1, a, s));
+            if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4 * cls + 2 * part + 1], s));
+            c->ev_recorded[cls][part] = c->profiling;
+        }
+    }
     if (fork)
         for (int i = 0; i < qcg_context::kStreams; ++i) {
             HIPCHK(c, hipEventRecord(c->ev_join[i], c->aux[i]));
@@ -513,6 +535,7 @@ int qcg_autotune(qcg_context* c, const double* dD, int iters, int* chosen) {
         for (int i = 0; i < 42; ++i)
             if (tot[i] < best[i]) { best[i] = tot[i]; pick[i] = k; }
     }
+    for (int i = 0; i < 21; ++i) c->class_ms[i] = (best[2 * i] + best[2 * i + 1]) / iters;
     for (int i = 0; i < 21; ++i)
         for (int part = 0; part < 2; ++part) {
             c->tile[i][part] = tile_of(pick[2 * i + part], i);
@@ -529,9 +552,9 @@ int qcg_autotune(qcg_context* c, const double* dD, int iters, int* chosen) {
 int qcg_set_tiles(qcg_context* c, const int* tiles) {
     if (!c || !tiles) return QCG_INVALID_ARG;
     for (int i = 0; i < 42; ++i)
-        if (tiles[2 * i] < 0 || tiles[2 * i] > 1024 || (tiles[2 * i] % 64) != 0 || tiles[2 * i + 1] < 1 ||
+        if (tiles[2 * i] < 0 || tiles[2 * i] > 256 || (tiles[2 * i] % 64) != 0 || tiles[2 * i + 1] < 1 ||
             tiles[2 * i + 1] > 64)
-            return fail(c, QCG_INVALID_ARG, "tile: block must be 0 or a multiple of 64 up to 1024, kets 1..64");
+            return fail(c, QCG_INVALID_ARG, "tile: block must be 0 or a multiple of 64 up to 256, kets 1..64");
     for (int i = 0; i < 21; ++i)
         for (int part = 0; part < 2; ++part)
             c->tile[i][part] = {tiles[4 * i + 2 * part], tiles[4 * i + 2 * part + 1]};
