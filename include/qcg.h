@@ -153,7 +153,7 @@ int qcg_autotune(qcg_context* ctx, const double* dD, int iters, int* chosen);
 /* Set / read the J/K tiles directly (84 ints, same layout as qcg_autotune's chosen; classes in the
  * order (dd|dd), (dd|dp), ..., (ss|ss)).  No counterpart in qc-core: launch configuration only.
  * Block 0 on a class with fewer than 27 components runs the per-lane kernel with 64 threads; classes
- * with more than 54 components always run the wave-cooperative kernel (kets per wave = the tile's
+ * with more than 81 components always run the wave-cooperative kernel (kets per wave = the tile's
  * kets for block 0, block x kets otherwise). */
 int qcg_set_tiles(qcg_context* ctx, const int* tiles);
 int qcg_get_tiles(qcg_context* ctx, int* tiles);

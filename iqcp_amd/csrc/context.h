@@ -81,6 +81,13 @@ struct qcg_context {
     // with all known, classes launch longest first onto the least-loaded auxiliary stream
     double class_ms[21] = {};
     int launch_order[21] = {}, launch_stream[21] = {};
+    // ERI gradient work items (built once per threshold and primitive split) and the device
+    // gradient (natom x 3)
+    int4* d_grad_items = nullptr;
+    int2* d_grad_segs = nullptr;
+    int grad_off[21] = {}, grad_cnt[21] = {};
+    double grad_key[2] = {-1.0, -1.0};
+    double* d_grad = nullptr;
     std::vector<int4> items;  // J/K work items over the ket segments segs (cut_items)
     std::vector<int2> segs;
     int item_off[21][2] = {}, item_cnt[21][2] = {};
@@ -140,7 +147,7 @@ struct qcg_context {
         for (void** p : {(void**)&d_pairs, (void**)&d_pp, (void**)&d_ppf, (void**)&d_D, (void**)&d_J, (void**)&d_K,
                          (void**)&d_Jacc, (void**)&d_Kacc, (void**)&d_H, (void**)&d_F, (void**)&d_mol_nbf,
                          (void**)&d_mat_off, (void**)&d_ket_off, (void**)&d_ket_cnt, (void**)&d_items,
-                         (void**)&d_segs}) {
+                         (void**)&d_segs, (void**)&d_grad_items, (void**)&d_grad_segs, (void**)&d_grad}) {
             release(*p);
             *p = nullptr;
         }
@@ -163,6 +170,12 @@ int run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStream_t
 // Everything run_jk needs that allocates or creates runtime objects (work-item plan, auxiliary
 // streams, fork/join events), done ahead so that run_jk itself can be captured into a graph.
 int prepare_jk(qcg_context* c);
+// ERI gradient at the device density dD, ADDED into dgrad on st; returns after the launches.
+int eri_grad_launch(qcg_context* c, const double* dD, double hf_frac, double* dgrad, hipStream_t st);
+// eri_grad_launch for the host density D into the context's zeroed d_grad, on c->stream
+int eri_grad_start(qcg_context* c, const double* D, double hf_frac);
+// waits on c->stream and copies d_grad to out
+int grad_finish(qcg_context* c, double* out);
 double radial_norm(double a, int l);
 
 // First position in [lo, hi) of the list starting at pairs[base] with Qp * Q < thr (Q descending

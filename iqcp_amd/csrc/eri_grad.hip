@@ -184,10 +184,12 @@ __global__ __launch_bounds__(256) void grad_wave_kernel(const int4* __restrict__
 
 // ------------------------------------------------------------------ per-lane form (light classes)
 // For classes with at most kGradLaneMaxNQ components one ket quartet per thread is cheaper than a
-// wavefront per quartet (which would leave most lanes idle in phase B).  Every table index below
-// is a compile-time constant (static_for), so the transfer tables stay in registers with no
-// indexed register access.
-constexpr int kGradLaneMaxNQ = 27;
+// wavefront per quartet (which would leave most lanes idle in phase B): octane/6-31+G* took 13.0
+// instead of 26.0 ms per ERI gradient with 54 here instead of 27, even though the 54-component
+// classes then use the whole register file (one wavefront per SIMD).  Every table index below is a
+// compile-time constant (static_for), so the transfer tables stay in registers with no indexed
+// register access.
+constexpr int kGradLaneMaxNQ = 54;
 constexpr int kGradLaneBlock = 128;
 
 template <int LA, int LB, int LC, int LD>

@@ -467,10 +467,10 @@ __global__ __launch_bounds__(256) void eri_tensor_kernel(
 
 constexpr size_t kKrowMaxBytes = 32768;
 
-// Classes that have a per-lane kernel: up to 54 Cartesian components (above that the
+// Classes that have a per-lane kernel: up to kJkLaneMaxNQ Cartesian components (above that the
 // wave-cooperative kernel always wins and the per-lane form is register-starved).
 template <typename S>
-inline constexpr bool kLaneForm = S::NQ <= 54;
+inline constexpr bool kLaneForm = S::NQ <= kJkLaneMaxNQ;
 
 template <int CB, int CK, typename R>
 static hipError_t launch_jk_t(const JkLaunch& a, hipStream_t st) {

@@ -53,6 +53,13 @@ inline void cut_items_split(std::vector<int4>& items, const std::vector<int2>& s
 
 inline QCG_HD int item_kets(const int4& it) { return it.w & kItemMaxKets; }
 
+// Largest class (Cartesian components) with a per-lane J/K kernel; larger classes always run the
+// wave-cooperative one, classes from 27 components up to this limit may run either (tile block 0).
+#ifndef QCG_JK_LANE_MAX_NQ
+#define QCG_JK_LANE_MAX_NQ 81
+#endif
+constexpr int kJkLaneMaxNQ = QCG_JK_LANE_MAX_NQ;
+
 // the bra pair of an item, restricted to the item's primitive range
 inline QCG_HD PairRec item_bra(const PairRec* __restrict__ bra, const int4& it) {
     PairRec P = bra[it.x];
@@ -178,7 +185,7 @@ struct GradLaunch {
 // per item for the light classes (per-lane kernel), a few kets per wavefront otherwise.
 inline int grad_item_kets(int cb, int ck) {
     auto nc = [](int c) { return c == 0 ? 1 : c == 1 ? 3 : c == 2 ? 9 : c == 3 ? 6 : c == 4 ? 18 : 36; };
-    return nc(cb) * nc(ck) <= 27 ? 128 : 4;
+    return nc(cb) * nc(ck) <= 54 ? 128 : 4;
 }
 
 hipError_t launch_grad_cb0(int ck, const GradLaunch& a, hipStream_t st);

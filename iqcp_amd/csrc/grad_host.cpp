@@ -13,16 +13,17 @@ namespace {
 
 // Work items of the gradient kernels: every bra pair's surviving kets (the exact screening
 // predicate Q_P Q_Q >= threshold, one segment per contraction bucket, as for J/K) cut into items
-// of grad_item_kets(class pair) kets.
-
-int eri_grad(qcg_context* c, const double* dD, double hf_frac, double* dgrad, hipStream_t st) {
+// of grad_item_kets(class pair) kets.  Built once per basis, threshold and primitive split and kept
+// on the device.
+int grad_plan(qcg_context* c) {
+    const double key[2] = {c->schwarz_thr, (double)c->prim_split};
+    if (c->d_grad_items && std::memcmp(key, c->grad_key, sizeof(key)) == 0) return QCG_OK;
     std::vector<int4> items;
     std::vector<int2> segs;
-    int off[21], cnt[21];
     int cls = 0;
     for (int cb = kNumPairClasses - 1; cb >= 0; --cb)
         for (int ck = cb; ck >= 0; --ck, ++cls) {
-            off[cls] = (int)items.size();
+            c->grad_off[cls] = (int)items.size();
             for (int p = 0; p < c->ket_cnt_h[cb]; ++p) {
                 const int g0 = (int)segs.size();
                 for_ket_ranges(c, cb, ck, 0, p, c->schwarz_thr, c->schwarz_thr,
@@ -30,52 +31,105 @@ int eri_grad(qcg_context* c, const double* dD, double hf_frac, double* dgrad, hi
                 cut_items_split(items, segs, g0, p, grad_item_kets(cb, ck), c->pairs[c->class_off[cb] + p].npp,
                                 c->prim_split);
             }
-            cnt[cls] = (int)items.size() - off[cls];
+            c->grad_cnt[cls] = (int)items.size() - c->grad_off[cls];
         }
-    if (items.empty()) return QCG_OK;
-    int4* ditems = nullptr;
-    int2* dsegs = nullptr;
-    HIPCHK(c, c->alloc(&ditems, items.size()));
-    HIPCHK(c, c->alloc(&dsegs, segs.size()));
-    HIPCHK(c, hipMemcpyAsync(ditems, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, st));
-    HIPCHK(c, hipMemcpyAsync(dsegs, segs.data(), segs.size() * sizeof(int2), hipMemcpyHostToDevice, st));
-    cls = 0;
-    int rc = QCG_OK;
-    for (int cb = kNumPairClasses - 1; cb >= 0 && rc == QCG_OK; --cb)
-        for (int ck = cb; ck >= 0; --ck, ++cls) {
-            GradLaunch a{};
-            a.bra = c->d_pairs + c->class_off[cb];
-            a.nbra = c->class_off[cb + 1] - c->class_off[cb];
-            a.ket = c->d_pairs + c->class_off[ck];
-            a.nket = c->class_off[ck + 1] - c->class_off[ck];
-            a.diag = cb == ck;
-            a.pp = c->d_pp;
-            a.rys = c->d_rys;
-            a.D = dD;
-            a.nbf = c->basis.nbf;
-            a.xfac = 0.25 * hf_frac;
-            a.thr = c->schwarz_thr;
-            a.natom = c->basis.natom;
-            a.grad = dgrad;
-            a.items = ditems + off[cls];
-            a.segs = dsegs;
-            a.nitems = cnt[cls];
-            const hipError_t e = launch_grad(cb, ck, a, st);
-            if (e != hipSuccess) {
-                rc = fail(c, QCG_DEVICE, std::string("launch_grad: ") + hipGetErrorString(e));
-                break;
-            }
-        }
-    // the item buffer is released only after the kernels that read it have finished
-    const hipError_t es = hipStreamSynchronize(st);
-    c->release(ditems);
-    c->release(dsegs);
-    if (rc) return rc;
-    HIPCHK(c, es);
+    c->release(c->d_grad_items);
+    c->release(c->d_grad_segs);
+    c->d_grad_items = nullptr;
+    c->d_grad_segs = nullptr;
+    HIPCHK(c, c->alloc(&c->d_grad_items, items.size()));
+    HIPCHK(c, c->alloc(&c->d_grad_segs, segs.size()));
+    HIPCHK(c, hipMemcpy(c->d_grad_items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_grad_segs, segs.data(), segs.size() * sizeof(int2), hipMemcpyHostToDevice));
+    std::memcpy(c->grad_key, key, sizeof(key));
     return QCG_OK;
 }
 
 }  // namespace
+
+// Launches the ERI gradient at the device density dD on st, ADDING into dgrad (natom x 3); the
+// class kernels run on the auxiliary streams, forked from and joined back to st.  Returns without
+// synchronising: the caller may do host work before it waits on st.
+int qcg::eri_grad_launch(qcg_context* c, const double* dD, double hf_frac, double* dgrad, hipStream_t st) {
+    int rc = grad_plan(c);
+    if (rc) return rc;
+    rc = prepare_jk(c);  // the auxiliary streams and events
+    if (rc) return rc;
+    HIPCHK(c, hipEventRecord(c->ev_fork, st));
+    for (int i = 0; i < qcg_context::kStreams; ++i) HIPCHK(c, hipStreamWaitEvent(c->aux[i], c->ev_fork, 0));
+    // heaviest classes first: the d-bra classes with many components, by component count
+    int order[21], cls_cb[21], cls_ck[21];
+    for (int cb = kNumPairClasses - 1, i = 0; cb >= 0; --cb)
+        for (int ck = cb; ck >= 0; --ck, ++i) {
+            cls_cb[i] = cb;
+            cls_ck[i] = ck;
+            order[i] = i;
+        }
+    std::stable_sort(order, order + 21, [&](int x, int y) {
+        return (double)c->grad_cnt[x] * c->class_nq[x] * (c->class_nq[x] > 27 ? 4 : 1) >
+               (double)c->grad_cnt[y] * c->class_nq[y] * (c->class_nq[y] > 27 ? 4 : 1);
+    });
+    for (int k = 0; k < 21; ++k) {
+        const int cls = order[k], cb = cls_cb[cls], ck = cls_ck[cls];
+        GradLaunch a{};
+        a.bra = c->d_pairs + c->class_off[cb];
+        a.nbra = c->class_off[cb + 1] - c->class_off[cb];
+        a.ket = c->d_pairs + c->class_off[ck];
+        a.nket = c->class_off[ck + 1] - c->class_off[ck];
+        a.diag = cb == ck;
+        a.pp = c->d_pp;
+        a.rys = c->d_rys;
+        a.D = dD;
+        a.nbf = c->basis.nbf;
+        a.xfac = 0.25 * hf_frac;
+        a.thr = c->schwarz_thr;
+        a.natom = c->basis.natom;
+        a.grad = dgrad;
+        a.items = c->d_grad_items + c->grad_off[cls];
+        a.segs = c->d_grad_segs;
+        a.nitems = c->grad_cnt[cls];
+        const hipError_t e = launch_grad(cb, ck, a, c->aux[k % qcg_context::kStreams]);
+        if (e != hipSuccess) return fail(c, QCG_DEVICE, std::string("launch_grad: ") + hipGetErrorString(e));
+    }
+    for (int i = 0; i < qcg_context::kStreams; ++i) {
+        HIPCHK(c, hipEventRecord(c->ev_join[i], c->aux[i]));
+        HIPCHK(c, hipStreamWaitEvent(st, c->ev_join[i], 0));
+    }
+    return QCG_OK;
+}
+
+namespace {
+
+// W = 2 sum_i eps_i C_mu,i C_nu,i  (build_energy_weighted_density, scf/gradient.rs:101)
+std::vector<double> energy_weighted_density(int nbf, int n_occ, const double* C, const double* eps) {
+    std::vector<double> W((size_t)nbf * nbf, 0.0);
+    for (int mu = 0; mu < nbf; ++mu)
+        for (int nu = 0; nu < nbf; ++nu) {
+            double v = 0.0;
+            for (int i = 0; i < n_occ; ++i) v += eps[i] * C[(size_t)mu * nbf + i] * C[(size_t)nu * nbf + i];
+            W[(size_t)mu * nbf + nu] = 2.0 * v;
+        }
+    return W;
+}
+
+}  // namespace
+
+// Starts the ERI gradient of the host density D on c->stream into c->d_grad (zeroed first).
+int qcg::eri_grad_start(qcg_context* c, const double* D, double hf_frac) {
+    const int nbf = c->basis.nbf, n3 = 3 * c->basis.natom;
+    if (!c->d_grad) HIPCHK(c, c->alloc(&c->d_grad, (size_t)n3));
+    HIPCHK(c, hipMemcpyAsync(c->d_D, D, (size_t)nbf * nbf * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_fill(c->d_grad, n3, 0.0, c->stream));
+    return eri_grad_launch(c, c->d_D, hf_frac, c->d_grad, c->stream);
+}
+
+// Waits for the device gradient and copies it to out (natom x 3).
+int qcg::grad_finish(qcg_context* c, double* out) {
+    HIPCHK(c, hipMemcpyAsync(out, c->d_grad, 3 * (size_t)c->basis.natom * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return QCG_OK;
+}
 
 extern "C" {
 
@@ -84,18 +138,9 @@ int qcg_eri_gradient(qcg_context* c, const double* D, double hf_frac, double* gr
     if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
     if (c->nmol != 1) return fail(c, QCG_INVALID_ARG, "gradient: one molecule per context");
     HIPCHK(c, hipSetDevice(c->device));
-    const int nbf = c->basis.nbf, n3 = 3 * c->basis.natom;
-    const size_t n2 = (size_t)nbf * nbf;
-    double* dg = nullptr;
-    HIPCHK(c, c->alloc(&dg, n3));
-    HIPCHK(c, hipMemcpyAsync(c->d_D, D, n2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, launch_fill(dg, n3, 0.0, c->stream));
-    int rc = eri_grad(c, c->d_D, hf_frac, dg, c->stream);
+    int rc = eri_grad_start(c, D, hf_frac);
     if (rc) return rc;
-    HIPCHK(c, hipMemcpyAsync(grad, dg, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->release(dg);
-    return QCG_OK;
+    return grad_finish(c, grad);
 }
 
 int qcg_one_electron_gradient(qcg_context* c, const double* D, const double* W, double* grad) {
@@ -110,21 +155,19 @@ int qcg_one_electron_gradient(qcg_context* c, const double* D, const double* W, 
 int qcg_rhf_gradient(qcg_context* c, const double* D, const double* C, const double* eps, int n_occ, double* grad) {
     if (!c || !D || !C || !eps || !grad) return QCG_INVALID_ARG;
     if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    if (c->nmol != 1) return fail(c, QCG_INVALID_ARG, "gradient: one molecule per context");
     const int nbf = c->basis.nbf, n3 = 3 * c->basis.natom;
     if (n_occ < 0 || n_occ > nbf) return fail(c, QCG_INVALID_ARG, "n_occ");
-    // W = 2 sum_i eps_i C_mu,i C_nu,i  (build_energy_weighted_density, scf/gradient.rs:101)
-    std::vector<double> W((size_t)nbf * nbf, 0.0);
-    for (int mu = 0; mu < nbf; ++mu)
-        for (int nu = 0; nu < nbf; ++nu) {
-            double v = 0.0;
-            for (int i = 0; i < n_occ; ++i) v += eps[i] * C[(size_t)mu * nbf + i] * C[(size_t)nu * nbf + i];
-            W[(size_t)mu * nbf + nu] = 2.0 * v;
-        }
-    std::vector<double> g2(n3);
-    int rc = qcg_eri_gradient(c, D, 1.0, g2.data());
+    HIPCHK(c, hipSetDevice(c->device));
+    const std::vector<double> W = energy_weighted_density(nbf, n_occ, C, eps);
+    // the ERI part runs on the device while the host does the one-electron part
+    int rc = eri_grad_start(c, D, 1.0);
     if (rc) return rc;
     rc = qcg_one_electron_gradient(c, D, W.data(), grad);
+    std::vector<double> g2(n3);
+    const int rf = grad_finish(c, g2.data());
     if (rc) return rc;
+    if (rf) return rf;
     for (int i = 0; i < n3; ++i) grad[i] += g2[i];
     return QCG_OK;
 }

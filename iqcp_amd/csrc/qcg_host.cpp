@@ -56,10 +56,10 @@ int qcg_create(qcg_context** out, int device) {
             const int lc = ck == 0 ? 0 : ck <= 2 ? 1 : 2, ld = (ck == 0 || ck == 1 || ck == 3) ? 0 : (ck == 2 || ck == 4) ? 1 : 2;
             const int nq = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
             // defaults close to what the autotuner picks on alkanes: the wave-cooperative kernel
-            // (block 0) above 54 components, the per-lane kernel up to 54
+            // (block 0) above kJkLaneMaxNQ components, the per-lane kernel up to it
             for (int part = 0; part < 2; ++part) {
-                c->tile[cls][part].block = nq > 54 ? 0 : (nq <= 3 ? 256 : 64);
-                c->tile[cls][part].reps = nq >= 162 ? 1 : nq > 54 ? 2 : nq >= 18 ? 4 : 1;
+                c->tile[cls][part].block = nq > kJkLaneMaxNQ ? 0 : (nq <= 3 ? 256 : 64);
+                c->tile[cls][part].reps = nq >= 162 ? 1 : nq > kJkLaneMaxNQ ? 2 : nq >= 18 ? 4 : 1;
             }
             c->class_nq[cls] = nq;
         }
@@ -502,12 +502,13 @@ int qcg_autotune(qcg_context* c, const double* dD, int iters, int* chosen) {
     // (wave-cooperative kernel, classes with at least 27 components only)
     const int cand[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {256, 1}, {64, 8}, {0, 1}, {0, 2}, {0, 4}};
     const int ncand = sizeof(cand) / sizeof(cand[0]);
-    // classes with fewer than 27 components have only the per-lane kernel, those above 54 only the
-    // wave-cooperative one: map each candidate onto what the class has
+    // classes with fewer than 27 components have only the per-lane kernel, those above kJkLaneMaxNQ
+    // only the wave-cooperative one: map each candidate onto what the class has
     auto tile_of = [&](int k, int cls) {
         const int nq = c->class_nq[cls];
         if (nq < 27) return cand[k][0] == 0 ? qcg_context::Tile{64, 1} : qcg_context::Tile{cand[k][0], cand[k][1]};
-        if (nq > 54) return cand[k][0] == 0 ? qcg_context::Tile{0, cand[k][1]} : qcg_context::Tile{0, std::min(8, cand[k][1] * 2)};
+        if (nq > kJkLaneMaxNQ)
+            return cand[k][0] == 0 ? qcg_context::Tile{0, cand[k][1]} : qcg_context::Tile{0, std::min(8, cand[k][1] * 2)};
         return qcg_context::Tile{cand[k][0], cand[k][1]};
     };
     std::vector<double> best(42, 1e300);
