@@ -225,17 +225,26 @@ void one_electron_gradient(const HostBasis& B, const double* D, const double* W,
                         return 0.5 * v;
                     };
                     const double pre = kab * std::pow(kPi / p, 1.5) * ca * cb;
-                    // nuclear attraction 1-D tables with one extra power on each side
-                    const int nr = (la + lb + 1) / 2 + 1;
-                    const int LV = kMaxL + 2;  // table dimension for i, j <= l + 1
+                    // component data of the pair: powers, norm, D and W entries
+                    int ipa[6][3], jpb[6][3];
+                    double nrm[6][6], dv[6][6];
+                    bool any_d = false;
+                    for (int ia = 0; ia < na; ++ia) {
+                        ipa[ia][0] = cart_x(la, ia); ipa[ia][1] = cart_y(la, ia); ipa[ia][2] = cart_z(la, ia);
+                    }
+                    for (int jb = 0; jb < nb; ++jb) {
+                        jpb[jb][0] = cart_x(lb, jb); jpb[jb][1] = cart_y(lb, jb); jpb[jb][2] = cart_z(lb, jb);
+                    }
                     for (int ia = 0; ia < na; ++ia)
                         for (int jb = 0; jb < nb; ++jb) {
-                            const int ip[3] = {cart_x(la, ia), cart_y(la, ia), cart_z(la, ia)};
-                            const int jp[3] = {cart_x(lb, jb), cart_y(lb, jb), cart_z(lb, jb)};
-                            const double nrm = comp_norm_host(la, ia) * comp_norm_host(lb, jb);
+                            const int* ip = ipa[ia];
+                            const int* jp = jpb[jb];
+                            nrm[ia][jb] = comp_norm_host(la, ia) * comp_norm_host(lb, jb);
                             const size_t idx = (size_t)(B.bf_off[si] + ia) * nbf + B.bf_off[sj] + jb;
-                            const double dv = D[idx], wv = W[idx];
-                            if (dv == 0.0 && wv == 0.0) continue;
+                            dv[ia][jb] = D[idx];
+                            const double wv = W[idx];
+                            any_d = any_d || dv[ia][jb] != 0.0;
+                            if (dv[ia][jb] == 0.0 && wv == 0.0) continue;
                             for (int x = 0; x < 3; ++x) {
                                 const int y = (x + 1) % 3, z = (x + 2) % 3;
                                 const double sy = sv(y, ip[y], jp[y]), sz = sv(z, ip[z], jp[z]);
@@ -248,55 +257,66 @@ void one_electron_gradient(const HostBasis& B, const double* D, const double* W,
                                 const double SA = dsA * sy * sz, SB = dsB * sy * sz;
                                 const double TA = dtA * sy * sz + dsA * (ty * sz + sy * tz);
                                 const double TB = dtB * sy * sz + dsB * (ty * sz + sy * tz);
-                                grad[3 * atA + x] += pre * nrm * (dv * TA - wv * SA);
-                                grad[3 * atB + x] += pre * nrm * (dv * TB - wv * SB);
-                            }
-                            // nuclear attraction derivative
-                            for (int cnuc = 0; cnuc < B.natom; ++cnuc) {
-                                const double* C = &B.xyz[3 * cnuc];
-                                const double PC[3] = {P[0] - C[0], P[1] - C[1], P[2] - C[2]};
-                                const double Tn = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
-                                double rx[QCG_RYS_NMAX], rw[QCG_RYS_NMAX];
-                                rys_eval_n<double>(nr, Tn, rys_tab, rx, rw);
-                                const double pv = -(double)B.Z[cnuc] * 2.0 * kPi / p * kab * ca * cb * nrm * dv;
-                                double gAx[3] = {0, 0, 0}, gBx[3] = {0, 0, 0};
-                                for (int r = 0; r < nr; ++r) {
-                                    const double u = rx[r];
-                                    double g[3][LV * LV];
-                                    for (int x = 0; x < 3; ++x) {
-                                        double h[2 * kMaxL + 3];
-                                        const double c0 = PA[x] - u * PC[x];
-                                        const double bb = (1.0 - u) * oo2p;
-                                        h[0] = 1.0;
-                                        const int nmax = la + lb + 2;
-                                        if (nmax > 0) h[1] = c0;
-                                        for (int n = 1; n < nmax; ++n) h[n + 1] = c0 * h[n] + n * bb * h[n - 1];
-                                        double t2[(2 * kMaxL + 3) * LV];
-                                        for (int n = 0; n <= nmax; ++n) t2[n * LV] = h[n];
-                                        for (int j = 1; j <= lb + 1; ++j)
-                                            for (int n = 0; n + j <= nmax; ++n)
-                                                t2[n * LV + j] = t2[(n + 1) * LV + j - 1] + AB[x] * t2[n * LV + j - 1];
-                                        for (int i = 0; i <= la + 1; ++i)
-                                            for (int j = 0; j <= lb + 1; ++j)
-                                                g[x][i * LV + j] = (i + j <= nmax) ? t2[i * LV + j] : 0.0;
-                                    }
-                                    auto gv = [&](int x, int i, int j) { return (i < 0 || j < 0) ? 0.0 : g[x][i * LV + j]; };
-                                    for (int x = 0; x < 3; ++x) {
-                                        const int y = (x + 1) % 3, z = (x + 2) % 3;
-                                        const double oth = gv(y, ip[y], jp[y]) * gv(z, ip[z], jp[z]);
-                                        gAx[x] += rw[r] * oth *
-                                                  (2.0 * a * gv(x, ip[x] + 1, jp[x]) - ip[x] * gv(x, ip[x] - 1, jp[x]));
-                                        gBx[x] += rw[r] * oth *
-                                                  (2.0 * b * gv(x, ip[x], jp[x] + 1) - jp[x] * gv(x, ip[x], jp[x] - 1));
-                                    }
-                                }
-                                for (int x = 0; x < 3; ++x) {
-                                    grad[3 * atA + x] += pv * gAx[x];
-                                    grad[3 * atB + x] += pv * gBx[x];
-                                    grad[3 * cnuc + x] -= pv * (gAx[x] + gBx[x]);
-                                }
+                                grad[3 * atA + x] += pre * nrm[ia][jb] * (dv[ia][jb] * TA - wv * SA);
+                                grad[3 * atB + x] += pre * nrm[ia][jb] * (dv[ia][jb] * TB - wv * SB);
                             }
                         }
+                    if (!any_d) continue;
+                    // nuclear attraction derivative: per nucleus one Rys rule and one set of 1-D
+                    // tables (with one extra power on each side), shared by all component pairs
+                    const int nr = (la + lb + 1) / 2 + 1;
+                    constexpr int LV = kMaxL + 2;  // table dimension for i, j <= l + 1
+                    const int nmax = la + lb + 2;
+                    for (int cnuc = 0; cnuc < B.natom; ++cnuc) {
+                        const double* C = &B.xyz[3 * cnuc];
+                        const double PC[3] = {P[0] - C[0], P[1] - C[1], P[2] - C[2]};
+                        const double Tn = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+                        double rx[QCG_RYS_NMAX], rw[QCG_RYS_NMAX];
+                        rys_eval_n<double>(nr, Tn, rys_tab, rx, rw);
+                        double g[QCG_RYS_NMAX][3][LV * LV];
+                        for (int r = 0; r < nr; ++r) {
+                            const double u = rx[r];
+                            for (int x = 0; x < 3; ++x) {
+                                double h[2 * kMaxL + 3];
+                                const double c0 = PA[x] - u * PC[x];
+                                const double bb = (1.0 - u) * oo2p;
+                                h[0] = 1.0;
+                                if (nmax > 0) h[1] = c0;
+                                for (int n = 1; n < nmax; ++n) h[n + 1] = c0 * h[n] + n * bb * h[n - 1];
+                                double t2[(2 * kMaxL + 3) * LV];
+                                for (int n = 0; n <= nmax; ++n) t2[n * LV] = h[n];
+                                for (int j = 1; j <= lb + 1; ++j)
+                                    for (int n = 0; n + j <= nmax; ++n)
+                                        t2[n * LV + j] = t2[(n + 1) * LV + j - 1] + AB[x] * t2[n * LV + j - 1];
+                                for (int i = 0; i <= la + 1; ++i)
+                                    for (int j = 0; j <= lb + 1; ++j)
+                                        g[r][x][i * LV + j] = (i + j <= nmax) ? t2[i * LV + j] : 0.0;
+                            }
+                        }
+                        const double pz = -(double)B.Z[cnuc] * 2.0 * kPi / p * kab * ca * cb;
+                        double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0};
+                        for (int ia = 0; ia < na; ++ia)
+                            for (int jb = 0; jb < nb; ++jb) {
+                                if (dv[ia][jb] == 0.0) continue;
+                                const int* ip = ipa[ia];
+                                const int* jp = jpb[jb];
+                                const double pv = pz * nrm[ia][jb] * dv[ia][jb];
+                                for (int r = 0; r < nr; ++r) {
+                                    auto gv = [&](int x, int i, int j) { return (i < 0 || j < 0) ? 0.0 : g[r][x][i * LV + j]; };
+                                    for (int x = 0; x < 3; ++x) {
+                                        const int y = (x + 1) % 3, z = (x + 2) % 3;
+                                        const double oth = rw[r] * pv * gv(y, ip[y], jp[y]) * gv(z, ip[z], jp[z]);
+                                        gA[x] += oth * (2.0 * a * gv(x, ip[x] + 1, jp[x]) - ip[x] * gv(x, ip[x] - 1, jp[x]));
+                                        gB[x] += oth * (2.0 * b * gv(x, ip[x], jp[x] + 1) - jp[x] * gv(x, ip[x], jp[x] - 1));
+                                    }
+                                }
+                            }
+                        for (int x = 0; x < 3; ++x) {
+                            grad[3 * atA + x] += gA[x];
+                            grad[3 * atB + x] += gB[x];
+                            grad[3 * cnuc + x] -= gA[x] + gB[x];
+                        }
+                    }
                 }
         }
 #pragma omp critical
