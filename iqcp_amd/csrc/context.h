@@ -108,21 +108,21 @@ struct qcg_context {
     double *d_pts = nullptr, *d_w = nullptr, *d_phi = nullptr, *d_dphx = nullptr, *d_dphy = nullptr,
            *d_dphz = nullptr, *d_X = nullptr, *d_aow = nullptr, *d_V = nullptr, *d_exc = nullptr,
            *d_Vparts = nullptr;  // split-K partial products of V = Phi^T aow
-    int v_chunks = 0;
+    int v_chunks = 0, v_chunks_mfma = 0;
     // XC build (qcg_set_xc_mode): 0 = FP32 MFMA kernels for the mixed option and rocBLAS DGEMMs
     // around point_kernel for FP64; 1 = rocBLAS DGEMMs for both; 2 = MFMA kernels for both
     int xc_mode = 0;
     // FP32 copies of Phi and grad Phi plus FP32 work arrays for the mixed-precision XC build,
     // made on its first use for the current grid
-    float *d_phi32 = nullptr, *d_dphx32 = nullptr, *d_dphy32 = nullptr, *d_dphz32 = nullptr, *d_aow32 = nullptr,
-          *d_D32 = nullptr;
+    float *d_phi32 = nullptr, *d_aow32 = nullptr, *d_D32 = nullptr;
+    float4* d_grid4 = nullptr;  // (Phi, dPhi/dx, dPhi/dy, dPhi/dz) per element, FP32
     GridShell* d_gshell = nullptr;
     double *d_gexp = nullptr, *d_gcn = nullptr;
     void free_grid() {
         for (void** p : {(void**)&d_pts, (void**)&d_w, (void**)&d_phi, (void**)&d_dphx, (void**)&d_dphy,
                          (void**)&d_dphz, (void**)&d_X, (void**)&d_aow, (void**)&d_V, (void**)&d_exc, (void**)&d_Vparts,
-                         (void**)&d_gshell, (void**)&d_gexp, (void**)&d_gcn, (void**)&d_phi32, (void**)&d_dphx32,
-                         (void**)&d_dphy32, (void**)&d_dphz32, (void**)&d_aow32, (void**)&d_D32}) {
+                         (void**)&d_gshell, (void**)&d_gexp, (void**)&d_gcn, (void**)&d_phi32, (void**)&d_grid4,
+                         (void**)&d_aow32, (void**)&d_D32}) {
             release(*p);
             *p = nullptr;
         }

@@ -47,7 +47,10 @@ int upload_grid(qcg_context* c) {
     // kVChunk points, one strided-batched GEMM over the chunks plus a sum, so the GEMM has
     // chunks x tiles workgroups instead of a handful of tiles with a 10^6-long K loop
     c->v_chunks = (int)std::max<int64_t>(1, G / kVChunk);
-    HIPCHK(c, c->alloc(&c->d_Vparts, (size_t)c->v_chunks * B.nbf * B.nbf));
+    // the MFMA V kernel: enough grid chunks for about 2048 blocks of (chunk, output tile), at
+    // least 1024 points per chunk
+    c->v_chunks_mfma = (int)std::max<int64_t>(1, std::min<int64_t>(G / 1024, 2048 / xc_mfma_vtiles(B.nbf)));
+    HIPCHK(c, c->alloc(&c->d_Vparts, (size_t)std::max(c->v_chunks, c->v_chunks_mfma) * B.nbf * B.nbf));
     HIPCHK(c, c->alloc(&c->d_exc, 1));
     HIPCHK(c, hipMemcpy(c->d_pts, c->grid_xyz.data(), 3 * (size_t)G * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_w, c->grid_w.data(), (size_t)G * sizeof(double), hipMemcpyHostToDevice));
@@ -88,34 +91,30 @@ int upload_grid(qcg_context* c) {
 
 // FP32 copies of Phi and grad Phi (and FP32 work arrays) for the mixed-precision XC build
 int ensure_xc32(qcg_context* c, hipStream_t st) {
-    if (c->d_phi32) return QCG_OK;
+    if (c->d_phi32 && c->d_grid4 && c->d_aow32 && c->d_D32) return QCG_OK;
     const size_t gn = (size_t)c->G * c->basis.nbf, nn = (size_t)c->basis.nbf * c->basis.nbf;
     HIPCHK(c, c->alloc(&c->d_phi32, gn));
-    HIPCHK(c, c->alloc(&c->d_dphx32, gn));
-    HIPCHK(c, c->alloc(&c->d_dphy32, gn));
-    HIPCHK(c, c->alloc(&c->d_dphz32, gn));
+    HIPCHK(c, c->alloc(&c->d_grid4, gn));
     HIPCHK(c, c->alloc(&c->d_aow32, gn));
     HIPCHK(c, c->alloc(&c->d_D32, nn));
     HIPCHK(c, launch_to_f32(c->d_phi, c->d_phi32, gn, st));
-    HIPCHK(c, launch_to_f32(c->d_dphx, c->d_dphx32, gn, st));
-    HIPCHK(c, launch_to_f32(c->d_dphy, c->d_dphy32, gn, st));
-    HIPCHK(c, launch_to_f32(c->d_dphz, c->d_dphz32, gn, st));
+    HIPCHK(c, launch_to_f32x4(c->d_phi, c->d_dphx, c->d_dphy, c->d_dphz, c->d_grid4, gn, st));
     return QCG_OK;
 }
 
 // V_xc and E_xc with the MFMA kernels (xc_mfma.hip): FP32 products in mixed precision, FP64 otherwise
 int xc_build_mfma(qcg_context* c, const double* dD, double* dV, double* dExc, hipStream_t st) {
     const int nbf = c->basis.nbf, G = (int)c->G;
-    const int nch = xc_mfma_chunks(G, c->v_chunks);
+    const int nch = xc_mfma_chunks(G, c->v_chunks_mfma);
     HIPCHK(c, launch_fill(dExc, 1, 0.0, st));
     if (c->precision == QCG_PREC_MIXED) {
         int rc = ensure_xc32(c, st);
         if (rc) return rc;
         HIPCHK(c, launch_to_f32(dD, c->d_D32, (size_t)nbf * nbf, st));
-        HIPCHK(c, launch_xc_mfma(G, nbf, c->v_chunks, c->d_w, c->d_phi32, c->d_dphx32, c->d_dphy32, c->d_dphz32,
-                                 c->d_D32, c->d_aow32, dExc, c->d_Vparts, st));
+        HIPCHK(c, launch_xc_mfma(G, nbf, c->v_chunks_mfma, c->d_w, c->d_phi32, c->d_grid4, c->d_D32, c->d_aow32, dExc,
+                                 c->d_Vparts, st));
     } else {
-        HIPCHK(c, launch_xc_mfma(G, nbf, c->v_chunks, c->d_w, c->d_phi, c->d_dphx, c->d_dphy, c->d_dphz, dD, c->d_aow,
+        HIPCHK(c, launch_xc_mfma(G, nbf, c->v_chunks_mfma, c->d_w, c->d_phi, c->d_dphx, c->d_dphy, c->d_dphz, dD, c->d_aow,
                                  dExc, c->d_Vparts, st));
     }
     HIPCHK(c, launch_sum_parts(c->d_Vparts, nch, (size_t)nbf * nbf, c->d_V, st));

@@ -22,17 +22,25 @@ hipError_t launch_sym_half(const double* V, double* out, int n, hipStream_t st);
 // MFMA form of the XC build (xc_mfma.hip): E_xc added to exc_sum, aow rows written, and the
 // partial products V_k = Phi[chunk k]^T aow[chunk k] in Vparts for k < xc_mfma_chunks(G, nchunk)
 // (at most nchunk); the caller sums them and symmetrises.
-hipError_t launch_xc_mfma(int G, int nbf, int nchunk, const double* w, const float* phi, const float* dpx,
-                          const float* dpy, const float* dpz, const float* Dm, float* aow, double* exc_sum,
-                          double* Vparts, hipStream_t st);
+// FP32: phi for the GEMM operands, g4 = (Phi, dPhi/dx, dPhi/dy, dPhi/dz) interleaved per element
+hipError_t launch_xc_mfma(int G, int nbf, int nchunk, const double* w, const float* phi, const float4* g4,
+                          const float* Dm, float* aow, double* exc_sum, double* Vparts, hipStream_t st);
 hipError_t launch_xc_mfma(int G, int nbf, int nchunk, const double* w, const double* phi, const double* dpx,
                           const double* dpy, const double* dpz, const double* Dm, double* aow, double* exc_sum,
                           double* Vparts, hipStream_t st);
+// output tiles of the MFMA V kernel (128 x 128 tiles of the nbf x nbf matrix)
+inline int xc_mfma_vtiles(int nbf) {
+    const int t = (nbf + 127) / 128;
+    return t * t;
+}
 inline int xc_mfma_chunks(int G, int nchunk) {
     const int chunk = (G + nchunk - 1) / nchunk;
     return (G + chunk - 1) / chunk;
 }
 hipError_t launch_to_f32(const double* in, float* out, size_t n, hipStream_t st);
+// out[i] = (a[i], b[i], c[i], d[i]) in FP32
+hipError_t launch_to_f32x4(const double* a, const double* b, const double* c, const double* d, float4* out, size_t n,
+                           hipStream_t st);
 // exc_sum += sum_i V[i] (D[i] - D32[i]): the first-order FP64 correction of E_xc for the FP32
 // rounding of the density matrix (V = V_xc of the same build)
 hipError_t launch_exc_d32_correction(const double* V, const double* D, const float* D32, size_t n, double* exc_sum,

@@ -4,7 +4,7 @@
 //                   grad rho (X never goes to HBM), then B3LYP per point (FP64), the aow rows and
 //                   E_xc -- the work of compute_density_and_gradient_matmul and the point part of
 //                   build_vxc_gga_matmul (dft/ks_scf.rs:1028-1093, :1150-1296) in one kernel;
-//   xc_v_kernel   : V = Phi^T aow as 64x64 output tiles over grid chunks (split K), one partial
+//   xc_v_kernel   : V = Phi^T aow as 128x128 output tiles over grid chunks (split K), one partial
 //                   matrix per chunk, summed and symmetrised afterwards (sum_parts, sym_half).
 // rho, grad rho, E_xc and the functional are accumulated in FP64 whatever T is; with T = float the
 // GEMM products are FP32 MFMA with FP32 accumulation.
@@ -38,18 +38,38 @@ struct Mfma<double> {
     static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
 };
 
-constexpr int kRhoBM = 64;  // grid points per block, 16 per wavefront
-constexpr int kRhoBK = 16;  // basis functions per K step
-constexpr int kPad = 16;    // LDS row padding: the 16 lanes of a fragment row hit distinct banks
-constexpr int kVBM = 64;    // V output tile (64 x 64), 16 rows per wavefront
-constexpr int kVBK = 16;    // grid points per K step
+constexpr int kRhoMT = 1;                 // MFMA row tiles (16 points each) per wavefront
+constexpr int kRhoBM = 4 * 16 * kRhoMT;   // grid points per block
+constexpr int kRhoPW = 16 * kRhoMT;       // grid points per wavefront
+constexpr int kRhoBK = 16;                // basis functions per K step
+constexpr int kPad = 16;                  // LDS row padding: the lanes of a fragment row hit distinct banks
+constexpr int kVW = 4;                    // V: MFMA tiles per wavefront and dimension (64 x 64)
+constexpr int kVBM = 2 * 16 * kVW;        // V output tile per block (2 x 2 wavefronts): 128 x 128
+constexpr int kVBK = 16;                  // grid points per K step
+static_assert(kVBM == 128, "xc_mfma_vtiles (xc_launch.h) counts 128 x 128 output tiles");
 
 // NT: MFMA column tiles (16 columns each) of X per pass over the columns; a pass covers 16 NT
-// columns, all passes together the nbf columns of X.
+// columns, all passes together the nbf columns of X.  Each wavefront owns kRhoPW points (kRhoMT
+// row tiles), so every D fragment read from LDS feeds kRhoMT MFMAs.
+// Phi, dPhi/dx, dPhi/dy, dPhi/dz at (point g, function mu) = element idx = g nbf + mu: FP32 from
+// one interleaved float4 array (one 16-byte load), FP64 from the four separate arrays.
+template <typename T>
+struct GridVals {
+    const T *phi, *dpx, *dpy, *dpz;
+    const float4* g4;
+    __device__ __forceinline__ void get(size_t idx, double& f, double& x, double& y, double& z) const {
+        if constexpr (sizeof(T) == 4) {
+            const float4 v = g4[idx];
+            f = v.x; x = v.y; y = v.z; z = v.w;
+        } else {
+            f = phi[idx]; x = dpx[idx]; y = dpy[idx]; z = dpz[idx];
+        }
+    }
+};
+
 template <typename T, int NT>
 __global__ __launch_bounds__(256) void xc_rho_kernel(int G, int nbf, const double* __restrict__ w,
-                                                     const T* __restrict__ phi, const T* __restrict__ dpx,
-                                                     const T* __restrict__ dpy, const T* __restrict__ dpz,
+                                                     const T* __restrict__ phi, const GridVals<T> gv,
                                                      const T* __restrict__ Dm, T* __restrict__ aow,
                                                      double* __restrict__ exc_sum) {
     using M = Mfma<T>;
@@ -59,13 +79,16 @@ __global__ __launch_bounds__(256) void xc_rho_kernel(int G, int nbf, const doubl
     static_assert(kRhoBM * kRhoBK % 256 == 0 && kRhoBK * BN % 256 == 0, "tile sizes");
     __shared__ T As[kRhoBK][kRhoBM + kPad];  // Phi block, transposed: As[k][point]
     __shared__ T Bs[kRhoBK][BN + kPad];      // rows k0.. of D, this pass's columns
-    __shared__ double red[4][16][4];         // per wavefront and point: rho, d rho / dx, dy, dz sums
-    __shared__ double coef[4][16][4];        // per wavefront and point: w vrho, 4 w vsigma grad rho
+    __shared__ double red[4][kRhoPW][4];     // per wavefront and point: rho, d rho / dx, dy, dz sums
+    __shared__ double coef[4][kRhoPW][4];    // per wavefront and point: w vrho, 4 w vsigma grad rho
     __shared__ double part[4];
     const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, li = l & 15, lk = l >> 4;
-    const int g0 = blockIdx.x * kRhoBM;
-    double rs[4] = {0.0, 0.0, 0.0, 0.0}, xs[4] = {0.0, 0.0, 0.0, 0.0}, ys[4] = {0.0, 0.0, 0.0, 0.0},
-           zs[4] = {0.0, 0.0, 0.0, 0.0};
+    const int g0 = blockIdx.x * kRhoBM, gw = g0 + kRhoPW * wv;
+    double rs[kRhoMT][4], xs[kRhoMT][4], ys[kRhoMT][4], zs[kRhoMT][4];
+#pragma unroll
+    for (int m = 0; m < kRhoMT; ++m)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) rs[m][r] = xs[m][r] = ys[m][r] = zs[m][r] = 0.0;
     T ra[NA], rb[NB];  // next K step's tiles, loaded while the current one is multiplied
     auto load = [&](int n0, int k0) {
 #pragma unroll
@@ -80,9 +103,11 @@ __global__ __launch_bounds__(256) void xc_rho_kernel(int G, int nbf, const doubl
         }
     };
     for (int n0 = 0; n0 < nbf; n0 += BN) {
-        acc_t acc[NT];
+        acc_t acc[kRhoMT][NT];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) acc[t] = acc_t{T(0), T(0), T(0), T(0)};
+        for (int m = 0; m < kRhoMT; ++m)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[m][t] = acc_t{T(0), T(0), T(0), T(0)};
         load(n0, 0);
         for (int k0 = 0; k0 < nbf; k0 += kRhoBK) {
             __syncthreads();  // the previous step's fragment reads are done
@@ -100,9 +125,15 @@ __global__ __launch_bounds__(256) void xc_rho_kernel(int G, int nbf, const doubl
             if (k0 + kRhoBK < nbf) load(n0, k0 + kRhoBK);
 #pragma unroll
             for (int ks = 0; ks < kRhoBK; ks += 4) {
-                const T a = As[ks + lk][16 * wv + li];
+                T a[kRhoMT];
 #pragma unroll
-                for (int t = 0; t < NT; ++t) acc[t] = M::mma(a, Bs[ks + lk][16 * t + li], acc[t]);
+                for (int m = 0; m < kRhoMT; ++m) a[m] = As[ks + lk][kRhoPW * wv + 16 * m + li];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const T b = Bs[ks + lk][16 * t + li];
+#pragma unroll
+                    for (int m = 0; m < kRhoMT; ++m) acc[m][t] = M::mma(a[m], b, acc[m][t]);
+                }
             }
         }
         // rho = sum_mu X Phi, grad rho / 2 = sum_mu X grad Phi over this pass's columns
@@ -111,46 +142,54 @@ __global__ __launch_bounds__(256) void xc_rho_kernel(int G, int nbf, const doubl
             const int col = n0 + 16 * t + li;
             if (col < nbf) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int g = g0 + 16 * wv + M::row(lk, r);
-                    if (g < G) {
-                        const size_t idx = (size_t)g * nbf + col;
-                        const double x = (double)acc[t][r];
-                        rs[r] += x * (double)phi[idx];
-                        xs[r] += x * (double)dpx[idx];
-                        ys[r] += x * (double)dpy[idx];
-                        zs[r] += x * (double)dpz[idx];
+                for (int m = 0; m < kRhoMT; ++m)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int g = gw + 16 * m + M::row(lk, r);
+                        if (g < G) {
+                            double f, dx, dy, dz;
+                            gv.get((size_t)g * nbf + col, f, dx, dy, dz);
+                            const double x = (double)acc[m][t][r];
+                            rs[m][r] += x * f;
+                            xs[m][r] += x * dx;
+                            ys[m][r] += x * dy;
+                            zs[m][r] += x * dz;
+                        }
                     }
-                }
             }
-            // a bounded number of tiles' loads in flight: hoisting all of them costs hundreds of VGPRs
-            if (t % 2 == 1) asm volatile("" ::: "memory");
+            // one tile's loads in flight at a time: hoisting all of them costs hundreds of VGPRs
+            asm volatile("" ::: "memory");
         }
     }
     // sums over the 16 lanes that share a point row (same l / 16)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-#pragma unroll
-        for (int off = 1; off < 16; off <<= 1) {
-            rs[r] += __shfl_xor(rs[r], off, 64);
-            xs[r] += __shfl_xor(xs[r], off, 64);
-            ys[r] += __shfl_xor(ys[r], off, 64);
-            zs[r] += __shfl_xor(zs[r], off, 64);
-        }
-    }
-    if (li == 0) {
+    for (int m = 0; m < kRhoMT; ++m)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            red[wv][M::row(lk, r)][0] = rs[r];
-            red[wv][M::row(lk, r)][1] = xs[r];
-            red[wv][M::row(lk, r)][2] = ys[r];
-            red[wv][M::row(lk, r)][3] = zs[r];
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                rs[m][r] += __shfl_xor(rs[m][r], off, 64);
+                xs[m][r] += __shfl_xor(xs[m][r], off, 64);
+                ys[m][r] += __shfl_xor(ys[m][r], off, 64);
+                zs[m][r] += __shfl_xor(zs[m][r], off, 64);
+            }
         }
+    if (li == 0) {
+#pragma unroll
+        for (int m = 0; m < kRhoMT; ++m)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int p = 16 * m + M::row(lk, r);
+                red[wv][p][0] = rs[m][r];
+                red[wv][p][1] = xs[m][r];
+                red[wv][p][2] = ys[m][r];
+                red[wv][p][3] = zs[m][r];
+            }
     }
     __syncthreads();
     double contrib = 0.0;
-    if (l < 16) {
-        const int g = g0 + 16 * wv + l;
+    if (l < kRhoPW) {
+        const int g = gw + l;
         double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
         if (g < G) {
             const double rho = fmax(red[wv][l][0], 0.0);
@@ -171,14 +210,15 @@ __global__ __launch_bounds__(256) void xc_rho_kernel(int G, int nbf, const doubl
     }
     __syncthreads();
     // aow = w vrho Phi + 4 w vsigma (grad rho . grad Phi), row by row of the wavefront's points
-    for (int p = 0; p < 16; ++p) {
-        const int g = g0 + 16 * wv + p;
+    for (int p = 0; p < kRhoPW; ++p) {
+        const int g = gw + p;
         if (g >= G) break;  // uniform over the wavefront
         const double c0 = coef[wv][p][0], c1 = coef[wv][p][1], c2 = coef[wv][p][2], c3 = coef[wv][p][3];
         const size_t row = (size_t)g * nbf;
         for (int col = l; col < nbf; col += 64) {
-            const size_t idx = row + col;
-            aow[idx] = (T)(c0 * (double)phi[idx] + c1 * (double)dpx[idx] + c2 * (double)dpy[idx] + c3 * (double)dpz[idx]);
+            double f, dx, dy, dz;
+            gv.get(row + col, f, dx, dy, dz);
+            aow[row + col] = (T)(c0 * f + c1 * dx + c2 * dy + c3 * dz);
         }
     }
 #pragma unroll
@@ -191,9 +231,10 @@ __global__ __launch_bounds__(256) void xc_rho_kernel(int G, int nbf, const doubl
     }
 }
 
-// Vparts[chunk] = Phi[chunk rows]^T aow[chunk rows]; block -> (chunk, tile) so that the tiles of one
-// chunk share an XCD (blocks are dealt round-robin over the 8 XCDs) and reuse its Phi and aow rows
-// from that XCD's L2.
+// Vparts[chunk] = Phi[chunk rows]^T aow[chunk rows] by 128 x 128 output tiles (each wavefront a
+// 64 x 64 quarter: every fragment read feeds kVW MFMAs); block -> (chunk, tile) so that the tiles of
+// one chunk share an XCD (blocks are dealt round-robin over the 8 XCDs) and reuse its Phi and aow
+// rows from that XCD's L2.
 template <typename T>
 __global__ __launch_bounds__(256) void xc_v_kernel(int G, int nbf, int ntile, int nchunk, int chunk,
                                                    const T* __restrict__ phi, const T* __restrict__ aow,
@@ -203,14 +244,17 @@ __global__ __launch_bounds__(256) void xc_v_kernel(int G, int nbf, int ntile, in
     __shared__ T As[kVBK][kVBM + kPad];  // Phi[g][m0 + i]
     __shared__ T Bs[kVBK][kVBM + kPad];  // aow[g][n0 + j]
     const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, li = l & 15, lk = l >> 4;
+    const int wr = wv >> 1, wc = wv & 1;  // this wavefront's quarter of the tile
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nt2 = ntile * ntile;
     const int ch = (slot / nt2) * 8 + xcd, tile = slot % nt2;
     if (ch >= nchunk) return;  // the whole block: before any barrier
     const int m0 = (tile / ntile) * kVBM, n0 = (tile % ntile) * kVBM;
     const int gb = ch * chunk, ge = min(G, gb + chunk);
-    acc_t acc[4];
+    acc_t acc[kVW][kVW];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = acc_t{T(0), T(0), T(0), T(0)};
+    for (int i = 0; i < kVW; ++i)
+#pragma unroll
+        for (int j = 0; j < kVW; ++j) acc[i][j] = acc_t{T(0), T(0), T(0), T(0)};
     constexpr int NE = kVBK * kVBM / 256;  // staged elements per thread and operand
     T ra[NE], rb[NE];                       // next K step's rows, loaded during the current MFMAs
     auto load = [&](int k0) {
@@ -235,21 +279,30 @@ __global__ __launch_bounds__(256) void xc_v_kernel(int G, int nbf, int ntile, in
         if (k0 + kVBK < ge) load(k0 + kVBK);
 #pragma unroll
         for (int ks = 0; ks < kVBK; ks += 4) {
-            const T a = As[ks + lk][16 * wv + li];
+            T a[kVW], b[kVW];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) acc[t] = M::mma(a, Bs[ks + lk][16 * t + li], acc[t]);
+            for (int i = 0; i < kVW; ++i) {
+                a[i] = As[ks + lk][64 * wr + 16 * i + li];
+                b[i] = Bs[ks + lk][64 * wc + 16 * i + li];
+            }
+#pragma unroll
+            for (int i = 0; i < kVW; ++i)
+#pragma unroll
+                for (int j = 0; j < kVW; ++j) acc[i][j] = M::mma(a[i], b[j], acc[i][j]);
         }
     }
     double* out = Vparts + (size_t)ch * nbf * nbf;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const int col = n0 + 16 * t + li;
+    for (int i = 0; i < kVW; ++i)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = m0 + 16 * wv + M::row(lk, r);
-            if (row < nbf && col < nbf) out[(size_t)row * nbf + col] = (double)acc[t][r];
+        for (int j = 0; j < kVW; ++j) {
+            const int col = n0 + 64 * wc + 16 * j + li;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + 64 * wr + 16 * i + M::row(lk, r);
+                if (row < nbf && col < nbf) out[(size_t)row * nbf + col] = (double)acc[i][j][r];
+            }
         }
-    }
 }
 
 // FP64 correction of E_xc for the rounding of D to FP32: dE_xc/dD = V_xc, so
@@ -280,6 +333,20 @@ hipError_t launch_exc_d32_correction(const double* V, const double* D, const flo
     return hipGetLastError();
 }
 
+__global__ void to_f32x4_kernel(const double* __restrict__ a, const double* __restrict__ b,
+                                const double* __restrict__ c, const double* __restrict__ d, float4* __restrict__ out,
+                                size_t n) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_float4((float)a[i], (float)b[i], (float)c[i], (float)d[i]);
+}
+
+hipError_t launch_to_f32x4(const double* a, const double* b, const double* c, const double* d, float4* out, size_t n,
+                           hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(to_f32x4_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, b, c, d, out, n);
+    return hipGetLastError();
+}
+
 __global__ void to_f32_kernel(const double* __restrict__ in, float* __restrict__ out, size_t n) {
     const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i < n) out[i] = (float)in[i];
@@ -293,8 +360,9 @@ hipError_t launch_to_f32(const double* in, float* out, size_t n, hipStream_t st)
 
 template <typename T>
 static hipError_t launch_xc_mfma_t(int G, int nbf, int nchunk, const double* w, const T* phi, const T* dpx,
-                                   const T* dpy, const T* dpz, const T* Dm, T* aow, double* exc_sum, double* Vparts,
-                                   hipStream_t st) {
+                                   const T* dpy, const T* dpz, const float4* g4, const T* Dm, T* aow, double* exc_sum,
+                                   double* Vparts, hipStream_t st) {
+    const GridVals<T> gv{phi, dpx, dpy, dpz, g4};
     if (G <= 0 || nbf <= 0 || nchunk <= 0) return hipErrorInvalidValue;
     // column passes of at most 8 MFMA tiles, with the tiles spread evenly over the passes
     const int tiles = (nbf + 15) / 16, passes = (tiles + 7) / 8, nt = (tiles + passes - 1) / passes;
@@ -302,8 +370,8 @@ static hipError_t launch_xc_mfma_t(int G, int nbf, int nchunk, const double* w, 
     switch (nt) {
 #define QCG_RHO_NT(n)                                                                                   \
     case n:                                                                                             \
-        hipLaunchKernelGGL((xc_rho_kernel<T, n>), grid, dim3(256), 0, st, G, nbf, w, phi, dpx, dpy, dpz, Dm, \
-                           aow, exc_sum);                                                               \
+        hipLaunchKernelGGL((xc_rho_kernel<T, n>), grid, dim3(256), 0, st, G, nbf, w, phi, gv, Dm, aow,      \
+                           exc_sum);                                                                    \
         break;
         QCG_RHO_NT(1) QCG_RHO_NT(2) QCG_RHO_NT(3) QCG_RHO_NT(4) QCG_RHO_NT(5) QCG_RHO_NT(6) QCG_RHO_NT(7) QCG_RHO_NT(8)
 #undef QCG_RHO_NT
@@ -319,16 +387,16 @@ static hipError_t launch_xc_mfma_t(int G, int nbf, int nchunk, const double* w, 
     return hipGetLastError();
 }
 
-hipError_t launch_xc_mfma(int G, int nbf, int nchunk, const double* w, const float* phi, const float* dpx,
-                          const float* dpy, const float* dpz, const float* Dm, float* aow, double* exc_sum,
-                          double* Vparts, hipStream_t st) {
-    return launch_xc_mfma_t<float>(G, nbf, nchunk, w, phi, dpx, dpy, dpz, Dm, aow, exc_sum, Vparts, st);
+hipError_t launch_xc_mfma(int G, int nbf, int nchunk, const double* w, const float* phi, const float4* g4,
+                          const float* Dm, float* aow, double* exc_sum, double* Vparts, hipStream_t st) {
+    return launch_xc_mfma_t<float>(G, nbf, nchunk, w, phi, nullptr, nullptr, nullptr, g4, Dm, aow, exc_sum, Vparts,
+                                   st);
 }
 
 hipError_t launch_xc_mfma(int G, int nbf, int nchunk, const double* w, const double* phi, const double* dpx,
                           const double* dpy, const double* dpz, const double* Dm, double* aow, double* exc_sum,
                           double* Vparts, hipStream_t st) {
-    return launch_xc_mfma_t<double>(G, nbf, nchunk, w, phi, dpx, dpy, dpz, Dm, aow, exc_sum, Vparts, st);
+    return launch_xc_mfma_t<double>(G, nbf, nchunk, w, phi, dpx, dpy, dpz, nullptr, Dm, aow, exc_sum, Vparts, st);
 }
 
 }  // namespace qcg
