@@ -232,6 +232,32 @@ def test_gpu_set_tiles_rejects_bad_tiles():
         eng.set_tiles([(32, 1)] * 21)
     with pytest.raises(Exception):
         eng.set_tiles([(64, 0)] * 21)
+    with pytest.raises(Exception):
+        eng.set_tiles([(512, 1)] * 21)  # the kernels are built for at most 256 threads
+    with pytest.raises(Exception):
+        eng.set_prim_split(-1)
+
+
+@pytest.mark.parametrize("split", [1, 3, 7])
+def test_gpu_prim_split_keeps_parity(split):
+    """Work items cut into bra primitive ranges give the same J/K (per-lane and wave-cooperative
+    kernels, FP64 and mixed) and the same ERI gradient as whole bra pairs."""
+    b = bench_molecule("NH3", "6-31+g*")
+    eri, _ = oracle.eri_compressed(b)
+    D = random_density(b.nbf, seed=13, nocc=b.n_occ)
+    Jo, Ko = oracle.build_jk(D, eri, b.nbf)
+    eng = _eng(b)
+    g0 = eng.eri_gradient(D, 1.0)
+    eng.set_prim_split(split)
+    for tile in ((64, 2), (0, 2)):
+        eng.set_tiles([tile] * 21)
+        for prec, tol in (("fp64", 1e-11), ("mixed", 1e-8)):
+            eng.set_options(prec, 1e-6)
+            jk = eng.build_jk(D)
+            np.testing.assert_allclose(jk.J, Jo, atol=tol, rtol=0, err_msg=f"{tile} {prec}")
+            np.testing.assert_allclose(jk.K, Ko, atol=tol, rtol=0, err_msg=f"{tile} {prec}")
+    eng.set_options("fp64")
+    np.testing.assert_allclose(eng.eri_gradient(D, 1.0), g0, atol=1e-11, rtol=0)
 
 
 # ------------------------------------------------------------------ SCF energies vs IQCP
