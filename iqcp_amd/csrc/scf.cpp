@@ -153,13 +153,20 @@ void ScfRun::step() {
     if (damp)
         for (size_t i = 0; i < n2; ++i) Fd[i] = F_last[i] * cfg.damp + F[i] * (1.0 - cfg.damp);
     std::vector<double> Fdiag = Fd;
-    if (cfg.use_diis && iter >= cfg.diis_start) {
-        std::vector<double> FD(n2), FDS(n2), SD(n2), SDF(n2), Err(n2);
-        matmul(Fd.data(), D.data(), FD.data(), n);
-        matmul(FD.data(), S.data(), FDS.data(), n);
-        matmul(S.data(), D.data(), SD.data(), n);
-        matmul(SD.data(), Fd.data(), SDF.data(), n);
-        for (size_t i = 0; i < n2; ++i) Err[i] = FDS[i] - SDF[i];
+    // D S, shared by the DIIS error and the level shift; F, D and S are symmetric, so
+    // S D F = (F D S)^T and the error FDS - SDF needs one product more
+    std::vector<double> DS;
+    const bool diis_now = cfg.use_diis && iter >= cfg.diis_start;
+    const bool shift = std::fabs(cfg.level_shift) >= 1e-10;
+    if (diis_now || shift) {
+        DS.resize(n2);
+        matmul(D.data(), S.data(), DS.data(), n);
+    }
+    if (diis_now) {
+        std::vector<double> FDS(n2), Err(n2);
+        matmul(Fd.data(), DS.data(), FDS.data(), n);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) Err[(size_t)i * n + j] = FDS[(size_t)i * n + j] - FDS[(size_t)j * n + i];
         if ((int)diis.f.size() >= diis.max_size) {
             diis.f.pop_front();
             diis.e.pop_front();
@@ -168,10 +175,9 @@ void ScfRun::step() {
         diis.e.push_back(Err);
         if (diis.f.size() >= 2) Fdiag = diis.extrapolate();
     }
-    if (std::fabs(cfg.level_shift) >= 1e-10) {
-        std::vector<double> SD(n2), SDS(n2);
-        matmul(S.data(), D.data(), SD.data(), n);
-        matmul(SD.data(), S.data(), SDS.data(), n);
+    if (shift) {
+        std::vector<double> SDS(n2);
+        matmul(S.data(), DS.data(), SDS.data(), n);
         for (size_t i = 0; i < n2; ++i) Fdiag[i] += cfg.level_shift * (S[i] - 0.5 * SDS[i]);
     }
     diagonalize(Fdiag);
