@@ -364,8 +364,10 @@ static hipError_t launch_xc_mfma_t(int G, int nbf, int nchunk, const double* w, 
                                    double* Vparts, hipStream_t st) {
     const GridVals<T> gv{phi, dpx, dpy, dpz, g4};
     if (G <= 0 || nbf <= 0 || nchunk <= 0) return hipErrorInvalidValue;
-    // column passes of at most 8 MFMA tiles, with the tiles spread evenly over the passes
-    const int tiles = (nbf + 15) / 16, passes = (tiles + 7) / 8, nt = (tiles + passes - 1) / passes;
+    // column passes of at most 8 MFMA tiles (4 for FP64, whose accumulators take twice the
+    // registers), with the tiles spread evenly over the passes
+    constexpr int kMaxNT = sizeof(T) == 8 ? 4 : 8;
+    const int tiles = (nbf + 15) / 16, passes = (tiles + kMaxNT - 1) / kMaxNT, nt = (tiles + passes - 1) / passes;
     const dim3 grid((G + kRhoBM - 1) / kRhoBM);
     switch (nt) {
 #define QCG_RHO_NT(n)                                                                                   \
