@@ -31,8 +31,37 @@ int leb_index(int n) {
 
 }  // namespace
 
+// Host Becke partition (the device one, becke_partition_kernel, repeats these operations in the
+// same order).
+static int host_partition(int64_t np, const double* pts, const int* owner, int nat, const double* axyz,
+                          const double* dist, double* part) {
+#pragma omp parallel for schedule(static)
+    for (long p = 0; p < (long)np; ++p) {
+#pragma clang fp contract(off)
+        std::vector<double> gd(nat), P(nat, 1.0);
+        for (int i = 0; i < nat; ++i) {
+            const double dx = pts[3 * p] - axyz[3 * i], dy = pts[3 * p + 1] - axyz[3 * i + 1],
+                         dz = pts[3 * p + 2] - axyz[3 * i + 2];
+            gd[i] = std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+        for (int i = 0; i < nat; ++i)
+            for (int j = 0; j < i; ++j) {
+                const double rij = dist[(size_t)i * nat + j];
+                if (rij < 1e-15) continue;
+                double mu = (gd[i] - gd[j]) / rij;
+                for (int it = 0; it < 3; ++it) mu = mu * (3.0 - mu * mu) * 0.5;
+                P[i] *= 0.5 * (1.0 - mu);
+                P[j] *= 0.5 * (1.0 + mu);
+            }
+        double ps = 0.0;
+        for (int i = 0; i < nat; ++i) ps += P[i];
+        part[p] = std::fabs(ps) > 1e-30 ? P[owner[p]] / ps : 1.0 / nat;
+    }
+    return QCG_OK;
+}
+
 int build_becke_grid(const HostBasis& B, const GridConfig& cfg, std::vector<double>& xyz, std::vector<double>& w,
-                     std::vector<int>& atom_of) {
+                     std::vector<int>& atom_of, BeckePartitionFn part_fn, void* user) {
     xyz.clear();
     w.clear();
     atom_of.clear();
@@ -45,11 +74,13 @@ int build_becke_grid(const HostBasis& B, const GridConfig& cfg, std::vector<doub
                          dz = B.xyz[3 * i + 2] - B.xyz[3 * j + 2];
             dist[(size_t)i * nat + j] = dist[(size_t)j * nat + i] = std::sqrt(dx * dx + dy * dy + dz * dz);
         }
+    // all atoms' raw points (Mura-Knowles x Lebedev), then one partition over all of them
+    std::vector<double> pts, raw;
+    std::vector<int> owner;
     for (int a = 0; a < nat; ++a) {
         const int z = B.Z[a];
         const double rscale = (z == 3 || z == 4 || z == 11 || z == 12 || z == 19 || z == 20) ? 7.0 : 5.2;
         const int n = cfg.n_radial;
-        std::vector<double> pts, raw;
         for (int i = 0; i < n; ++i) {
             const double x = (i + 0.5) / (double)n;
             const double x3 = x * x * x;
@@ -76,42 +107,25 @@ int build_becke_grid(const HostBasis& B, const GridConfig& cfg, std::vector<doub
                 pts.push_back(B.xyz[3 * a + 1] + r * g[4 * k + 1]);
                 pts.push_back(B.xyz[3 * a + 2] + r * g[4 * k + 2]);
                 raw.push_back(rf * g[4 * k + 3]);
+                owner.push_back(a);
             }
         }
-        const size_t np = raw.size();
-        std::vector<double> part(np, 1.0);
-        if (nat > 1) {
-#pragma omp parallel for schedule(static)
-            for (long p = 0; p < (long)np; ++p) {
-                std::vector<double> gd(nat), P(nat, 1.0);
-                for (int i = 0; i < nat; ++i) {
-                    const double dx = pts[3 * p] - B.xyz[3 * i], dy = pts[3 * p + 1] - B.xyz[3 * i + 1],
-                                 dz = pts[3 * p + 2] - B.xyz[3 * i + 2];
-                    gd[i] = std::sqrt(dx * dx + dy * dy + dz * dz);
-                }
-                for (int i = 0; i < nat; ++i)
-                    for (int j = 0; j < i; ++j) {
-                        const double rij = dist[(size_t)i * nat + j];
-                        if (rij < 1e-15) continue;
-                        double mu = (gd[i] - gd[j]) / rij;
-                        for (int it = 0; it < 3; ++it) mu = mu * (3.0 - mu * mu) * 0.5;
-                        P[i] *= 0.5 * (1.0 - mu);
-                        P[j] *= 0.5 * (1.0 + mu);
-                    }
-                double ps = 0.0;
-                for (int i = 0; i < nat; ++i) ps += P[i];
-                part[p] = std::fabs(ps) > 1e-30 ? P[a] / ps : 1.0 / nat;
-            }
-        }
-        for (size_t p = 0; p < np; ++p) {
-            const double wf = raw[p] * part[p];
-            if (std::fabs(wf) > 1e-30) {
-                xyz.push_back(pts[3 * p]);
-                xyz.push_back(pts[3 * p + 1]);
-                xyz.push_back(pts[3 * p + 2]);
-                w.push_back(wf);
-                atom_of.push_back(a);
-            }
+    }
+    const int64_t np = (int64_t)raw.size();
+    std::vector<double> part(np, 1.0);
+    if (nat > 1) {
+        const int rc = part_fn ? part_fn(user, np, pts.data(), owner.data(), nat, B.xyz.data(), dist.data(), part.data())
+                               : host_partition(np, pts.data(), owner.data(), nat, B.xyz.data(), dist.data(), part.data());
+        if (rc) return rc;
+    }
+    for (int64_t p = 0; p < np; ++p) {
+        const double wf = raw[p] * part[p];
+        if (std::fabs(wf) > 1e-30) {
+            xyz.push_back(pts[3 * p]);
+            xyz.push_back(pts[3 * p + 1]);
+            xyz.push_back(pts[3 * p + 2]);
+            w.push_back(wf);
+            atom_of.push_back(owner[p]);
         }
     }
     return QCG_OK;

@@ -250,6 +250,36 @@ int qcg_ks_gradient(qcg_context* c, const double* D, const double* C, const doub
     return QCG_OK;
 }
 
+namespace {
+// build_becke_grid's partition step on the device of the context
+int device_partition(void* user, int64_t np, const double* pts, const int* owner, int nat, const double* axyz,
+                     const double* dist, double* part) {
+    qcg_context* c = static_cast<qcg_context*>(user);
+    double *dp = nullptr, *da = nullptr, *dd = nullptr, *dw = nullptr;
+    int* dow = nullptr;
+    int rc = QCG_OK;
+    auto chk = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && rc == QCG_OK) rc = fail(c, QCG_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    chk(c->alloc(&dp, 3 * (size_t)np), "hipMalloc");
+    chk(c->alloc(&dow, (size_t)np), "hipMalloc");
+    chk(c->alloc(&da, 3 * (size_t)nat), "hipMalloc");
+    chk(c->alloc(&dd, (size_t)nat * nat), "hipMalloc");
+    chk(c->alloc(&dw, (size_t)np), "hipMalloc");
+    if (rc == QCG_OK) {
+        chk(hipMemcpyAsync(dp, pts, 3 * (size_t)np * sizeof(double), hipMemcpyHostToDevice, c->stream), "upload");
+        chk(hipMemcpyAsync(dow, owner, (size_t)np * sizeof(int), hipMemcpyHostToDevice, c->stream), "upload");
+        chk(hipMemcpyAsync(da, axyz, 3 * (size_t)nat * sizeof(double), hipMemcpyHostToDevice, c->stream), "upload");
+        chk(hipMemcpyAsync(dd, dist, (size_t)nat * nat * sizeof(double), hipMemcpyHostToDevice, c->stream), "upload");
+        chk(launch_becke_partition(np, dp, dow, nat, da, dd, dw, c->stream), "becke_partition");
+        chk(hipMemcpyAsync(part, dw, (size_t)np * sizeof(double), hipMemcpyDeviceToHost, c->stream), "download");
+        chk(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    }
+    for (void* q : {(void*)dp, (void*)dow, (void*)da, (void*)dd, (void*)dw}) c->release(q);
+    return rc;
+}
+}  // namespace
+
 int qcg_build_grid(qcg_context* c, int n_radial, int quality, int pruning, int64_t* npts) {
     if (!c) return QCG_INVALID_ARG;
     if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
@@ -259,8 +289,8 @@ int qcg_build_grid(qcg_context* c, int n_radial, int quality, int pruning, int64
     cfg.n_radial = n_radial;
     cfg.quality = quality;
     cfg.pruning = pruning ? 1 : 0;
-    int rc = build_becke_grid(c->basis, cfg, c->grid_xyz, c->grid_w, c->grid_atom);
-    if (rc) return fail(c, rc, "grid construction failed");
+    int rc = build_becke_grid(c->basis, cfg, c->grid_xyz, c->grid_w, c->grid_atom, device_partition, c);
+    if (rc) return rc == QCG_DEVICE ? rc : fail(c, rc, "grid construction failed");
     rc = upload_grid(c);
     if (rc) return rc;
     if (npts) *npts = c->G;

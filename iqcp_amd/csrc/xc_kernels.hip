@@ -272,6 +272,52 @@ hipError_t launch_xc_grad_accumulate(int G, int nshell, const GridShell* shells,
     return hipGetLastError();
 }
 
+// Becke partition weight of one grid point per thread (becke_partition, dft/grid.rs:400-468): the
+// same operations in the same order as the host loop in grid.cpp, with contraction into FMA off on
+// both sides, so the kept-point decision |w| > 1e-30 matches.  Each thread keeps its nuclear
+// distances and cell products in LDS (2 nat doubles).
+__global__ __launch_bounds__(64) void becke_partition_kernel(int64_t np, const double* __restrict__ pts,
+                                                             const int* __restrict__ owner, int nat,
+                                                             const double* __restrict__ axyz,
+                                                             const double* __restrict__ dist, double* __restrict__ part) {
+#pragma clang fp contract(off)
+    extern __shared__ double sh[];
+    double* gd = sh + (size_t)threadIdx.x * 2 * nat;
+    double* P = gd + nat;
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= np) return;  // no barrier below
+    for (int i = 0; i < nat; ++i) {
+        const double dx = pts[3 * p] - axyz[3 * i], dy = pts[3 * p + 1] - axyz[3 * i + 1],
+                     dz = pts[3 * p + 2] - axyz[3 * i + 2];
+        gd[i] = sqrt(dx * dx + dy * dy + dz * dz);
+        P[i] = 1.0;
+    }
+    for (int i = 0; i < nat; ++i)
+        for (int j = 0; j < i; ++j) {
+            const double rij = dist[(size_t)i * nat + j];
+            if (rij < 1e-15) continue;
+            double mu = (gd[i] - gd[j]) / rij;
+            for (int it = 0; it < 3; ++it) mu = mu * (3.0 - mu * mu) * 0.5;
+            P[i] *= 0.5 * (1.0 - mu);
+            P[j] *= 0.5 * (1.0 + mu);
+        }
+    double ps = 0.0;
+    for (int i = 0; i < nat; ++i) ps += P[i];
+    part[p] = fabs(ps) > 1e-30 ? P[owner[p]] / ps : 1.0 / nat;
+}
+
+hipError_t launch_becke_partition(int64_t np, const double* pts, const int* owner, int nat, const double* axyz,
+                                  const double* dist, double* part, hipStream_t st) {
+    if (np <= 0) return hipSuccess;
+    if (nat <= 0 || nat > 4096) return hipErrorInvalidValue;
+    int tpb = 64;  // threads per block: 2 nat doubles each, at most 64 KB of LDS per block
+    while (tpb > 1 && (size_t)tpb * 2 * nat * sizeof(double) > 65536) tpb >>= 1;
+    const size_t lds = (size_t)tpb * 2 * nat * sizeof(double);
+    hipLaunchKernelGGL(becke_partition_kernel, dim3((unsigned)((np + tpb - 1) / tpb)), dim3(tpb), lds, st, np, pts,
+                       owner, nat, axyz, dist, part);
+    return hipGetLastError();
+}
+
 hipError_t launch_basis_on_grid(int G, int nshell, const GridShell* shells, const double* exps, const double* cn,
                                 const double* pts, int nbf, double* phi, double* dphx, double* dphy, double* dphz,
                                 hipStream_t st) {

@@ -10,6 +10,9 @@ struct GridShell {
     int atom, pad;
 };
 
+// Becke partition weights of np points (owner[p]: the point's atom), on the device
+hipError_t launch_becke_partition(int64_t np, const double* pts, const int* owner, int nat, const double* axyz,
+                                  const double* dist, double* part, hipStream_t st);
 hipError_t launch_basis_on_grid(int G, int nshell, const GridShell* shells, const double* exps, const double* cn,
                                 const double* pts, int nbf, double* phi, double* dphx, double* dphy, double* dphz,
                                 hipStream_t st);

@@ -42,6 +42,23 @@ def test_gpu_grid_matches_oracle(mol, bas, pruning):
     np.testing.assert_allclose(w, wo, rtol=1e-12, atol=1e-300)
 
 
+@pytest.mark.parametrize("pruning", [0, 1])
+def test_gpu_grid_water_cluster_matches_oracle(pruning):
+    """The device Becke partition on a 2x2x1 water cluster (12 atoms, every pair of cells active):
+    the kept-point set is bit-exact and the weights agree with the oracle's."""
+    from iqcp_amd import workloads
+    from iqcp_amd.basis import build_basis
+    b = build_basis(workloads.water_cluster((2, 2, 1), 2.9, 3), "sto-3g", units="angstrom")
+    eng = Engine(b)
+    n = eng.build_grid(75, "standard", bool(pruning))
+    pts, w, at = eng.get_grid()
+    po, wo, ato = oracle.becke_grid(b.Z, b.xyz, 75, 0, pruning)
+    assert n == len(wo)
+    assert np.array_equal(at, ato)
+    np.testing.assert_allclose(pts, po, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(w, wo, rtol=1e-12, atol=1e-300)
+
+
 @pytest.mark.parametrize("mol,bas", [("H2O", "6-31g*"), ("CH4", "6-31g*"), ("NH3", "6-31+g*")])
 def test_gpu_vxc_matches_oracle(mol, bas):
     b = bench_molecule(mol, bas)
