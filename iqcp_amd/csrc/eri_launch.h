@@ -185,7 +185,7 @@ struct GradLaunch {
 // per item for the light classes (per-lane kernel), a few kets per wavefront otherwise.
 inline int grad_item_kets(int cb, int ck) {
     auto nc = [](int c) { return c == 0 ? 1 : c == 1 ? 3 : c == 2 ? 9 : c == 3 ? 6 : c == 4 ? 18 : 36; };
-    return nc(cb) * nc(ck) <= 54 ? 128 : 4;
+    return nc(cb) * nc(ck) <= 54 ? 128 : 8;
 }
 
 hipError_t launch_grad_cb0(int ck, const GradLaunch& a, hipStream_t st);
