@@ -9,6 +9,7 @@
 #include "../../include/qcg.h"
 #include "eri_launch.h"
 #include "onee.h"
+#include "onee_launch.h"
 #include "xc_launch.h"
 
 using namespace qcg;
@@ -88,6 +89,13 @@ struct qcg_context {
     int grad_off[21] = {}, grad_cnt[21] = {};
     double grad_key[2] = {-1.0, -1.0};
     double* d_grad = nullptr;
+    double* d_W = nullptr;  // energy-weighted density of the gradient (nbf x nbf)
+    // device basis copy and work list of the one-electron gradient kernel
+    int* d_onee_int = nullptr;
+    double* d_onee_dbl = nullptr;
+    int4* d_onee_work = nullptr;
+    int onee_nwork = 0;
+    BasisView onee_view{};
     std::vector<int4> items;  // J/K work items over the ket segments segs (cut_items)
     std::vector<int2> segs;
     int item_off[21][2] = {}, item_cnt[21][2] = {};
@@ -147,7 +155,8 @@ struct qcg_context {
         for (void** p : {(void**)&d_pairs, (void**)&d_pp, (void**)&d_ppf, (void**)&d_D, (void**)&d_J, (void**)&d_K,
                          (void**)&d_Jacc, (void**)&d_Kacc, (void**)&d_H, (void**)&d_F, (void**)&d_mol_nbf,
                          (void**)&d_mat_off, (void**)&d_ket_off, (void**)&d_ket_cnt, (void**)&d_items,
-                         (void**)&d_segs, (void**)&d_grad_items, (void**)&d_grad_segs, (void**)&d_grad}) {
+                         (void**)&d_segs, (void**)&d_grad_items, (void**)&d_grad_segs, (void**)&d_grad,
+                         (void**)&d_W, (void**)&d_onee_int, (void**)&d_onee_dbl, (void**)&d_onee_work}) {
             release(*p);
             *p = nullptr;
         }
@@ -171,9 +180,14 @@ int run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStream_t
 // streams, fork/join events), done ahead so that run_jk itself can be captured into a graph.
 int prepare_jk(qcg_context* c);
 // ERI gradient at the device density dD, ADDED into dgrad on st; returns after the launches.
-int eri_grad_launch(qcg_context* c, const double* dD, double hf_frac, double* dgrad, hipStream_t st);
-// eri_grad_launch for the host density D into the context's zeroed d_grad, on c->stream
-int eri_grad_start(qcg_context* c, const double* D, double hf_frac);
+// dW != nullptr: also the one-electron terms sum D dH/dR - W dS/dR (onee_grad_kernel).
+int eri_grad_launch(qcg_context* c, const double* dD, double hf_frac, double* dgrad, hipStream_t st,
+                    const double* dW = nullptr);
+// W = 2 sum_i eps_i C_mu,i C_nu,i over the n_occ occupied orbitals (C row-major nbf x nbf)
+std::vector<double> energy_weighted_density(int nbf, int n_occ, const double* C, const double* eps);
+// eri_grad_launch for the host density D (and energy-weighted density W, optional) into the
+// context's zeroed d_grad, on c->stream
+int eri_grad_start(qcg_context* c, const double* D, double hf_frac, const double* W = nullptr);
 // waits on c->stream and copies d_grad to out
 int grad_finish(qcg_context* c, double* out);
 double radial_norm(double a, int l);

@@ -50,11 +50,49 @@ int grad_plan(qcg_context* c) {
 // Launches the ERI gradient at the device density dD on st, ADDING into dgrad (natom x 3); the
 // class kernels run on the auxiliary streams, forked from and joined back to st.  Returns without
 // synchronising: the caller may do host work before it waits on st.
-int qcg::eri_grad_launch(qcg_context* c, const double* dD, double hf_frac, double* dgrad, hipStream_t st) {
+// Device copy of the basis and the (ordered shell pair, primitive pair) work list of the
+// one-electron gradient kernel, made once per basis.
+static int onee_setup(qcg_context* c) {
+    if (c->d_onee_work) return QCG_OK;
+    const HostBasis& B = c->basis;
+    std::vector<int4> work;
+    for (int si = 0; si < B.nshell; ++si)
+        for (int sj = 0; sj < B.nshell; ++sj)
+            for (int pi = 0; pi < B.nprim[si]; ++pi)
+                for (int pj = 0; pj < B.nprim[sj]; ++pj) work.push_back(make_int4(si, sj, pi, pj));
+    // ints: l, nprim, prim_off, shell_atom, bf_off (nshell each), Z (natom); doubles: exps, coefs
+    // (nprim total each), centers (3 nshell), xyz (3 natom)
+    std::vector<int> iv;
+    for (const std::vector<int>* v : {&B.l, &B.nprim, &B.prim_off, &B.shell_atom, &B.bf_off, &B.Z})
+        iv.insert(iv.end(), v->begin(), v->end());
+    std::vector<double> dv;
+    for (const std::vector<double>* v : {&B.exps, &B.coefs, &B.centers, &B.xyz}) dv.insert(dv.end(), v->begin(), v->end());
+    HIPCHK(c, c->alloc(&c->d_onee_int, iv.size()));
+    HIPCHK(c, c->alloc(&c->d_onee_dbl, dv.size()));
+    HIPCHK(c, c->alloc(&c->d_onee_work, work.size()));
+    HIPCHK(c, hipMemcpy(c->d_onee_int, iv.data(), iv.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_onee_dbl, dv.data(), dv.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_onee_work, work.data(), work.size() * sizeof(int4), hipMemcpyHostToDevice));
+    c->onee_nwork = (int)work.size();
+    const int ns = B.nshell, np = (int)B.exps.size();
+    const int* ip = c->d_onee_int;
+    const double* dp = c->d_onee_dbl;
+    c->onee_view = BasisView{ns,          B.nbf,           B.natom,         ip,          ip + ns,
+                             ip + 2 * ns, ip + 3 * ns,     ip + 4 * ns,     ip + 5 * ns, dp,
+                             dp + np,     dp + 2 * np,     dp + 2 * np + 3 * ns};
+    return QCG_OK;
+}
+
+int qcg::eri_grad_launch(qcg_context* c, const double* dD, double hf_frac, double* dgrad, hipStream_t st,
+                         const double* dW) {
     int rc = grad_plan(c);
     if (rc) return rc;
     rc = prepare_jk(c);  // the auxiliary streams and events
     if (rc) return rc;
+    if (dW) {
+        rc = onee_setup(c);
+        if (rc) return rc;
+    }
     HIPCHK(c, hipEventRecord(c->ev_fork, st));
     for (int i = 0; i < qcg_context::kStreams; ++i) HIPCHK(c, hipStreamWaitEvent(c->aux[i], c->ev_fork, 0));
     // heaviest classes first: the d-bra classes with many components, by component count
@@ -91,6 +129,13 @@ int qcg::eri_grad_launch(qcg_context* c, const double* dD, double hf_frac, doubl
         const hipError_t e = launch_grad(cb, ck, a, c->aux[k % qcg_context::kStreams]);
         if (e != hipSuccess) return fail(c, QCG_DEVICE, std::string("launch_grad: ") + hipGetErrorString(e));
     }
+    // the one-electron terms last, on the stream whose round-robin share of the (sorted) ERI
+    // classes is the lightest
+    if (dW) {
+        const hipError_t e = launch_onee_grad(c->onee_view, c->d_onee_work, c->onee_nwork, dD, dW, c->d_rys, dgrad,
+                                              c->aux[qcg_context::kStreams - 1]);
+        if (e != hipSuccess) return fail(c, QCG_DEVICE, std::string("launch_onee_grad: ") + hipGetErrorString(e));
+    }
     for (int i = 0; i < qcg_context::kStreams; ++i) {
         HIPCHK(c, hipEventRecord(c->ev_join[i], c->aux[i]));
         HIPCHK(c, hipStreamWaitEvent(st, c->ev_join[i], 0));
@@ -98,29 +143,37 @@ int qcg::eri_grad_launch(qcg_context* c, const double* dD, double hf_frac, doubl
     return QCG_OK;
 }
 
-namespace {
-
-// W = 2 sum_i eps_i C_mu,i C_nu,i  (build_energy_weighted_density, scf/gradient.rs:101)
-std::vector<double> energy_weighted_density(int nbf, int n_occ, const double* C, const double* eps) {
+// W = 2 sum_i eps_i C_mu,i C_nu,i  (build_energy_weighted_density, scf/gradient.rs:101); rows over
+// the host threads, the upper triangle mirrored.
+std::vector<double> qcg::energy_weighted_density(int nbf, int n_occ, const double* C, const double* eps) {
     std::vector<double> W((size_t)nbf * nbf, 0.0);
-    for (int mu = 0; mu < nbf; ++mu)
-        for (int nu = 0; nu < nbf; ++nu) {
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int mu = 0; mu < nbf; ++mu) {
+        std::vector<double> ce(n_occ);
+        for (int i = 0; i < n_occ; ++i) ce[i] = eps[i] * C[(size_t)mu * nbf + i];
+        for (int nu = mu; nu < nbf; ++nu) {
+            const double* cn = C + (size_t)nu * nbf;
             double v = 0.0;
-            for (int i = 0; i < n_occ; ++i) v += eps[i] * C[(size_t)mu * nbf + i] * C[(size_t)nu * nbf + i];
+            for (int i = 0; i < n_occ; ++i) v += ce[i] * cn[i];
             W[(size_t)mu * nbf + nu] = 2.0 * v;
         }
+    }
+    for (int mu = 0; mu < nbf; ++mu)
+        for (int nu = 0; nu < mu; ++nu) W[(size_t)mu * nbf + nu] = W[(size_t)nu * nbf + mu];
     return W;
 }
 
-}  // namespace
-
-// Starts the ERI gradient of the host density D on c->stream into c->d_grad (zeroed first).
-int qcg::eri_grad_start(qcg_context* c, const double* D, double hf_frac) {
+// Starts the ERI gradient of the host density D on c->stream into c->d_grad (zeroed first); with
+// W (energy-weighted density, host) also the one-electron terms.
+int qcg::eri_grad_start(qcg_context* c, const double* D, double hf_frac, const double* W) {
     const int nbf = c->basis.nbf, n3 = 3 * c->basis.natom;
+    const size_t n2 = (size_t)nbf * nbf;
     if (!c->d_grad) HIPCHK(c, c->alloc(&c->d_grad, (size_t)n3));
-    HIPCHK(c, hipMemcpyAsync(c->d_D, D, (size_t)nbf * nbf * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (W && !c->d_W) HIPCHK(c, c->alloc(&c->d_W, n2));
+    HIPCHK(c, hipMemcpyAsync(c->d_D, D, n2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (W) HIPCHK(c, hipMemcpyAsync(c->d_W, W, n2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, launch_fill(c->d_grad, n3, 0.0, c->stream));
-    return eri_grad_launch(c, c->d_D, hf_frac, c->d_grad, c->stream);
+    return eri_grad_launch(c, c->d_D, hf_frac, c->d_grad, c->stream, W ? c->d_W : nullptr);
 }
 
 // Waits for the device gradient and copies it to out (natom x 3).
@@ -160,14 +213,13 @@ int qcg_rhf_gradient(qcg_context* c, const double* D, const double* C, const dou
     if (n_occ < 0 || n_occ > nbf) return fail(c, QCG_INVALID_ARG, "n_occ");
     HIPCHK(c, hipSetDevice(c->device));
     const std::vector<double> W = energy_weighted_density(nbf, n_occ, C, eps);
-    // the ERI part runs on the device while the host does the one-electron part
-    int rc = eri_grad_start(c, D, 1.0);
+    // ERI and one-electron terms on the device, nuclear repulsion on the host meanwhile
+    int rc = eri_grad_start(c, D, 1.0, W.data());
     if (rc) return rc;
-    rc = qcg_one_electron_gradient(c, D, W.data(), grad);
-    std::vector<double> g2(n3);
-    const int rf = grad_finish(c, g2.data());
+    std::vector<double> g2(n3, 0.0);
+    nuclear_repulsion_gradient(c->basis, g2.data());
+    rc = grad_finish(c, grad);
     if (rc) return rc;
-    if (rf) return rf;
     for (int i = 0; i < n3; ++i) grad[i] += g2[i];
     return QCG_OK;
 }

@@ -176,14 +176,10 @@ void nuclear_repulsion_gradient(const HostBasis& B, double* grad) {
         }
 }
 
-// sum_ab D_ab dH_ab/dR - W_ab dS_ab/dR, H = T + V, over all ordered shell pairs (the role of
-// overlap_derivative_matrix :188 and hcore_derivative_matrix :355 contracted as in rhf_gradient :930-950).
-// Basis-function derivatives: d/dA_x phi_i = 2 alpha phi_(i+1) - i phi_(i-1); for V the operator
-// centre C gets -(dA + dB) by translational invariance of each three-centre integral.
+// sum_ab D_ab dH_ab/dR - W_ab dS_ab/dR, H = T + V, over all ordered shell pairs and their primitive
+// pairs (onee_grad_overlap_kinetic and onee_grad_nuclear, shared with the device kernel).
 void one_electron_gradient(const HostBasis& B, const double* D, const double* W, double* grad_out) {
-    const int nbf = B.nbf;
-    const double* rys_tab = qcg_rys_coef;
-    constexpr int LD = kMaxL + 4;
+    const BasisView V = basis_view(B);
     const int n3 = 3 * B.natom;
     // shell pairs spread over the host threads; each thread sums into its own gradient copy
 #pragma omp parallel
@@ -192,133 +188,13 @@ void one_electron_gradient(const HostBasis& B, const double* D, const double* W,
     double* grad = gloc.data();
 #pragma omp for schedule(dynamic, 1) collapse(2)
     for (int si = 0; si < B.nshell; ++si)
-        for (int sj = 0; sj < B.nshell; ++sj) {
-            const int la = B.l[si], lb = B.l[sj];
-            const int na = ncart(la), nb = ncart(lb);
-            const int atA = B.shell_atom[si], atB = B.shell_atom[sj];
-            const double* A = &B.centers[3 * si];
-            const double* Bc = &B.centers[3 * sj];
-            const double AB[3] = {A[0] - Bc[0], A[1] - Bc[1], A[2] - Bc[2]};
-            const double ab2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
+        for (int sj = 0; sj < B.nshell; ++sj)
             for (int pi = 0; pi < B.nprim[si]; ++pi)
                 for (int pj = 0; pj < B.nprim[sj]; ++pj) {
-                    const double a = B.exps[B.prim_off[si] + pi], b = B.exps[B.prim_off[sj] + pj];
-                    const double ca = B.coefs[B.prim_off[si] + pi] * prim_norm_radial(a, la);
-                    const double cb = B.coefs[B.prim_off[sj] + pj] * prim_norm_radial(b, lb);
-                    const double p = a + b, mu = a * b / p, oo2p = 0.5 / p;
-                    const double kab = std::exp(-mu * ab2);
-                    double P[3], PA[3], PB[3];
-                    for (int x = 0; x < 3; ++x) {
-                        P[x] = (a * A[x] + b * Bc[x]) / p;
-                        PA[x] = P[x] - A[x];
-                        PB[x] = P[x] - Bc[x];
-                    }
-                    double s1[3][LD * LD];
-                    for (int x = 0; x < 3; ++x) os_overlap_1d(PA[x], PB[x], oo2p, la + 2, lb + 2, s1[x], LD);
-                    auto sv = [&](int x, int i, int j) { return (i < 0 || j < 0) ? 0.0 : s1[x][i * LD + j]; };
-                    auto tv = [&](int x, int i, int j) {
-                        if (i < 0 || j < 0) return 0.0;
-                        double v = 4.0 * a * b * sv(x, i + 1, j + 1);
-                        if (i > 0) v -= 2.0 * b * i * sv(x, i - 1, j + 1);
-                        if (j > 0) v -= 2.0 * a * j * sv(x, i + 1, j - 1);
-                        if (i > 0 && j > 0) v += (double)(i * j) * sv(x, i - 1, j - 1);
-                        return 0.5 * v;
-                    };
-                    const double pre = kab * std::pow(kPi / p, 1.5) * ca * cb;
-                    // component data of the pair: powers, norm, D and W entries
-                    int ipa[6][3], jpb[6][3];
-                    double nrm[6][6], dv[6][6];
-                    bool any_d = false;
-                    for (int ia = 0; ia < na; ++ia) {
-                        ipa[ia][0] = cart_x(la, ia); ipa[ia][1] = cart_y(la, ia); ipa[ia][2] = cart_z(la, ia);
-                    }
-                    for (int jb = 0; jb < nb; ++jb) {
-                        jpb[jb][0] = cart_x(lb, jb); jpb[jb][1] = cart_y(lb, jb); jpb[jb][2] = cart_z(lb, jb);
-                    }
-                    for (int ia = 0; ia < na; ++ia)
-                        for (int jb = 0; jb < nb; ++jb) {
-                            const int* ip = ipa[ia];
-                            const int* jp = jpb[jb];
-                            nrm[ia][jb] = comp_norm_host(la, ia) * comp_norm_host(lb, jb);
-                            const size_t idx = (size_t)(B.bf_off[si] + ia) * nbf + B.bf_off[sj] + jb;
-                            dv[ia][jb] = D[idx];
-                            const double wv = W[idx];
-                            any_d = any_d || dv[ia][jb] != 0.0;
-                            if (dv[ia][jb] == 0.0 && wv == 0.0) continue;
-                            for (int x = 0; x < 3; ++x) {
-                                const int y = (x + 1) % 3, z = (x + 2) % 3;
-                                const double sy = sv(y, ip[y], jp[y]), sz = sv(z, ip[z], jp[z]);
-                                const double ty = tv(y, ip[y], jp[y]), tz = tv(z, ip[z], jp[z]);
-                                // derivative with respect to A_x and B_x of S and T
-                                const double dsA = 2.0 * a * sv(x, ip[x] + 1, jp[x]) - ip[x] * sv(x, ip[x] - 1, jp[x]);
-                                const double dsB = 2.0 * b * sv(x, ip[x], jp[x] + 1) - jp[x] * sv(x, ip[x], jp[x] - 1);
-                                const double dtA = 2.0 * a * tv(x, ip[x] + 1, jp[x]) - ip[x] * tv(x, ip[x] - 1, jp[x]);
-                                const double dtB = 2.0 * b * tv(x, ip[x], jp[x] + 1) - jp[x] * tv(x, ip[x], jp[x] - 1);
-                                const double SA = dsA * sy * sz, SB = dsB * sy * sz;
-                                const double TA = dtA * sy * sz + dsA * (ty * sz + sy * tz);
-                                const double TB = dtB * sy * sz + dsB * (ty * sz + sy * tz);
-                                grad[3 * atA + x] += pre * nrm[ia][jb] * (dv[ia][jb] * TA - wv * SA);
-                                grad[3 * atB + x] += pre * nrm[ia][jb] * (dv[ia][jb] * TB - wv * SB);
-                            }
-                        }
-                    if (!any_d) continue;
-                    // nuclear attraction derivative: per nucleus one Rys rule and one set of 1-D
-                    // tables (with one extra power on each side), shared by all component pairs
-                    const int nr = (la + lb + 1) / 2 + 1;
-                    constexpr int LV = kMaxL + 2;  // table dimension for i, j <= l + 1
-                    const int nmax = la + lb + 2;
-                    for (int cnuc = 0; cnuc < B.natom; ++cnuc) {
-                        const double* C = &B.xyz[3 * cnuc];
-                        const double PC[3] = {P[0] - C[0], P[1] - C[1], P[2] - C[2]};
-                        const double Tn = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
-                        double rx[QCG_RYS_NMAX], rw[QCG_RYS_NMAX];
-                        rys_eval_n<double>(nr, Tn, rys_tab, rx, rw);
-                        double g[QCG_RYS_NMAX][3][LV * LV];
-                        for (int r = 0; r < nr; ++r) {
-                            const double u = rx[r];
-                            for (int x = 0; x < 3; ++x) {
-                                double h[2 * kMaxL + 3];
-                                const double c0 = PA[x] - u * PC[x];
-                                const double bb = (1.0 - u) * oo2p;
-                                h[0] = 1.0;
-                                if (nmax > 0) h[1] = c0;
-                                for (int n = 1; n < nmax; ++n) h[n + 1] = c0 * h[n] + n * bb * h[n - 1];
-                                double t2[(2 * kMaxL + 3) * LV];
-                                for (int n = 0; n <= nmax; ++n) t2[n * LV] = h[n];
-                                for (int j = 1; j <= lb + 1; ++j)
-                                    for (int n = 0; n + j <= nmax; ++n)
-                                        t2[n * LV + j] = t2[(n + 1) * LV + j - 1] + AB[x] * t2[n * LV + j - 1];
-                                for (int i = 0; i <= la + 1; ++i)
-                                    for (int j = 0; j <= lb + 1; ++j)
-                                        g[r][x][i * LV + j] = (i + j <= nmax) ? t2[i * LV + j] : 0.0;
-                            }
-                        }
-                        const double pz = -(double)B.Z[cnuc] * 2.0 * kPi / p * kab * ca * cb;
-                        double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0};
-                        for (int ia = 0; ia < na; ++ia)
-                            for (int jb = 0; jb < nb; ++jb) {
-                                if (dv[ia][jb] == 0.0) continue;
-                                const int* ip = ipa[ia];
-                                const int* jp = jpb[jb];
-                                const double pv = pz * nrm[ia][jb] * dv[ia][jb];
-                                for (int r = 0; r < nr; ++r) {
-                                    auto gv = [&](int x, int i, int j) { return (i < 0 || j < 0) ? 0.0 : g[r][x][i * LV + j]; };
-                                    for (int x = 0; x < 3; ++x) {
-                                        const int y = (x + 1) % 3, z = (x + 2) % 3;
-                                        const double oth = rw[r] * pv * gv(y, ip[y], jp[y]) * gv(z, ip[z], jp[z]);
-                                        gA[x] += oth * (2.0 * a * gv(x, ip[x] + 1, jp[x]) - ip[x] * gv(x, ip[x] - 1, jp[x]));
-                                        gB[x] += oth * (2.0 * b * gv(x, ip[x], jp[x] + 1) - jp[x] * gv(x, ip[x], jp[x] - 1));
-                                    }
-                                }
-                            }
-                        for (int x = 0; x < 3; ++x) {
-                            grad[3 * atA + x] += gA[x];
-                            grad[3 * atB + x] += gB[x];
-                            grad[3 * cnuc + x] -= gA[x] + gB[x];
-                        }
-                    }
+                    auto add = [&](int atom, int x, double v) { grad[3 * atom + x] += v; };
+                    onee_grad_overlap_kinetic(V, si, sj, pi, pj, D, W, add);
+                    for (int cn = 0; cn < B.natom; ++cn) onee_grad_nuclear(V, si, sj, pi, pj, cn, D, qcg_rys_coef, add);
                 }
-        }
 #pragma omp critical
     for (int t = 0; t < n3; ++t) grad_out[t] += gloc[t];
     }

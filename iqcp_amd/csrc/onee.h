@@ -2,6 +2,7 @@
 #pragma once
 #include <vector>
 
+#include "onee_pair.h"
 #include "qcg_common.h"
 
 namespace qcg {
@@ -12,6 +13,12 @@ struct HostBasis {
     std::vector<int> l, nprim, prim_off, shell_atom, bf_off, Z;
     std::vector<double> exps, coefs, centers, xyz;
 };
+
+inline BasisView basis_view(const HostBasis& B) {
+    return BasisView{B.nshell,           B.nbf,          B.natom,         B.l.data(),       B.nprim.data(),
+                     B.prim_off.data(),  B.shell_atom.data(), B.bf_off.data(), B.Z.data(),  B.exps.data(),
+                     B.coefs.data(),     B.centers.data(), B.xyz.data()};
+}
 
 // S, T, V (any may be null), full nbf x nbf row-major.
 void one_electron_matrices(const HostBasis& B, double* S, double* T, double* V);

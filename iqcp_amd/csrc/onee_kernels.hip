@@ -1,0 +1,41 @@
+// One-electron gradient terms on the device: one thread per (ordered shell pair, primitive pair)
+// and term (blockIdx.y = 0: overlap and kinetic; blockIdx.y = 1 + c: attraction to nucleus c),
+// each running the same per-pair code as the host loop, per-atom sums in LDS.
+#include "onee_pair.h"
+#include "onee_launch.h"
+
+namespace qcg {
+
+__global__ __launch_bounds__(128) void onee_grad_kernel(BasisView B, const int4* __restrict__ work, int nwork,
+                                                       const double* __restrict__ D, const double* __restrict__ W,
+                                                       const double* __restrict__ rys_tab, double* __restrict__ grad) {
+    extern __shared__ double gl[];  // natom x 3
+    const int n3 = 3 * B.natom;
+    for (int i = threadIdx.x; i < n3; i += blockDim.x) gl[i] = 0.0;
+    __syncthreads();
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < nwork) {
+        const int4 w = work[u];
+        auto add = [&](int atom, int x, double v) {
+            if (v != 0.0) __hip_atomic_fetch_add(&gl[3 * atom + x], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        };
+        if (blockIdx.y == 0) onee_grad_overlap_kinetic(B, w.x, w.y, w.z, w.w, D, W, add);
+        else onee_grad_nuclear(B, w.x, w.y, w.z, w.w, (int)blockIdx.y - 1, D, rys_tab, add);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n3; i += blockDim.x)
+        if (gl[i] != 0.0) atomicAdd(&grad[i], gl[i]);
+}
+
+hipError_t launch_onee_grad(const BasisView& B, const int4* work, int nwork, const double* D, const double* W,
+                            const double* rys_tab, double* grad, hipStream_t st) {
+    if (nwork <= 0) return hipSuccess;
+    const size_t lds = 3 * (size_t)B.natom * sizeof(double);
+    if (lds > 65536) return hipErrorInvalidValue;
+    if (B.natom + 1 > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(onee_grad_kernel, dim3((nwork + 127) / 128, B.natom + 1), dim3(128), lds, st, B, work, nwork, D,
+                       W, rys_tab, grad);
+    return hipGetLastError();
+}
+
+}  // namespace qcg

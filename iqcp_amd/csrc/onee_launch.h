@@ -1,0 +1,12 @@
+// Host-side launch interface of the one-electron gradient kernel.
+#pragma once
+#include "onee_pair.h"
+
+namespace qcg {
+
+// work[u] = (si, sj, pi, pj): one ordered shell pair and one of its primitive pairs per entry; the
+// terms are ADDED into grad (natom x 3, device).  D, W: nbf x nbf on the device.
+hipError_t launch_onee_grad(const BasisView& B, const int4* work, int nwork, const double* D, const double* W,
+                            const double* rys_tab, double* grad, hipStream_t st);
+
+}  // namespace qcg

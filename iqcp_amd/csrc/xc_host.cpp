@@ -228,22 +228,15 @@ int qcg_ks_gradient(qcg_context* c, const double* D, const double* C, const doub
     if (n_occ < 0 || n_occ > nbf) return fail(c, QCG_INVALID_ARG, "n_occ");
     if (c->nmol != 1) return fail(c, QCG_INVALID_ARG, "gradient: one molecule per context");
     HIPCHK(c, hipSetDevice(c->device));
-    // W = 2 sum_i eps_i C_mu,i C_nu,i  (build_energy_weighted_density, scf/gradient.rs:101)
-    std::vector<double> W((size_t)nbf * nbf, 0.0);
-    for (int mu = 0; mu < nbf; ++mu)
-        for (int nu = 0; nu < nbf; ++nu) {
-            double v = 0.0;
-            for (int i = 0; i < n_occ; ++i) v += eps[i] * C[(size_t)mu * nbf + i] * C[(size_t)nu * nbf + i];
-            W[(size_t)mu * nbf + nu] = 2.0 * v;
-        }
-    // ERI part (B3lyp::A = 0.20 exact exchange) on the device while the host does the one-electron
-    // part, then the XC part into the same device gradient
-    int rc = eri_grad_start(c, D, 0.20);
+    const std::vector<double> W = energy_weighted_density(nbf, n_occ, C, eps);
+    // ERI part (B3lyp::A = 0.20 exact exchange) and one-electron part on the device, then the XC
+    // part into the same device gradient; nuclear repulsion on the host
+    int rc = eri_grad_start(c, D, 0.20, W.data());
     if (rc) return rc;
-    rc = qcg_one_electron_gradient(c, D, W.data(), grad);
-    if (!rc) rc = xc_gradient(c, c->d_D, c->d_grad, c->stream);
-    std::vector<double> g2(n3);
-    const int rf = grad_finish(c, g2.data());
+    rc = xc_gradient(c, c->d_D, c->d_grad, c->stream);
+    std::vector<double> g2(n3, 0.0);
+    nuclear_repulsion_gradient(c->basis, g2.data());
+    const int rf = grad_finish(c, grad);
     if (rc) return rc;
     if (rf) return rf;
     for (int i = 0; i < n3; ++i) grad[i] += g2[i];

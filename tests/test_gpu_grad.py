@@ -66,6 +66,26 @@ def test_one_electron_gradient_vs_fd():
             assert abs(g[a, x] - fd) < 2e-7, (a, x, g[a, x], fd)
 
 
+@pytest.mark.parametrize("mol,bas", [("H2O", "6-31+g*"), ("CH4", "6-31g*"), ("NH3", "sto-3g")])
+def test_device_one_electron_gradient_vs_host(mol, bas):
+    """rhf_gradient runs the one-electron terms on the device (onee_grad_kernel) beside the ERI
+    terms; the host loop (one_electron_gradient) runs the same per-pair code: the sum of the host
+    one-electron and the device ERI gradients must agree to summation order."""
+    b = build_basis(_distorted(mol, bas, seed=7), bas)
+    rng = np.random.default_rng(11)
+    C = rng.normal(0, 0.3, (b.nbf, b.nbf))
+    eps = np.sort(rng.normal(-0.5, 0.4, b.nbf))
+    no = b.n_occ
+    D = 2.0 * C[:, :no] @ C[:, :no].T
+    W = 2.0 * (C[:, :no] * eps[:no]) @ C[:, :no].T
+    eng = Engine(b)
+    g = eng.rhf_gradient(D, C, eps, no)
+    ref = eng.one_electron_gradient(D, W) + eng.eri_gradient(D, 1.0)
+    np.testing.assert_allclose(g, ref, atol=1e-10 * max(1.0, np.abs(ref).max()), rtol=0)
+    g2 = eng.rhf_gradient(D, C, eps, no)  # the cached device basis is reused
+    np.testing.assert_allclose(g2, g, atol=1e-12 * max(1.0, np.abs(ref).max()), rtol=0)
+
+
 def test_rhf_gradient_vs_fd_of_scf_energy():
     """Full analytic RHF gradient against central differences of converged energies (h = 1e-4)."""
     bas = "6-31g*"
