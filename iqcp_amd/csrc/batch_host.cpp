@@ -34,20 +34,26 @@ int qcg_batch_rhf_scf(qcg_context* c, const qcg_scf_config* cfg, int use_graph, 
     HIPCHK(c, hipSetDevice(c->device));
     const int nmol = c->nmol;
     std::vector<ScfRun> runs(nmol);
-    std::vector<std::vector<double>> Hs(nmol);
+    std::vector<std::vector<double>> Ss(nmol), Hs(nmol);
+    // one-electron matrices on the device, molecule by molecule
+    for (int m = 0; m < nmol; ++m) {
+        const HostBasis& B = c->mols[m];
+        const size_t n2 = (size_t)B.nbf * B.nbf;
+        std::vector<double> T(n2), V(n2);
+        Ss[m].resize(n2);
+        Hs[m].resize(n2);
+        const int r1 = one_electron_device(c, B, Ss[m].data(), T.data(), V.data());
+        if (r1) return r1;
+        for (size_t i = 0; i < n2; ++i) Hs[m][i] = T[i] + V[i];
+    }
     int bad = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
     for (int m = 0; m < nmol; ++m) {
         const HostBasis& B = c->mols[m];
         int nelec = 0;
         for (int z : B.Z) nelec += z;
-        const size_t n2 = (size_t)B.nbf * B.nbf;
-        std::vector<double> S(n2), T(n2), V(n2);
-        one_electron_matrices(B, S.data(), T.data(), V.data());
-        Hs[m].resize(n2);
-        for (size_t i = 0; i < n2; ++i) Hs[m][i] = T[i] + V[i];
         if (nelec % 2) { bad |= 1; continue; }
-        bad |= runs[m].init(B.nbf, nelec / 2, S.data(), Hs[m].data(), nuclear_repulsion(B), *cfg, nullptr) ? 1 : 0;
+        bad |= runs[m].init(B.nbf, nelec / 2, Ss[m].data(), Hs[m].data(), nuclear_repulsion(B), *cfg, nullptr) ? 1 : 0;
     }
     if (bad) return fail(c, QCG_INVALID_ARG, "batch: odd electron count or singular overlap");
     const size_t ntot = (size_t)c->mat_total;

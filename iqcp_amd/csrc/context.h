@@ -183,6 +183,10 @@ int prepare_jk(qcg_context* c);
 // dW != nullptr: also the one-electron terms sum D dH/dR - W dS/dR (onee_grad_kernel).
 int eri_grad_launch(qcg_context* c, const double* dD, double hf_frac, double* dgrad, hipStream_t st,
                     const double* dW = nullptr);
+// Device basis and work list of the one-electron gradient kernel (once per basis).
+int onee_setup(qcg_context* c);
+// S, T, V of the basis B (any output may be null), nbf x nbf row-major host arrays.
+int one_electron_device(qcg_context* c, const HostBasis& B, double* S, double* T, double* V);
 // W = 2 sum_i eps_i C_mu,i C_nu,i over the n_occ occupied orbitals (C row-major nbf x nbf)
 std::vector<double> energy_weighted_density(int nbf, int n_occ, const double* C, const double* eps);
 // eri_grad_launch for the host density D (and energy-weighted density W, optional) into the

@@ -1,5 +1,5 @@
-// Host side of the analytic nuclear gradient: the adjoint ERI kernels on the device plus the
-// one-electron and nuclear-repulsion terms on the host (rhf_gradient, scf/gradient.rs:900-976).
+// Host side of the analytic nuclear gradient: the adjoint ERI kernels and the one-electron terms on
+// the device, the nuclear-repulsion term on the host (rhf_gradient, scf/gradient.rs:900-976).
 #include <algorithm>
 #include <cstring>
 #include <string>
@@ -50,39 +50,6 @@ int grad_plan(qcg_context* c) {
 // Launches the ERI gradient at the device density dD on st, ADDING into dgrad (natom x 3); the
 // class kernels run on the auxiliary streams, forked from and joined back to st.  Returns without
 // synchronising: the caller may do host work before it waits on st.
-// Device copy of the basis and the (ordered shell pair, primitive pair) work list of the
-// one-electron gradient kernel, made once per basis.
-static int onee_setup(qcg_context* c) {
-    if (c->d_onee_work) return QCG_OK;
-    const HostBasis& B = c->basis;
-    std::vector<int4> work;
-    for (int si = 0; si < B.nshell; ++si)
-        for (int sj = 0; sj < B.nshell; ++sj)
-            for (int pi = 0; pi < B.nprim[si]; ++pi)
-                for (int pj = 0; pj < B.nprim[sj]; ++pj) work.push_back(make_int4(si, sj, pi, pj));
-    // ints: l, nprim, prim_off, shell_atom, bf_off (nshell each), Z (natom); doubles: exps, coefs
-    // (nprim total each), centers (3 nshell), xyz (3 natom)
-    std::vector<int> iv;
-    for (const std::vector<int>* v : {&B.l, &B.nprim, &B.prim_off, &B.shell_atom, &B.bf_off, &B.Z})
-        iv.insert(iv.end(), v->begin(), v->end());
-    std::vector<double> dv;
-    for (const std::vector<double>* v : {&B.exps, &B.coefs, &B.centers, &B.xyz}) dv.insert(dv.end(), v->begin(), v->end());
-    HIPCHK(c, c->alloc(&c->d_onee_int, iv.size()));
-    HIPCHK(c, c->alloc(&c->d_onee_dbl, dv.size()));
-    HIPCHK(c, c->alloc(&c->d_onee_work, work.size()));
-    HIPCHK(c, hipMemcpy(c->d_onee_int, iv.data(), iv.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_onee_dbl, dv.data(), dv.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_onee_work, work.data(), work.size() * sizeof(int4), hipMemcpyHostToDevice));
-    c->onee_nwork = (int)work.size();
-    const int ns = B.nshell, np = (int)B.exps.size();
-    const int* ip = c->d_onee_int;
-    const double* dp = c->d_onee_dbl;
-    c->onee_view = BasisView{ns,          B.nbf,           B.natom,         ip,          ip + ns,
-                             ip + 2 * ns, ip + 3 * ns,     ip + 4 * ns,     ip + 5 * ns, dp,
-                             dp + np,     dp + 2 * np,     dp + 2 * np + 3 * ns};
-    return QCG_OK;
-}
-
 int qcg::eri_grad_launch(qcg_context* c, const double* dD, double hf_frac, double* dgrad, hipStream_t st,
                          const double* dW) {
     int rc = grad_plan(c);
@@ -194,15 +161,6 @@ int qcg_eri_gradient(qcg_context* c, const double* D, double hf_frac, double* gr
     int rc = eri_grad_start(c, D, hf_frac);
     if (rc) return rc;
     return grad_finish(c, grad);
-}
-
-int qcg_one_electron_gradient(qcg_context* c, const double* D, const double* W, double* grad) {
-    if (!c || !D || !W || !grad) return QCG_INVALID_ARG;
-    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
-    std::memset(grad, 0, 3 * c->basis.natom * sizeof(double));
-    nuclear_repulsion_gradient(c->basis, grad);
-    one_electron_gradient(c->basis, D, W, grad);
-    return QCG_OK;
 }
 
 int qcg_rhf_gradient(qcg_context* c, const double* D, const double* C, const double* eps, int n_occ, double* grad) {

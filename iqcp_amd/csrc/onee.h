@@ -20,13 +20,6 @@ inline BasisView basis_view(const HostBasis& B) {
                      B.coefs.data(),     B.centers.data(), B.xyz.data()};
 }
 
-// S, T, V (any may be null), full nbf x nbf row-major.
-void one_electron_matrices(const HostBasis& B, double* S, double* T, double* V);
-
-// Nuclear derivatives of the one-electron terms contracted with densities, accumulated into
-// grad[3*natom]: sum_mn D_mn dH_mn/dR - sum_mn W_mn dS_mn/dR.
-void one_electron_gradient(const HostBasis& B, const double* D, const double* W, double* grad);
-
 double nuclear_repulsion(const HostBasis& B);
 void nuclear_repulsion_gradient(const HostBasis& B, double* grad);
 

@@ -1,6 +1,7 @@
-// One-electron gradient terms on the device: one thread per (ordered shell pair, primitive pair)
-// and term (blockIdx.y = 0: overlap and kinetic; blockIdx.y = 1 + c: attraction to nucleus c),
-// each running the same per-pair code as the host loop, per-atom sums in LDS.
+// One-electron integrals and gradient terms on the device.
+//
+// Gradient: one thread per (ordered shell pair, primitive pair) and term (blockIdx.y = 0: overlap
+// and kinetic; blockIdx.y = 1 + c: attraction to nucleus c), per-atom sums in LDS.
 #include "onee_pair.h"
 #include "onee_launch.h"
 
@@ -25,6 +26,35 @@ __global__ __launch_bounds__(128) void onee_grad_kernel(BasisView B, const int4*
     __syncthreads();
     for (int i = threadIdx.x; i < n3; i += blockDim.x)
         if (gl[i] != 0.0) atomicAdd(&grad[i], gl[i]);
+}
+
+// S, T, V: one thread per Cartesian component pair of the shell pairs si <= sj (work = {si, sj, ia,
+// jb}); a thread writes its element and, off the diagonal blocks, the mirrored one.
+__global__ __launch_bounds__(64) void onee_matrix_kernel(BasisView B, const int4* __restrict__ work, int nwork,
+                                                        const double* __restrict__ rys_tab, double* __restrict__ S,
+                                                        double* __restrict__ T, double* __restrict__ V) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nwork) return;
+    const int4 w = work[u];
+    double out[3];
+    onee_element(B, w.x, w.y, w.z, w.w, rys_tab, out);
+    const int nbf = B.nbf;
+    const size_t i = (size_t)B.bf_off[w.x] + w.z, j = (size_t)B.bf_off[w.y] + w.w;
+    S[i * nbf + j] = out[0];
+    T[i * nbf + j] = out[1];
+    V[i * nbf + j] = out[2];
+    if (w.x != w.y) {
+        S[j * nbf + i] = out[0];
+        T[j * nbf + i] = out[1];
+        V[j * nbf + i] = out[2];
+    }
+}
+
+hipError_t launch_onee_matrices(const BasisView& B, const int4* work, int nwork, const double* rys_tab, double* S,
+                                double* T, double* V, hipStream_t st) {
+    if (nwork <= 0) return hipSuccess;
+    hipLaunchKernelGGL(onee_matrix_kernel, dim3((nwork + 63) / 64), dim3(64), 0, st, B, work, nwork, rys_tab, S, T, V);
+    return hipGetLastError();
 }
 
 hipError_t launch_onee_grad(const BasisView& B, const int4* work, int nwork, const double* D, const double* W,

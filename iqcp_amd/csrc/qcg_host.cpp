@@ -810,13 +810,6 @@ int qcg_rys_roots(qcg_context* c, int nroots, int n, const double* T, double* ro
     return QCG_OK;
 }
 
-int qcg_one_electron(qcg_context* c, double* S, double* T, double* V) {
-    if (!c) return QCG_INVALID_ARG;
-    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
-    one_electron_matrices(c->basis, S, T, V);
-    return QCG_OK;
-}
-
 namespace {
 struct RhfUser {
     qcg_context* c;
@@ -849,7 +842,9 @@ int qcg_rhf_scf(qcg_context* c, const qcg_scf_config* cfg, const double* D_init,
     if (nelec <= 0 || nelec % 2) return fail(c, QCG_INVALID_ARG, "RHF needs an even electron count");
     const size_t n2 = (size_t)B.nbf * B.nbf;
     std::vector<double> S(n2), T(n2), V(n2), H(n2);
-    one_electron_matrices(B, S.data(), T.data(), V.data());
+    HIPCHK(c, hipSetDevice(c->device));
+    const int r1 = one_electron_device(c, B, S.data(), T.data(), V.data());
+    if (r1) return r1;
     for (size_t i = 0; i < n2; ++i) H[i] = T[i] + V[i];
     RhfUser u{c, H.data()};
     qcg_scf_result r{};

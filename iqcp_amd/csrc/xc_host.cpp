@@ -417,7 +417,9 @@ extern "C" int qcg_ks_scf(qcg_context* c, const qcg_scf_config* cfg, const doubl
     if (nelec <= 0 || nelec % 2) return fail(c, QCG_INVALID_ARG, "closed-shell KS needs an even electron count");
     const size_t n2 = (size_t)B.nbf * B.nbf;
     std::vector<double> S(n2), T(n2), V(n2), H(n2);
-    one_electron_matrices(B, S.data(), T.data(), V.data());
+    HIPCHK(c, hipSetDevice(c->device));
+    const int r1 = one_electron_device(c, B, S.data(), T.data(), V.data());
+    if (r1) return r1;
     for (size_t i = 0; i < n2; ++i) H[i] = T[i] + V[i];
     KsUser u{c, H.data(), 0.20, 0, 0, 0, 0};
     qcg_scf_result r{};

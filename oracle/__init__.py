@@ -93,6 +93,18 @@ def eri_compressed(b):
     return out, cnt.value
 
 
+def one_electron(b):
+    """S, T, V (nbf x nbf) of the basis b with the nuclei b.Z at b.xyz."""
+    n = b.nbf
+    S, T, V = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+    Z = np.ascontiguousarray(b.Z, dtype=np.int32)
+    xyz = np.ascontiguousarray(b.xyz, dtype=np.float64)
+    rc = lib().or_one_electron(*_basis_args(b), len(Z), _i(Z), _d(xyz), _d(S), _d(T), _d(V))
+    if rc:
+        raise ValueError("one_electron failed")
+    return S, T, V
+
+
 def build_fock(h, d, eri, nbf):
     f = np.zeros((nbf, nbf))
     lib().or_build_fock(_d(np.ascontiguousarray(h)), _d(np.ascontiguousarray(d)), _d(eri), nbf, _d(f))

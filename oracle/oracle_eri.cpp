@@ -16,6 +16,8 @@
 //   or_eri_compressed  integrals/eri/mod.rs:992-1070                          pinned: PySCF STO-3G tensors
 //   or_build_fock      scf/mod.rs:825-881                                     pinned: H2 PySCF J/K/F (scf/fock.rs:192-229) + numpy einsum
 //   or_build_jk        dft/ks_scf.rs:1306 / :1342 / :1369                     pinned: numpy einsum on the full tensor
+//   or_one_electron    integrals/overlap.rs:647, kinetic.rs, nuclear.rs:226   pinned: PySCF S and h_core of the STO-3G presets
+//                      (binomial Gaussian moments; V by Rys over or_rys_roots)  (content/presets), closed-form s-s values
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -653,6 +655,98 @@ int or_build_fock_omp(const double* H, const double* D, const double* eri, int n
             }
             F[mu * nbf + nu] += g;
             if (mu != nu) F[nu * nbf + mu] += g;
+        }
+    return 0;
+}
+}  // extern "C"
+
+extern "C" {
+// ---------------------------------------------------------------- one-electron S, T, V
+// Written with Gaussian moments instead of recurrences.  Each 1-D factor is the expectation of
+// (x-A)^i (x-B)^j under a normal law of mean m and variance v, by binomial expansion about m:
+//   overlap: m = P, v = 1/(2p);  nuclear attraction (after the Rys change of variable with root
+//   u = t^2): m = P - u (P - C), v = (1 - u)/(2p).
+// Kinetic: -1/2 d^2/dx^2 applied to the ket, (x-B)^j e^{-b(x-B)^2} ->
+//   j(j-1)(x-B)^{j-2} - 2b(2j+1)(x-B)^j + 4b^2 (x-B)^{j+2}.
+static double binom(int n, int k) {
+    double r = 1.0;
+    for (int t = 1; t <= k; ++t) r = r * (n - k + t) / t;
+    return r;
+}
+
+static double gauss_moment_1d(int i, int j, double mA, double mB, double v) {
+    if (i < 0 || j < 0) return 0.0;
+    double s = 0.0;
+    for (int k1 = 0; k1 <= i; ++k1)
+        for (int k2 = 0; k2 <= j; ++k2) {
+            const int k = k1 + k2;
+            if (k % 2) continue;
+            s += binom(i, k1) * binom(j, k2) * std::pow(mA, i - k1) * std::pow(mB, j - k2) * dfact(k - 1) *
+                 std::pow(v, k / 2);
+        }
+    return s;
+}
+
+int or_one_electron(int nshell, const int* l, const int* nprim, const int* prim_off, const double* exps,
+                    const double* coefs, const double* centers, const int* bf_off, int nbf, int natom, const int* Z,
+                    const double* xyz, double* S, double* T, double* V) {
+    const double PI = 3.14159265358979323846;
+    for (int i = 0; i < nbf * nbf; ++i) S[i] = T[i] = V[i] = 0.0;
+    int ca_[15][3], cb_[15][3];
+    for (int si = 0; si < nshell; ++si)
+        for (int sj = 0; sj < nshell; ++sj) {
+            const int la = l[si], lb = l[sj];
+            cart_comps(la, ca_);
+            cart_comps(lb, cb_);
+            const double* A = centers + 3 * si;
+            const double* B = centers + 3 * sj;
+            double ab2 = 0.0;
+            for (int x = 0; x < 3; ++x) ab2 += (A[x] - B[x]) * (A[x] - B[x]);
+            for (int pa = 0; pa < nprim[si]; ++pa)
+                for (int pb = 0; pb < nprim[sj]; ++pb) {
+                    const double a = exps[prim_off[si] + pa], b = exps[prim_off[sj] + pb], p = a + b;
+                    const double K = std::exp(-a * b / p * ab2);
+                    double P[3];
+                    for (int x = 0; x < 3; ++x) P[x] = (a * A[x] + b * B[x]) / p;
+                    for (int ia = 0; ia < ncart(la); ++ia)
+                        for (int jb = 0; jb < ncart(lb); ++jb) {
+                            const int* ip = ca_[ia];
+                            const int* jp = cb_[jb];
+                            const double c = coefs[prim_off[si] + pa] * coefs[prim_off[sj] + pb] * cnorm(a, ip) *
+                                             cnorm(b, jp);
+                            double s1[3], t1[3];
+                            for (int x = 0; x < 3; ++x) {
+                                const double mA = P[x] - A[x], mB = P[x] - B[x], v = 0.5 / p;
+                                const int i = ip[x], j = jp[x];
+                                s1[x] = gauss_moment_1d(i, j, mA, mB, v);
+                                t1[x] = -0.5 * (j * (j - 1) * gauss_moment_1d(i, j - 2, mA, mB, v) -
+                                                2.0 * b * (2 * j + 1) * s1[x] +
+                                                4.0 * b * b * gauss_moment_1d(i, j + 2, mA, mB, v));
+                            }
+                            const double pre = K * std::pow(PI / p, 1.5) * c;
+                            const int idx = (bf_off[si] + ia) * nbf + bf_off[sj] + jb;
+                            S[idx] += pre * s1[0] * s1[1] * s1[2];
+                            T[idx] += pre * (t1[0] * s1[1] * s1[2] + s1[0] * t1[1] * s1[2] + s1[0] * s1[1] * t1[2]);
+                            const int nr = (la + lb) / 2 + 1;
+                            for (int cn = 0; cn < natom; ++cn) {
+                                const double* C = xyz + 3 * cn;
+                                double pc2 = 0.0;
+                                for (int x = 0; x < 3; ++x) pc2 += (P[x] - C[x]) * (P[x] - C[x]);
+                                double rr[MAXR], ww[MAXR];
+                                if (or_rys_roots(nr, p * pc2, rr, ww)) return 1;
+                                double acc = 0.0;
+                                for (int r = 0; r < nr; ++r) {
+                                    double prod = ww[r];
+                                    for (int x = 0; x < 3; ++x) {
+                                        const double m = P[x] - rr[r] * (P[x] - C[x]);
+                                        prod *= gauss_moment_1d(ip[x], jp[x], m - A[x], m - B[x], (1.0 - rr[r]) / (2.0 * p));
+                                    }
+                                    acc += prod;
+                                }
+                                V[idx] += -Z[cn] * 2.0 * PI / p * K * c * acc;
+                            }
+                        }
+                }
         }
     return 0;
 }

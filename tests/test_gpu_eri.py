@@ -280,6 +280,17 @@ def test_gpu_one_electron_vs_presets():
         assert np.abs(T + V - np.array(d["h_core"]).reshape(n, n)).max() < 5e-9
 
 
+@pytest.mark.parametrize("mol,bas", [("H2O", "6-31+g*"), ("CH4", "6-31g*"), ("NH3", "sto-3g")])
+def test_gpu_one_electron_vs_oracle(mol, bas):
+    """Device S, T, V (onee_matrix_kernel: Obara-Saika tables, Rys moments) against the oracle's
+    binomial Gaussian-moment restatement."""
+    b = bench_molecule(mol, bas)
+    S, T, V = _eng(b).one_electron()
+    So, To, Vo = oracle.one_electron(b)
+    for x, y in ((S, So), (T, To), (V, Vo)):
+        np.testing.assert_allclose(x, y, atol=1e-12 * np.abs(y).max(), rtol=0)
+
+
 def test_gpu_rhf_presets_heh_plus():
     """Charged system through qcg_set_charge: HeH+ STO-3G against the PySCF preset energy."""
     d, b = preset("heh_plus_sto3g")

@@ -67,10 +67,10 @@ def test_one_electron_gradient_vs_fd():
 
 
 @pytest.mark.parametrize("mol,bas", [("H2O", "6-31+g*"), ("CH4", "6-31g*"), ("NH3", "sto-3g")])
-def test_device_one_electron_gradient_vs_host(mol, bas):
-    """rhf_gradient runs the one-electron terms on the device (onee_grad_kernel) beside the ERI
-    terms; the host loop (one_electron_gradient) runs the same per-pair code: the sum of the host
-    one-electron and the device ERI gradients must agree to summation order."""
+def test_rhf_gradient_is_sum_of_parts(mol, bas):
+    """rhf_gradient runs the one-electron kernel beside the ERI kernels on the auxiliary streams with
+    W = 2 C_occ diag(eps) C_occ^T formed on the host: it equals one_electron_gradient(D, W) +
+    eri_gradient(D) run separately, also on a second call with the cached device basis."""
     b = build_basis(_distorted(mol, bas, seed=7), bas)
     rng = np.random.default_rng(11)
     C = rng.normal(0, 0.3, (b.nbf, b.nbf))
@@ -82,7 +82,7 @@ def test_device_one_electron_gradient_vs_host(mol, bas):
     g = eng.rhf_gradient(D, C, eps, no)
     ref = eng.one_electron_gradient(D, W) + eng.eri_gradient(D, 1.0)
     np.testing.assert_allclose(g, ref, atol=1e-10 * max(1.0, np.abs(ref).max()), rtol=0)
-    g2 = eng.rhf_gradient(D, C, eps, no)  # the cached device basis is reused
+    g2 = eng.rhf_gradient(D, C, eps, no)
     np.testing.assert_allclose(g2, g, atol=1e-12 * max(1.0, np.abs(ref).max()), rtol=0)
 
 

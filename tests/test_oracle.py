@@ -126,6 +126,40 @@ def test_eri_compressed_vs_pyscf_presets(name):
     assert np.abs(ren - ref).max() < PRESET_TOL[name]
 
 
+# S and h_core against the PySCF matrices of the presets (renormalised as above); h2o and nh3
+# carry the basis-table digit differences
+ONEE_TOL = {"h2_sto3g_r1.4": 1e-11, "heh_plus_sto3g": 1e-11, "lih_sto3g": 1e-10, "h2o_sto3g": 5e-9,
+            "nh3_sto3g": 1e-6}
+
+
+@pytest.mark.parametrize("name", sorted(ONEE_TOL))
+def test_one_electron_vs_pyscf_presets(name):
+    d, b = preset(name)
+    n = b.nbf
+    S, T, V = oracle.one_electron(_renormalize(b))
+    assert np.abs(S - np.array(d["s_matrix"]).reshape(n, n)).max() < ONEE_TOL[name]
+    assert np.abs(T + V - np.array(d["h_core"]).reshape(n, n)).max() < ONEE_TOL[name]
+
+
+def test_one_electron_closed_forms():
+    """Single normalised primitives: a d shell (exponent a) has S_xx,xx = 1, S_xx,yy = 1/3,
+    T_xx,xx = 13a/6, T_xy,xy = 7a/2; an s primitive on its nucleus has T = 3a/2 and
+    V = -2 Z sqrt(2a/pi).  T and V come out symmetric although T acts on the ket only."""
+    b = build_basis([("C", [0.1, -0.2, 0.3]), ("H", [1.0, 0.5, -0.7])], "6-31g*")
+    S, T, V = oracle.one_electron(b)
+    np.testing.assert_allclose(T, T.T, atol=1e-14)
+    np.testing.assert_allclose(V, V.T, atol=1e-13)
+    o = b.shell_bf_off[list(b.shell_l).index(2)]
+    a = b.exps[b.shell_prim_off[list(b.shell_l).index(2)]]
+    assert abs(S[o, o] - 1) < 1e-14 and abs(S[o, o + 3] - 1 / 3) < 1e-14 and abs(S[o + 1, o + 1] - 1) < 1e-14
+    assert abs(T[o, o] - 13 * a / 6) < 1e-13 and abs(T[o + 1, o + 1] - 7 * a / 2) < 1e-13
+    h = build_basis([("H", [0.0, 0.0, 0.0])], "6-31g")
+    S, T, V = oracle.one_electron(h)
+    a, c = h.exps[h.shell_prim_off[1]], h.coefs[h.shell_prim_off[1]]
+    assert abs(T[1, 1] - 1.5 * a * c * c) < 1e-15
+    assert abs(V[1, 1] + 2 * math.sqrt(2 * a / math.pi) * c * c) < 1e-15
+
+
 def test_eri_h2o_631gs_pyscf_values():
     """PySCF int2e incl. Cartesian d (integrals/eri/mod.rs:4839-4925), tolerance 5e-6 as there."""
     d = json.loads((GOLDEN / "h2o_631gs_eri.json").read_text())
