@@ -202,6 +202,19 @@ int qcg_one_electron(qcg_context* ctx, double* S, double* T, double* V);
 int qcg_rhf_scf(qcg_context* ctx, const qcg_scf_config* cfg, const double* D_init, qcg_scf_result* res,
                 double* D, double* C, double* eps, double* F);
 
+/* ---- preset systems: a stored compressed tensor instead of a basis ---------------------- */
+/* build_fock (scf/mod.rs:825-881) as the reference calls it: F = H + sum_ls D_ls [(mn|ls) -
+ * 1/2 (ml|ns)] from the compressed 8-fold tensor eri (eri_len = npair(npair+1)/2, npair =
+ * nbf(nbf+1)/2, eri_index order, scf/mod.rs:608).  H, D, F: nbf x nbf, D symmetric.  The context
+ * needs no basis; a wrong eri_len is QCG_INVALID_ARG ("Invalid ERI array size"). */
+int qcg_build_fock_stored(qcg_context* ctx, int nbf, const double* H, const double* D, const double* eri,
+                          int64_t eri_len, double* F);
+/* rhf_scf (scf/mod.rs:1363) on a PresetSystem (:284): S, H row-major nbf x nbf, the compressed
+ * tensor, nelec, e_nuc; validated as PresetSystem::validate (:359).  Outputs as qcg_rhf_scf. */
+int qcg_rhf_scf_stored(qcg_context* ctx, int nbf, int nelec, double e_nuc, const double* S, const double* H,
+                       const double* eri, int64_t eri_len, const qcg_scf_config* cfg, const double* D_init,
+                       qcg_scf_result* res, double* D, double* C, double* eps, double* F);
+
 /* ---- batches of molecules (after qcg_set_basis_batch) ------------------------------------ */
 /* J, K for every molecule of the batch in one set of launches; D, J, K packed block-diagonally. */
 int qcg_batch_build_jk(qcg_context* ctx, const double* D, double* J, double* K);

@@ -39,7 +39,7 @@ EXPORTS = [
     "qcg_eri_gradient", "qcg_one_electron_gradient", "qcg_rhf_gradient",
     "qcg_set_basis_batch", "qcg_batch_build_jk", "qcg_batch_rhf_scf", "qcg_autotune",
     "qcg_set_tiles", "qcg_get_tiles", "qcg_xc_gradient", "qcg_ks_gradient", "qcg_set_graph",
-    "qcg_set_prim_split", "qcg_set_xc_mode",
+    "qcg_set_prim_split", "qcg_set_xc_mode", "qcg_build_fock_stored", "qcg_rhf_scf_stored",
 ]
 
 
