@@ -169,9 +169,6 @@ QCG_HD void contract_quartet(const PairRec& P, const PairRec& Q,
                                                  const R* __restrict__ rys_tab, R* eri) {
     using S = EriShape<LA, LB, LC, LD>;
     constexpr int N = S::NROOTS;
-    // one-node classes ((ss|ss), (ps|ss)) take two bra primitives per inner iteration (see below):
-    // measured 8 % and 5 % faster; the two-node classes lost more to registers than they gained
-    constexpr bool kDual = N == 1;
     const R abx = (R)P.abx, aby = (R)P.aby, abz = (R)P.abz;
     const R cdx = (R)Q.abx, cdy = (R)Q.aby, cdz = (R)Q.abz;
     const R Ax = (R)P.ax, Ay = (R)P.ay, Az = (R)P.az;
@@ -179,16 +176,14 @@ QCG_HD void contract_quartet(const PairRec& P, const PairRec& Q,
     const R two_pi_52 = (R)(2.0 * kPi * kPi * 1.7724538509055160273);
     // ket primitives outer: in the J/K kernels the bra pair is uniform over the block, so the inner
     // loop's bra primitive loads are uniform (scalar) and each lane loads its ket primitive once
-    // Both loops load the next primitive pair before working on the current one (the last
-    // iteration reloads the last pair), so a load's latency overlaps one iteration's arithmetic.
-    PrimPair<R> cn = pp[Q.pp_off];
     for (int j = 0; j < Q.npp; ++j) {
-        const PrimPair<R> c = cn;
-        cn = pp[Q.pp_off + (j + 1 < Q.npp ? j + 1 : j)];
+        const PrimPair<R> c = pp[Q.pp_off + j];
         const R q = c.p;
         const R hq = c.hp;
         const R QCx = c.px - Cx, QCy = c.py - Cy, QCz = c.pz - Cz;
-        auto quartet = [&](const PrimPair<R>& a, R scale) {
+        for (int i = 0; i < P.npp; ++i) {
+            const PrimPair<R> a = pp[P.pp_off + i];
+            if (a.kab * c.kab < (R)kPrimScreen) continue;
             const R p = a.p;
             const R hp = a.hp;
             // 1/(p+q) = rs^2: one reciprocal square root per primitive quartet
@@ -197,11 +192,10 @@ QCG_HD void contract_quartet(const PairRec& P, const PairRec& Q,
             const R PQx = a.px - c.px, PQy = a.py - c.py, PQz = a.pz - c.pz;
             const R rho = p * q * ipq;
             const R T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-            const R pref = two_pi_52 * rs * a.ckp * c.ckp * scale;
+            const R pref = two_pi_52 * rs * a.ckp * c.ckp;
             R rx[N], rw[N];
-            const R* tabN = TAB_N ? rys_tab : rys_tab + qcg_rys_off[N];
-            if constexpr (kDual) rys_eval_tab_sel<R, N>(T, tabN, rx, rw);
-            else rys_eval_tab<R, N>(T, tabN, rx, rw);
+            if constexpr (TAB_N) rys_eval_tab<R, N>(T, rys_tab, rx, rw);
+            else rys_eval<R, N>(T, rys_tab, rx, rw);
             const R qf = q * ipq, pf = p * ipq;
             const R PAx = a.px - Ax, PAy = a.py - Ay, PAz = a.pz - Az;
 #pragma unroll
@@ -234,28 +228,6 @@ QCG_HD void contract_quartet(const PairRec& P, const PairRec& Q,
                     constexpr const CompTab<LA, LB, LC, LD>& ct = kCompTab<LA, LB, LC, LD>;
                     for_n<false, S::NQ>([&](int k) { eri[k] += ix[ct.hx[k]] * iy[ct.hy[k]] * iz[ct.hz[k]]; });
                 }
-            }
-        };
-        if constexpr (kDual) {
-            // two bra primitives per iteration: two independent chains to interleave; a
-            // screened (or missing) primitive runs with weight 0
-            for (int i = 0; i < P.npp; i += 2) {
-                const int i1 = i + 1 < P.npp ? i + 1 : i;
-                const PrimPair<R> a0 = pp[P.pp_off + i];
-                const PrimPair<R> a1 = pp[P.pp_off + i1];
-                const R s0 = a0.kab * c.kab < (R)kPrimScreen ? R(0) : R(1);
-                const R s1 = (i1 == i || a1.kab * c.kab < (R)kPrimScreen) ? R(0) : R(1);
-                if (s0 == R(0) && s1 == R(0)) continue;
-                quartet(a0, s0);
-                quartet(a1, s1);
-            }
-        } else {
-            PrimPair<R> an = pp[P.pp_off];
-            for (int i = 0; i < P.npp; ++i) {
-                const PrimPair<R> a = an;
-                an = pp[P.pp_off + (i + 1 < P.npp ? i + 1 : i)];
-                if (a.kab * c.kab < (R)kPrimScreen) continue;
-                quartet(a, R(1));
             }
         }
     }

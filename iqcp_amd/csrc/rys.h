@@ -57,25 +57,6 @@ QCG_HD void rys_eval_tab(R T, const R* __restrict__ tabN, R* x, R* w) {
     }
 }
 
-// rys_eval_tab without a branch: both the fit and the large-T limit are formed and one is
-// selected (the fit at T = 0 stands in for large T), so two evaluations can interleave.
-template <typename R, int N>
-QCG_HD void rys_eval_tab_sel(R T, const R* __restrict__ tabN, R* x, R* w) {
-    const bool big = T >= (R)qcg_rys_tmax[N];
-    const R Tt = big ? R(0) : T;
-    const int k = (int)(Tt * (R)(1.0 / QCG_RYS_H));
-    const R s = (Tt - (R)(k * QCG_RYS_H)) * (R)(2.0 / QCG_RYS_H) - R(1);
-    const R* c = tabN + k * (2 * N * kRysStride);
-    const R rs = qrsqrt(big ? T : R(1));
-    const R it = rs * rs;
-#pragma unroll
-    for (int i = 0; i < 2 * N; ++i) {
-        const R v = poly13(c + i * kRysStride, s);
-        if (i < N) x[i] = big ? (R)qcg_rys_hx[rys_tri(N) + i] * it : v;
-        else w[i - N] = big ? (R)qcg_rys_hw[rys_tri(N) + i - N] * rs : v;
-    }
-}
-
 // tab: qcg_rys_coef (double) or its float copy, resident in device memory.
 template <typename R, int N>
 QCG_HD void rys_eval(R T, const R* __restrict__ tab, R* x, R* w) {
