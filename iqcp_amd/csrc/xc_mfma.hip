@@ -364,9 +364,11 @@ static hipError_t launch_xc_mfma_t(int G, int nbf, int nchunk, const double* w, 
                                    double* Vparts, hipStream_t st) {
     const GridVals<T> gv{phi, dpx, dpy, dpz, g4};
     if (G <= 0 || nbf <= 0 || nchunk <= 0) return hipErrorInvalidValue;
-    // column passes of at most 8 MFMA tiles (4 for FP64, whose accumulators take twice the
-    // registers), with the tiles spread evenly over the passes
-    constexpr int kMaxNT = sizeof(T) == 8 ? 4 : 8;
+    // column passes of at most 5 MFMA tiles (4 for FP64, whose accumulators take twice the
+    // registers), with the tiles spread evenly over the passes.  FP32: 5 tiles keep the kernel at
+    // 164 registers, three wavefronts per SIMD; 7 or 8 tiles need about 200 (two wavefronts) and
+    // were 7 % slower on (H2O)16, 4 tiles 1.5 % slower.
+    constexpr int kMaxNT = sizeof(T) == 8 ? 4 : 5;
     const int tiles = (nbf + 15) / 16, passes = (tiles + kMaxNT - 1) / kMaxNT, nt = (tiles + passes - 1) / passes;
     const dim3 grid((G + kRhoBM - 1) / kRhoBM);
     switch (nt) {
@@ -375,7 +377,7 @@ static hipError_t launch_xc_mfma_t(int G, int nbf, int nchunk, const double* w, 
         hipLaunchKernelGGL((xc_rho_kernel<T, n>), grid, dim3(256), 0, st, G, nbf, w, phi, gv, Dm, aow,      \
                            exc_sum);                                                                    \
         break;
-        QCG_RHO_NT(1) QCG_RHO_NT(2) QCG_RHO_NT(3) QCG_RHO_NT(4) QCG_RHO_NT(5) QCG_RHO_NT(6) QCG_RHO_NT(7) QCG_RHO_NT(8)
+        QCG_RHO_NT(1) QCG_RHO_NT(2) QCG_RHO_NT(3) QCG_RHO_NT(4) QCG_RHO_NT(5)
 #undef QCG_RHO_NT
         default: return hipErrorInvalidValue;
     }
