@@ -28,32 +28,45 @@ __global__ __launch_bounds__(128) void onee_grad_kernel(BasisView B, const int4*
         if (gl[i] != 0.0) atomicAdd(&grad[i], gl[i]);
 }
 
-// S, T, V: one thread per Cartesian component pair of the shell pairs si <= sj (work = {si, sj, ia,
-// jb}); a thread writes its element and, off the diagonal blocks, the mirrored one.
-__global__ __launch_bounds__(64) void onee_matrix_kernel(BasisView B, const int4* __restrict__ work, int nwork,
-                                                        const double* __restrict__ rys_tab, double* __restrict__ S,
-                                                        double* __restrict__ T, double* __restrict__ V) {
-    const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= nwork) return;
+// S, T, V: one wavefront per Cartesian component pair of the shell pairs si <= sj (work = {si, sj,
+// ia, jb}); its lanes take the (primitive pair, term) combinations, term = overlap+kinetic or one
+// nucleus, and a fixed-order wavefront reduction sums them.  The wavefront writes its element
+// and, off the diagonal blocks, the mirrored one.
+__global__ __launch_bounds__(256) void onee_matrix_kernel(BasisView B, const int4* __restrict__ work, int nwork,
+                                                         const double* __restrict__ rys_tab, double* __restrict__ S,
+                                                         double* __restrict__ T, double* __restrict__ V) {
+    const int u = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (u >= nwork) return;  // the whole wavefront leaves together
     const int4 w = work[u];
-    double out[3];
-    onee_element(B, w.x, w.y, w.z, w.w, rys_tab, out);
+    const int nb = B.nprim[w.y], nterm = B.natom + 1;
+    const int ncomb = B.nprim[w.x] * nb * nterm;
+    double out[3] = {0.0, 0.0, 0.0};
+    for (int t = lane; t < ncomb; t += 64) {
+        const int pp = t / nterm, c = t - pp * nterm;
+        onee_element_term(B, w.x, w.y, w.z, w.w, pp / nb, pp % nb, c == B.natom ? -1 : c, rys_tab, out);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) out[k] += __shfl_xor(out[k], off, 64);
+    if (lane != 0) return;
+    const double nrm = onee_norm_comp(B.l[w.x], w.z) * onee_norm_comp(B.l[w.y], w.w);
     const int nbf = B.nbf;
     const size_t i = (size_t)B.bf_off[w.x] + w.z, j = (size_t)B.bf_off[w.y] + w.w;
-    S[i * nbf + j] = out[0];
-    T[i * nbf + j] = out[1];
-    V[i * nbf + j] = out[2];
+    S[i * nbf + j] = nrm * out[0];
+    T[i * nbf + j] = nrm * out[1];
+    V[i * nbf + j] = nrm * out[2];
     if (w.x != w.y) {
-        S[j * nbf + i] = out[0];
-        T[j * nbf + i] = out[1];
-        V[j * nbf + i] = out[2];
+        S[j * nbf + i] = nrm * out[0];
+        T[j * nbf + i] = nrm * out[1];
+        V[j * nbf + i] = nrm * out[2];
     }
 }
 
 hipError_t launch_onee_matrices(const BasisView& B, const int4* work, int nwork, const double* rys_tab, double* S,
                                 double* T, double* V, hipStream_t st) {
     if (nwork <= 0) return hipSuccess;
-    hipLaunchKernelGGL(onee_matrix_kernel, dim3((nwork + 63) / 64), dim3(64), 0, st, B, work, nwork, rys_tab, S, T, V);
+    hipLaunchKernelGGL(onee_matrix_kernel, dim3((nwork + 3) / 4), dim3(256), 0, st, B, work, nwork, rys_tab, S, T, V);
     return hipGetLastError();
 }
 

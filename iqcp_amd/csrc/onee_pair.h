@@ -57,6 +57,15 @@ QCG_HD void onee_overlap_1d(double PA, double PB, double oo2p, int imax, int jma
         }
 }
 
+// Rys rule of n <= 3 nodes (the most the one-electron terms of s, p, d shells need, derivatives
+// included): only these orders are instantiated, which keeps the kernels' registers down.
+QCG_HD void onee_rys(int n, double T, const double* rys_tab, double* x, double* w) {
+    static_assert(kMaxL <= 2, "one-electron Rys orders above 3");
+    if (n == 1) rys_eval<double, 1>(T, rys_tab, x, w);
+    else if (n == 2) rys_eval<double, 2>(T, rys_tab, x, w);
+    else rys_eval<double, 3>(T, rys_tab, x, w);
+}
+
 // Geometry and contraction data of one primitive pair.
 struct OneePair {
     int la, lb, na, nb, atA, atB, bfa, bfb;
@@ -159,7 +168,7 @@ QCG_HD void onee_grad_nuclear(const BasisView& B, int si, int sj, int pi, int pj
     const double PC[3] = {q.P[0] - C[0], q.P[1] - C[1], q.P[2] - C[2]};
     const double Tn = q.p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
     double rx[QCG_RYS_NMAX], rw[QCG_RYS_NMAX];
-    rys_eval_n<double>(nr, Tn, rys_tab, rx, rw);
+    onee_rys(nr, Tn, rys_tab, rx, rw);
     double g[kNr][3][LV * LV];
     for (int r = 0; r < nr; ++r) {
         const double u = rx[r];
@@ -205,74 +214,68 @@ QCG_HD void onee_grad_nuclear(const BasisView& B, int si, int sj, int pi, int pj
     }
 }
 
-// S, T and V (summed over all nuclei) of the Cartesian component pair (ia, jb) of the ordered shell
-// pair (si, sj), contracted over its primitive pairs (overlap_matrix, integrals/overlap.rs:647;
-// kinetic, integrals/kinetic.rs; nuclear attraction, integrals/nuclear.rs:226): out[0..2].
-// Overlap and kinetic from the Obara-Saika 1-D tables; V by Rys quadrature, the 1-D moments
-// h_n = <(x-A)^n> by their two-term recurrence and (x-B)^j moved onto A binomially.
-QCG_HD void onee_element(const BasisView& B, int si, int sj, int ia, int jb, const double* rys_tab, double* out) {
+// One term of S, T, V for the Cartesian component pair (ia, jb) of the ordered shell pair (si, sj)
+// and its primitive pair (pi, pj) (overlap_matrix, integrals/overlap.rs:647; kinetic,
+// integrals/kinetic.rs; nuclear attraction, integrals/nuclear.rs:226), component norms left out:
+// cnuc < 0: overlap and kinetic into out[0], out[1]; cnuc >= 0: the attraction to nucleus cnuc
+// into out[2].  Overlap and kinetic from the Obara-Saika 1-D tables; V by Rys quadrature, the 1-D
+// moments h_n = <(x-A)^n> by their two-term recurrence and (x-B)^j moved onto A binomially.
+QCG_HD void onee_element_term(const BasisView& B, int si, int sj, int ia, int jb, int pi, int pj, int cnuc,
+                              const double* rys_tab, double* out) {
     constexpr int LD = kMaxL + 3;
     const int la = B.l[si], lb = B.l[sj];
     const int ip[3] = {cart_x(la, ia), cart_y(la, ia), cart_z(la, ia)};
     const int jp[3] = {cart_x(lb, jb), cart_y(lb, jb), cart_z(lb, jb)};
-    const int nr = (la + lb) / 2 + 1;
-    double s_acc = 0.0, t_acc = 0.0, v_acc = 0.0;
-    for (int pi = 0; pi < B.nprim[si]; ++pi)
-        for (int pj = 0; pj < B.nprim[sj]; ++pj) {
-            const OneePair q = onee_pair_setup(B, si, sj, pi, pj);
-            const double a = q.a, b = q.b;
-            double s1[3][LD * LD];
-            double sx[3], tx[3];
-            for (int x = 0; x < 3; ++x) {
-                const int i = ip[x], j = jp[x];
-                onee_overlap_1d(q.PA[x], q.PB[x], q.oo2p, i + 1, j + 1, s1[x], LD);
-                const double* t = s1[x];
-                sx[x] = t[i * LD + j];
-                double v = 4.0 * a * b * t[(i + 1) * LD + j + 1];
-                if (i > 0) v -= 2.0 * b * i * t[(i - 1) * LD + j + 1];
-                if (j > 0) v -= 2.0 * a * j * t[(i + 1) * LD + j - 1];
-                if (i > 0 && j > 0) v += (double)(i * j) * t[(i - 1) * LD + j - 1];
-                tx[x] = 0.5 * v;
-            }
-            const double pre_s = q.kab * pow(kPi / q.p, 1.5) * q.ca * q.cb;
-            s_acc += pre_s * sx[0] * sx[1] * sx[2];
-            t_acc += pre_s * (tx[0] * sx[1] * sx[2] + sx[0] * tx[1] * sx[2] + sx[0] * sx[1] * tx[2]);
-            for (int cn = 0; cn < B.natom; ++cn) {
-                const double* C = &B.xyz[3 * cn];
-                const double PC[3] = {q.P[0] - C[0], q.P[1] - C[1], q.P[2] - C[2]};
-                const double Tn = q.p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
-                double rx[QCG_RYS_NMAX], rw[QCG_RYS_NMAX];
-                rys_eval_n<double>(nr, Tn, rys_tab, rx, rw);
-                double acc = 0.0;
-                for (int r = 0; r < nr; ++r) {
-                    const double u = rx[r];
-                    double prod = rw[r];
-                    for (int x = 0; x < 3; ++x) {
-                        const int i = ip[x], j = jp[x];
-                        double h[2 * kMaxL + 1];
-                        const double c0 = q.PA[x] - u * PC[x];
-                        const double bb = (1.0 - u) * q.oo2p;
-                        h[0] = 1.0;
-                        if (i + j > 0) h[1] = c0;
-                        for (int n = 1; n < i + j; ++n) h[n + 1] = c0 * h[n] + n * bb * h[n - 1];
-                        // (x-B)^j = sum_k C(j,k) (x-A)^k (A-B)^(j-k), from k = j down
-                        double g = 0.0, pw = 1.0, cb = 1.0;
-                        for (int k = j; k >= 0; --k) {
-                            g += cb * pw * h[i + k];
-                            pw *= q.AB[x];
-                            cb = cb * k / (j - k + 1);
-                        }
-                        prod *= g;
-                    }
-                    acc += prod;
-                }
-                v_acc += -(double)B.Z[cn] * 2.0 * kPi / q.p * q.kab * q.ca * q.cb * acc;
-            }
+    const OneePair q = onee_pair_setup(B, si, sj, pi, pj);
+    if (cnuc < 0) {
+        const double a = q.a, b = q.b;
+        double s1[LD * LD];
+        double sx[3], tx[3];
+        for (int x = 0; x < 3; ++x) {
+            const int i = ip[x], j = jp[x];
+            onee_overlap_1d(q.PA[x], q.PB[x], q.oo2p, i + 1, j + 1, s1, LD);
+            sx[x] = s1[i * LD + j];
+            double v = 4.0 * a * b * s1[(i + 1) * LD + j + 1];
+            if (i > 0) v -= 2.0 * b * i * s1[(i - 1) * LD + j + 1];
+            if (j > 0) v -= 2.0 * a * j * s1[(i + 1) * LD + j - 1];
+            if (i > 0 && j > 0) v += (double)(i * j) * s1[(i - 1) * LD + j - 1];
+            tx[x] = 0.5 * v;
         }
-    const double nrm = onee_norm_comp(la, ia) * onee_norm_comp(lb, jb);
-    out[0] = nrm * s_acc;
-    out[1] = nrm * t_acc;
-    out[2] = nrm * v_acc;
+        const double pre_s = q.kab * pow(kPi / q.p, 1.5) * q.ca * q.cb;
+        out[0] += pre_s * sx[0] * sx[1] * sx[2];
+        out[1] += pre_s * (tx[0] * sx[1] * sx[2] + sx[0] * tx[1] * sx[2] + sx[0] * sx[1] * tx[2]);
+        return;
+    }
+    const int nr = (la + lb) / 2 + 1;
+    const double* C = &B.xyz[3 * cnuc];
+    const double PC[3] = {q.P[0] - C[0], q.P[1] - C[1], q.P[2] - C[2]};
+    const double Tn = q.p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+    double rx[QCG_RYS_NMAX], rw[QCG_RYS_NMAX];
+    onee_rys(nr, Tn, rys_tab, rx, rw);
+    double acc = 0.0;
+    for (int r = 0; r < nr; ++r) {
+        const double u = rx[r];
+        double prod = rw[r];
+        for (int x = 0; x < 3; ++x) {
+            const int i = ip[x], j = jp[x];
+            double h[2 * kMaxL + 1];
+            const double c0 = q.PA[x] - u * PC[x];
+            const double bb = (1.0 - u) * q.oo2p;
+            h[0] = 1.0;
+            if (i + j > 0) h[1] = c0;
+            for (int n = 1; n < i + j; ++n) h[n + 1] = c0 * h[n] + n * bb * h[n - 1];
+            // (x-B)^j = sum_k C(j,k) (x-A)^k (A-B)^(j-k), from k = j down
+            double g = 0.0, pw = 1.0, cb = 1.0;
+            for (int k = j; k >= 0; --k) {
+                g += cb * pw * h[i + k];
+                pw *= q.AB[x];
+                cb = cb * k / (j - k + 1);
+            }
+            prod *= g;
+        }
+        acc += prod;
+    }
+    out[2] += -(double)B.Z[cnuc] * 2.0 * kPi / q.p * q.kab * q.ca * q.cb * acc;
 }
 
 }  // namespace qcg
