@@ -35,16 +35,26 @@ int qcg_batch_rhf_scf(qcg_context* c, const qcg_scf_config* cfg, int use_graph, 
     const int nmol = c->nmol;
     std::vector<ScfRun> runs(nmol);
     std::vector<std::vector<double>> Ss(nmol), Hs(nmol);
-    // one-electron matrices on the device, molecule by molecule
-    for (int m = 0; m < nmol; ++m) {
-        const HostBasis& B = c->mols[m];
-        const size_t n2 = (size_t)B.nbf * B.nbf;
-        std::vector<double> T(n2), V(n2);
-        Ss[m].resize(n2);
-        Hs[m].resize(n2);
-        const int r1 = one_electron_device(c, B, Ss[m].data(), T.data(), V.data());
+    // one-electron matrices of all molecules on the device in one go
+    {
+        std::vector<std::vector<double>> Ts(nmol), Vs(nmol);
+        std::vector<const HostBasis*> bp(nmol);
+        std::vector<double*> sp(nmol), tp(nmol), vp(nmol);
+        for (int m = 0; m < nmol; ++m) {
+            const size_t n2 = (size_t)c->mols[m].nbf * c->mols[m].nbf;
+            Ss[m].resize(n2);
+            Hs[m].resize(n2);
+            Ts[m].resize(n2);
+            Vs[m].resize(n2);
+            bp[m] = &c->mols[m];
+            sp[m] = Ss[m].data();
+            tp[m] = Ts[m].data();
+            vp[m] = Vs[m].data();
+        }
+        const int r1 = one_electron_device_many(c, nmol, bp.data(), sp.data(), tp.data(), vp.data());
         if (r1) return r1;
-        for (size_t i = 0; i < n2; ++i) Hs[m][i] = T[i] + V[i];
+        for (int m = 0; m < nmol; ++m)
+            for (size_t i = 0; i < Hs[m].size(); ++i) Hs[m][i] = Ts[m][i] + Vs[m][i];
     }
     int bad = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)

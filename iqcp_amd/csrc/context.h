@@ -187,6 +187,9 @@ int eri_grad_launch(qcg_context* c, const double* dD, double hf_frac, double* dg
 int onee_setup(qcg_context* c);
 // S, T, V of the basis B (any output may be null), nbf x nbf row-major host arrays.
 int one_electron_device(qcg_context* c, const HostBasis& B, double* S, double* T, double* V);
+// The same for n bases in one go (S, T, V: arrays of n host pointers, or null).
+int one_electron_device_many(qcg_context* c, int n, const HostBasis* const* B, double* const* S, double* const* T,
+                             double* const* V);
 // W = 2 sum_i eps_i C_mu,i C_nu,i over the n_occ occupied orbitals (C row-major nbf x nbf)
 std::vector<double> energy_weighted_density(int nbf, int n_occ, const double* C, const double* eps);
 // eri_grad_launch for the host density D (and energy-weighted density W, optional) into the

@@ -74,39 +74,77 @@ int onee_setup(qcg_context* c) {
     return QCG_OK;
 }
 
-// S, T, V of the basis B (any output may be null), full nbf x nbf row-major, on c's device.
-int one_electron_device(qcg_context* c, const HostBasis& B, double* S, double* T, double* V) {
-    const size_t n2 = (size_t)B.nbf * B.nbf;
+// S, T, V of the bases B[0..n) (any output may be null), full nbf x nbf row-major, on c's device:
+// one allocation per kind for all of them, one launch per basis, one synchronisation.
+int one_electron_device_many(qcg_context* c, int n, const HostBasis* const* B, double* const* S, double* const* T,
+                             double* const* V) {
+    std::vector<int> iv;
+    std::vector<double> dv;
     std::vector<int4> work;
-    for (int si = 0; si < B.nshell; ++si)
-        for (int sj = si; sj < B.nshell; ++sj)
-            for (int ia = 0; ia < ncart(B.l[si]); ++ia)
-                for (int jb = 0; jb < ncart(B.l[sj]); ++jb) work.push_back(make_int4(si, sj, ia, jb));
+    std::vector<size_t> ioff(n), doff(n), woff(n + 1), moff(n + 1);
+    moff[0] = 0;
+    for (int m = 0; m < n; ++m) {
+        const HostBasis& b = *B[m];
+        ioff[m] = iv.size();
+        doff[m] = dv.size();
+        woff[m] = work.size();
+        for (const std::vector<int>* v : {&b.l, &b.nprim, &b.prim_off, &b.shell_atom, &b.bf_off, &b.Z})
+            iv.insert(iv.end(), v->begin(), v->end());
+        for (const std::vector<double>* v : {&b.exps, &b.coefs, &b.centers, &b.xyz})
+            dv.insert(dv.end(), v->begin(), v->end());
+        for (int si = 0; si < b.nshell; ++si)
+            for (int sj = si; sj < b.nshell; ++sj)
+                for (int ia = 0; ia < ncart(b.l[si]); ++ia)
+                    for (int jb = 0; jb < ncart(b.l[sj]); ++jb) work.push_back(make_int4(si, sj, ia, jb));
+        moff[m + 1] = moff[m] + 3 * (size_t)b.nbf * b.nbf;
+    }
+    woff[n] = work.size();
     int* di = nullptr;
     double *dd = nullptr, *dm = nullptr;
     int4* dw = nullptr;
-    BasisView view{};
-    int rc = upload_basis(c, B, &di, &dd, &view);
+    int rc = QCG_OK;
     auto chk = [&](hipError_t e, const char* what) {
         if (e != hipSuccess && rc == QCG_OK) rc = fail(c, QCG_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
     };
-    if (rc == QCG_OK) chk(c->alloc(&dw, work.size()), "hipMalloc");
-    if (rc == QCG_OK) chk(c->alloc(&dm, 3 * n2), "hipMalloc");
-    if (rc == QCG_OK)
+    chk(c->alloc(&di, iv.size()), "hipMalloc");
+    chk(c->alloc(&dd, dv.size()), "hipMalloc");
+    chk(c->alloc(&dw, work.size()), "hipMalloc");
+    chk(c->alloc(&dm, moff[n]), "hipMalloc");
+    if (rc == QCG_OK) {
+        chk(hipMemcpyAsync(di, iv.data(), iv.size() * sizeof(int), hipMemcpyHostToDevice, c->stream), "upload");
+        chk(hipMemcpyAsync(dd, dv.data(), dv.size() * sizeof(double), hipMemcpyHostToDevice, c->stream), "upload");
         chk(hipMemcpyAsync(dw, work.data(), work.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream), "upload");
-    if (rc == QCG_OK)
-        chk(launch_onee_matrices(view, dw, (int)work.size(), c->d_rys, dm, dm + n2, dm + 2 * n2, c->stream),
+    }
+    for (int m = 0; m < n && rc == QCG_OK; ++m) {
+        const HostBasis& b = *B[m];
+        const int ns = b.nshell, np = (int)b.exps.size();
+        const int* ip = di + ioff[m];
+        const double* dp = dd + doff[m];
+        const BasisView view{ns,          b.nbf,       b.natom,     ip,      ip + ns,     ip + 2 * ns,
+                             ip + 3 * ns, ip + 4 * ns, ip + 5 * ns, dp,      dp + np,     dp + 2 * np,
+                             dp + 2 * np + 3 * ns};
+        const size_t n2 = (size_t)b.nbf * b.nbf;
+        double* mm = dm + moff[m];
+        chk(launch_onee_matrices(view, dw + woff[m], (int)(woff[m + 1] - woff[m]), c->d_rys, mm, mm + n2, mm + 2 * n2,
+                                 c->stream),
             "launch_onee_matrices");
-    double* outs[3] = {S, T, V};
-    for (int k = 0; k < 3 && rc == QCG_OK; ++k)
-        if (outs[k])
-            chk(hipMemcpyAsync(outs[k], dm + k * n2, n2 * sizeof(double), hipMemcpyDeviceToHost, c->stream), "download");
+        double* outs[3] = {S ? S[m] : nullptr, T ? T[m] : nullptr, V ? V[m] : nullptr};
+        for (int k = 0; k < 3; ++k)
+            if (outs[k])
+                chk(hipMemcpyAsync(outs[k], mm + k * n2, n2 * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+                    "download");
+    }
     chk(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     c->release(di);
     c->release(dd);
     c->release(dw);
     c->release(dm);
     return rc;
+}
+
+int one_electron_device(qcg_context* c, const HostBasis& B, double* S, double* T, double* V) {
+    const HostBasis* b = &B;
+    return one_electron_device_many(c, 1, &b, S ? &S : nullptr, T ? &T : nullptr, V ? &V : nullptr);
 }
 
 }  // namespace qcg
