@@ -285,10 +285,14 @@ __global__ __launch_bounds__(256) void xc_v_kernel(int G, int nbf, int ntile, in
                 a[i] = As[ks + lk][64 * wr + 16 * i + li];
                 b[i] = Bs[ks + lk][64 * wc + 16 * i + li];
             }
+            // 16 x 16 sub-tiles wholly past nbf (the edge tiles of the 128 x 128 grid) are skipped:
+            // for nbf = 304 that is 37 % of the products; the condition is uniform per wavefront
 #pragma unroll
             for (int i = 0; i < kVW; ++i)
 #pragma unroll
-                for (int j = 0; j < kVW; ++j) acc[i][j] = M::mma(a[i], b[j], acc[i][j]);
+                for (int j = 0; j < kVW; ++j)
+                    if (m0 + 64 * wr + 16 * i < nbf && n0 + 64 * wc + 16 * j < nbf)
+                        acc[i][j] = M::mma(a[i], b[j], acc[i][j]);
         }
     }
     double* out = Vparts + (size_t)ch * nbf * nbf;
