@@ -195,7 +195,8 @@ int qcg_ks_scf(qcg_context* ctx, const qcg_scf_config* cfg, const double* D_init
 
 /* ---- one-electron terms and SCF ---------------------------------------------------------- */
 /* overlap_matrix (integrals/overlap.rs:647), kinetic and nuclear attraction (hcore_matrix,
- * integrals/hcore.rs:103 = T + V).  Any output may be NULL. */
+ * integrals/hcore.rs:103 = T + V), computed on the device.  Any output may be NULL.  One molecule
+ * per context (a batch context is QCG_INVALID_ARG). */
 int qcg_one_electron(qcg_context* ctx, double* S, double* T, double* V);
 /* rhf_scf (scf/mod.rs:1363) with the integral-direct GPU Fock build.  D_init (optional) seeds the
  * density (rhf_scf_with_guess, :1610).  Outputs (optional): D, C (columns = MOs), eps, F. */
@@ -229,8 +230,8 @@ int qcg_batch_rhf_scf(qcg_context* ctx, const qcg_scf_config* cfg, int use_graph
  * Schwarz-surviving quartets of [1/2 D_pq D_rs - (hf_frac/4) D_pr D_qs] d(pq|rs)/dR.  grad is
  * overwritten (not accumulated). */
 int qcg_eri_gradient(qcg_context* ctx, const double* D, double hf_frac, double* grad);
-/* One-electron + nuclear part: dVnn/dR + Tr(D dH/dR) - Tr(W dS/dR) (rhf_gradient :915-950).
- * grad overwritten. */
+/* One-electron + nuclear part: dVnn/dR + Tr(D dH/dR) - Tr(W dS/dR) (rhf_gradient :915-950), the
+ * one-electron terms on the device.  grad overwritten. */
 int qcg_one_electron_gradient(qcg_context* ctx, const double* D, const double* W, double* grad);
 /* rhf_gradient (scf/gradient.rs:900): W = 2 sum_i<n_occ eps_i C_i C_i^T (:101), then both parts.
  * C: nbf x nbf, columns = MOs (row-major storage, C[mu*nbf + i]). */
