@@ -110,7 +110,11 @@ def main():
     def timed(prec):
         eng.set_options(prec, args.fp32_cutoff)
         # per-class tile choice for this precision mode
-        tiles[prec] = eng.get_tiles() if args.no_autotune else eng.autotune(dD.data_ptr(), 2)
+        if args.no_autotune:
+            tiles[prec] = eng.get_tiles()
+        else:
+            eng.autotune(dD.data_ptr(), 2)
+            tiles[prec] = eng.autotune_refine(dD.data_ptr(), 30)
         eng.set_graph(not args.no_graph)  # one HIP-graph replay per Fock build
         with torch.cuda.stream(stream):
             for _ in range(args.warmup):

@@ -150,6 +150,11 @@ int qcg_set_profiling(qcg_context* ctx, int on);
  * (iters timed builds per candidate) and keep the fastest.  chosen (optional, 84 ints) receives
  * (block, kets) per class and part: [class][part][2]. */
 int qcg_autotune(qcg_context* ctx, const double* dD, int iters, int* chosen);
+/* No counterpart (launch configuration).  After qcg_autotune: greedy over the classes, longest
+ * first, and over the same candidates, changing one class's FP64-part tile at a time and keeping
+ * a change when `builds` whole builds (all classes, four streams) run at least 0.5 % faster in two
+ * measurements.  chosen (optional) as for qcg_autotune. */
+int qcg_autotune_refine(qcg_context* ctx, const double* dD, int builds, int* chosen);
 /* Set / read the J/K tiles directly (84 ints, same layout as qcg_autotune's chosen; classes in the
  * order (dd|dd), (dd|dp), ..., (ss|ss)).  No counterpart in qc-core: launch configuration only.
  * Block 0 on a class with fewer than 27 components runs the per-lane kernel with 64 threads; classes

@@ -492,25 +492,47 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
 
 extern "C" {
 
+}  // extern "C"
+
+namespace {
+
+// Autotuner candidates {threads per block, kets per thread}; block 0 = one wave per work item of
+// `reps` kets (wave-cooperative kernel, classes with at least 27 components only).
+constexpr int kTuneCand[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {256, 1}, {64, 8}, {0, 1}, {0, 2}, {0, 4}};
+constexpr int kTuneNCand = sizeof(kTuneCand) / sizeof(kTuneCand[0]);
+
+// Candidate k mapped onto what class cls has: classes with fewer than 27 components have only the
+// per-lane kernel, those above kJkLaneMaxNQ only the wave-cooperative one.
+qcg_context::Tile tune_tile(const qcg_context* c, int k, int cls) {
+    const int nq = c->class_nq[cls];
+    if (nq < 27) return kTuneCand[k][0] == 0 ? qcg_context::Tile{64, 1} : qcg_context::Tile{kTuneCand[k][0], kTuneCand[k][1]};
+    if (nq > kJkLaneMaxNQ)
+        return kTuneCand[k][0] == 0 ? qcg_context::Tile{0, kTuneCand[k][1]}
+                                    : qcg_context::Tile{0, std::min(8, kTuneCand[k][1] * 2)};
+    return qcg_context::Tile{kTuneCand[k][0], kTuneCand[k][1]};
+}
+
+void export_tiles(const qcg_context* c, int* chosen) {
+    if (!chosen) return;
+    for (int i = 0; i < 21; ++i)
+        for (int part = 0; part < 2; ++part) {
+            chosen[4 * i + 2 * part] = c->tile[i][part].block;
+            chosen[4 * i + 2 * part + 1] = c->tile[i][part].reps;
+        }
+}
+
+}  // namespace
+
+extern "C" {
+
 // Autotuner: time every class kernel under each candidate tile (threads per block x kets per
 // thread) on the given density and keep the fastest candidate per class.
 int qcg_autotune(qcg_context* c, const double* dD, int iters, int* chosen) {
     if (!c || !dD) return QCG_INVALID_ARG;
     if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
     HIPCHK(c, hipSetDevice(c->device));
-    // {threads per block, kets per thread}; block 0 = one wave per work item of `reps` kets
-    // (wave-cooperative kernel, classes with at least 27 components only)
-    const int cand[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {256, 1}, {64, 8}, {0, 1}, {0, 2}, {0, 4}};
-    const int ncand = sizeof(cand) / sizeof(cand[0]);
-    // classes with fewer than 27 components have only the per-lane kernel, those above kJkLaneMaxNQ
-    // only the wave-cooperative one: map each candidate onto what the class has
-    auto tile_of = [&](int k, int cls) {
-        const int nq = c->class_nq[cls];
-        if (nq < 27) return cand[k][0] == 0 ? qcg_context::Tile{64, 1} : qcg_context::Tile{cand[k][0], cand[k][1]};
-        if (nq > kJkLaneMaxNQ)
-            return cand[k][0] == 0 ? qcg_context::Tile{0, cand[k][1]} : qcg_context::Tile{0, std::min(8, cand[k][1] * 2)};
-        return qcg_context::Tile{cand[k][0], cand[k][1]};
-    };
+    const int ncand = kTuneNCand;
+    auto tile_of = [&](int k, int cls) { return tune_tile(c, k, cls); };
     std::vector<double> best(42, 1e300);
     std::vector<int> pick(42, 0);
     const bool prof = c->profiling;
@@ -538,15 +560,87 @@ int qcg_autotune(qcg_context* c, const double* dD, int iters, int* chosen) {
     }
     for (int i = 0; i < 21; ++i) c->class_ms[i] = (best[2 * i] + best[2 * i + 1]) / iters;
     for (int i = 0; i < 21; ++i)
-        for (int part = 0; part < 2; ++part) {
-            c->tile[i][part] = tile_of(pick[2 * i + part], i);
-            if (chosen) {
-                chosen[4 * i + 2 * part] = c->tile[i][part].block;
-                chosen[4 * i + 2 * part + 1] = c->tile[i][part].reps;
-            }
-        }
+        for (int part = 0; part < 2; ++part) c->tile[i][part] = tile_of(pick[2 * i + part], i);
+    export_tiles(c, chosen);
     c->tile_sig += 1;
     c->profiling = prof;
+    return QCG_OK;
+}
+
+// Second tuning pass on the whole build: the per-class timings above run one class at a time,
+// while a build runs the 21 class kernels on four streams at once, where a tile's register and
+// LDS footprint also decides how the kernels share the GPU.  Greedy over the classes (longest
+// first) and the candidates of each: keep a change of the FP64-part tile (the tile the fused
+// mixed launch cuts both parts with) when `builds` whole builds get at least 0.5 % faster, twice.
+int qcg_autotune_refine(qcg_context* c, const double* dD, int builds, int* chosen) {
+    if (!c || !dD) return QCG_INVALID_ARG;
+    if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
+    HIPCHK(c, hipSetDevice(c->device));
+    builds = std::max(builds, 4);
+    const bool prof = c->profiling;
+    c->profiling = false;
+    int rc = prepare_jk(c);
+    if (rc) return rc;
+    if (!c->ev_made) {  // the timing events (made by the first profiled build otherwise)
+        for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
+        c->ev_made = true;
+    }
+    double ms = 0.0;
+    // eager builds (the class kernels on the four streams, as in a build); a variant that timed
+    // captured-graph replays instead crashed the host process once (cause not found), so it is
+    // not used
+    auto measure = [&]() -> int {
+        for (int w = 0; w < 2; ++w) {  // warm-up; the first also rebuilds the plan for new tiles
+            int r = run_jk(c, dD, c->d_J, c->d_K, c->stream);
+            if (r) return r;
+        }
+        HIPCHK(c, hipEventRecord(c->ev[84], c->stream));
+        for (int b = 0; b < builds; ++b) {
+            int r = run_jk(c, dD, c->d_J, c->d_K, c->stream);
+            if (r) return r;
+        }
+        HIPCHK(c, hipEventRecord(c->ev[85], c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev[85]));
+        float el = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&el, c->ev[84], c->ev[85]));
+        ms = (double)el / builds;
+        return QCG_OK;
+    };
+    double best = 1e300;
+    for (int r = 0; r < 2; ++r) {
+        if ((rc = measure())) break;
+        best = std::min(best, ms);
+    }
+    int order[21];
+    for (int i = 0; i < 21; ++i) order[i] = i;
+    std::stable_sort(order, order + 21, [&](int x, int y) { return c->class_ms[x] > c->class_ms[y]; });
+    for (int oi = 0; oi < 21 && !rc; ++oi) {
+        const int cls = order[oi];
+        if (c->item_cnt[cls][0] + c->item_cnt[cls][1] == 0) continue;
+        for (int k = 0; k < kTuneNCand && !rc; ++k) {
+            const qcg_context::Tile cur = c->tile[cls][0], t = tune_tile(c, k, cls);
+            if (t.block == cur.block && t.reps == cur.reps) continue;
+            c->tile[cls][0] = t;
+            c->tile_sig += 1;
+            if ((rc = measure())) break;
+            double got = ms;
+            bool keep = got < 0.995 * best;
+            if (keep) {
+                if ((rc = measure())) break;
+                keep = ms < 0.995 * best;
+                got = std::min(got, ms);
+            }
+            if (keep) {
+                best = got;
+            } else {
+                c->tile[cls][0] = cur;
+                c->tile_sig += 1;
+            }
+        }
+    }
+    c->profiling = prof;
+    if (rc) return rc;
+    export_tiles(c, chosen);
     return QCG_OK;
 }
 

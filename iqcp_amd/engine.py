@@ -40,6 +40,7 @@ EXPORTS = [
     "qcg_set_basis_batch", "qcg_batch_build_jk", "qcg_batch_rhf_scf", "qcg_autotune",
     "qcg_set_tiles", "qcg_get_tiles", "qcg_xc_gradient", "qcg_ks_gradient", "qcg_set_graph",
     "qcg_set_prim_split", "qcg_set_xc_mode", "qcg_build_fock_stored", "qcg_rhf_scf_stored",
+    "qcg_autotune_refine",
 ]
 
 
@@ -226,6 +227,13 @@ class Engine:
         Returns [[(block, kets) FP64 part, (block, kets) FP32 part] per class]."""
         ch = (ctypes.c_int * 84)()
         self._check(self.L.qcg_autotune(self.ctx, ctypes.c_void_p(dD_ptr), int(iters), ch))
+        return [[(ch[4 * i], ch[4 * i + 1]), (ch[4 * i + 2], ch[4 * i + 3])] for i in range(21)]
+
+    def autotune_refine(self, dD_ptr: int, builds: int = 30):
+        """Refine the autotuned tiles on whole builds (all classes on the four streams at once).
+        Returns the tiles in the form of autotune()."""
+        ch = (ctypes.c_int * 84)()
+        self._check(self.L.qcg_autotune_refine(self.ctx, ctypes.c_void_p(dD_ptr), int(builds), ch))
         return [[(ch[4 * i], ch[4 * i + 1]), (ch[4 * i + 2], ch[4 * i + 3])] for i in range(21)]
 
     def set_tiles(self, tiles):

@@ -223,6 +223,14 @@ def test_gpu_autotune_keeps_parity():
         tol = 1e-11 if prec == "fp64" else 1e-8
         np.testing.assert_allclose(jk.J, Jo, atol=tol, rtol=0)
         np.testing.assert_allclose(jk.K, Ko, atol=tol, rtol=0)
+        # the whole-build refinement changes only FP64-part tiles, among the same candidates
+        refined = eng.autotune_refine(dD.data_ptr(), 4)
+        assert all(tuple(t) in cands for pair in refined for t in pair)
+        assert [pair[1] for pair in refined] == [tuple(pair[1]) for pair in tiles]
+        assert eng.get_tiles() == [[tuple(t) for t in pair] for pair in refined]
+        jk = eng.build_jk(D)
+        np.testing.assert_allclose(jk.J, Jo, atol=tol, rtol=0)
+        np.testing.assert_allclose(jk.K, Ko, atol=tol, rtol=0)
 
 
 def test_gpu_set_tiles_rejects_bad_tiles():
