@@ -576,30 +576,54 @@ int qcg_autotune_refine(qcg_context* c, const double* dD, int builds, int* chose
     if (!c || !dD) return QCG_INVALID_ARG;
     if (!c->has_basis) return fail(c, QCG_NO_BASIS, "no basis");
     HIPCHK(c, hipSetDevice(c->device));
+    if (c->nmol != 1) return fail(c, QCG_INVALID_ARG, "autotune_refine: one molecule per context");
     builds = std::max(builds, 4);
-    const bool prof = c->profiling;
-    c->profiling = false;
     int rc = prepare_jk(c);
     if (rc) return rc;
     if (!c->ev_made) {  // the timing events (made by the first profiled build otherwise)
         for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
         c->ev_made = true;
     }
+    const bool prof = c->profiling;
+    c->profiling = false;
     double ms = 0.0;
-    // eager builds (the class kernels on the four streams, as in a build); a variant that timed
-    // captured-graph replays instead crashed the host process once (cause not found), so it is
-    // not used
+    // Timed as the device Fock builds run: qcg_build_fock_device replaying its captured graph on
+    // a stream of its own (recaptured for every tile change), with a zero H into a scratch F.
+    // (An earlier variant that captured run_jk on the context's stream crashed the host process
+    // once, cause not found; it is not used.)
+    const size_t n2 = (size_t)c->basis.nbf * c->basis.nbf;
+    double *dH = nullptr, *dF = nullptr;
+    hipStream_t ts = nullptr;
+    const bool graph = c->use_graph;
+    auto cleanup = [&]() {
+        if (ts) {
+            (void)hipStreamSynchronize(ts);
+            (void)hipStreamDestroy(ts);
+        }
+        c->release(dH);
+        c->release(dF);
+        c->drop_fock_graph();
+        c->use_graph = graph;
+        c->profiling = prof;
+    };
+    if (c->alloc(&dH, n2) != hipSuccess || c->alloc(&dF, n2) != hipSuccess ||
+        hipStreamCreateWithFlags(&ts, hipStreamNonBlocking) != hipSuccess ||
+        launch_fill(dH, n2, 0.0, ts) != hipSuccess) {
+        cleanup();
+        return fail(c, QCG_DEVICE, "autotune_refine: set-up");
+    }
+    c->use_graph = true;
     auto measure = [&]() -> int {
-        for (int w = 0; w < 2; ++w) {  // warm-up; the first also rebuilds the plan for new tiles
-            int r = run_jk(c, dD, c->d_J, c->d_K, c->stream);
+        for (int w = 0; w < 2; ++w) {  // warm-up; the first also captures the graph for new tiles
+            int r = qcg_build_fock_device(c, dH, dD, 0.5, dF, ts);
             if (r) return r;
         }
-        HIPCHK(c, hipEventRecord(c->ev[84], c->stream));
+        HIPCHK(c, hipEventRecord(c->ev[84], ts));
         for (int b = 0; b < builds; ++b) {
-            int r = run_jk(c, dD, c->d_J, c->d_K, c->stream);
+            int r = qcg_build_fock_device(c, dH, dD, 0.5, dF, ts);
             if (r) return r;
         }
-        HIPCHK(c, hipEventRecord(c->ev[85], c->stream));
+        HIPCHK(c, hipEventRecord(c->ev[85], ts));
         HIPCHK(c, hipEventSynchronize(c->ev[85]));
         float el = 0.f;
         HIPCHK(c, hipEventElapsedTime(&el, c->ev[84], c->ev[85]));
@@ -638,7 +662,7 @@ int qcg_autotune_refine(qcg_context* c, const double* dD, int builds, int* chose
             }
         }
     }
-    c->profiling = prof;
+    cleanup();
     if (rc) return rc;
     export_tiles(c, chosen);
     return QCG_OK;
