@@ -153,7 +153,8 @@ int qcg_autotune(qcg_context* ctx, const double* dD, int iters, int* chosen);
 /* No counterpart (launch configuration).  After qcg_autotune: greedy over the classes, longest
  * first, and over the same candidates, changing one class's FP64-part tile at a time and keeping
  * a change when `builds` whole builds (all classes, four streams) run at least 0.5 % faster in two
- * measurements.  chosen (optional) as for qcg_autotune. */
+ * measurements; in mixed precision it also tries each class's FP32 items in a launch of their own
+ * (with the FP32-part tile).  chosen (optional) as for qcg_autotune. */
 int qcg_autotune_refine(qcg_context* ctx, const double* dD, int builds, int* chosen);
 /* Set / read the J/K tiles directly (84 ints, same layout as qcg_autotune's chosen; classes in the
  * order (dd|dd), (dd|dp), ..., (ss|ss)).  No counterpart in qc-core: launch configuration only.

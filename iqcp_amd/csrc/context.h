@@ -74,8 +74,10 @@ struct qcg_context {
     struct Tile { int block = 64, reps = 1; };
     Tile tile[21][2];  // per quartet class and precision part (0 FP64, 1 FP32)
     int class_nq[21] = {};  // Cartesian components per quartet class
-    // mixed precision: run the FP64 and FP32 items of a class in one launch (jk_*_mp kernels)
-    bool fuse_mixed = true;
+    // mixed precision, per class: run its FP64 and FP32 items in one launch (jk_*_mp kernels, with
+    // the FP64 part's tile) or in two; qcg_autotune_refine may split a class
+    bool fuse_cls[21] = {true, true, true, true, true, true, true, true, true, true, true,
+                         true, true, true, true, true, true, true, true, true, true};
     // bra pairs with more primitive pairs than this are cut into primitive ranges (cut_items_split)
     int prim_split = 0;
     // launch schedule of the class kernels: measured time per class (qcg_autotune, 0 = unknown);

@@ -346,7 +346,7 @@ static int build_plan(qcg_context* c) {
         for (int ck = cb; ck >= 0; --ck, ++cls) {
             for (int part = 0; part < 2; ++part) {
                 // fused mixed launches run both parts with the FP64 part's tile
-                const qcg_context::Tile& t = c->tile[cls][(mixed && c->fuse_mixed) ? 0 : part];
+                const qcg_context::Tile& t = c->tile[cls][(mixed && c->fuse_cls[cls]) ? 0 : part];
                 const int ch = std::min(kItemMaxKets, std::max(1, t.block > 0 ? t.block * t.reps : t.reps));
                 c->item_off[cls][part] = (int)c->items.size();
                 if (part == 1 && !mixed) {
@@ -444,8 +444,8 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
             if (part == 1 && !mixed) continue;
             // mixed precision: both parts of the class in one launch (the plan cut the FP32
             // items with the FP64 part's tile and put them right after the FP64 items)
-            if (part == 1 && c->fuse_mixed) continue;
-            const bool fused = mixed && c->fuse_mixed;
+            if (part == 1 && c->fuse_cls[cls]) continue;
+            const bool fused = mixed && c->fuse_cls[cls];
             JkLaunch a{};
             a.bra = c->d_pairs + c->class_off[cb];
             a.ket = c->d_pairs + c->class_off[ck];
@@ -533,6 +533,7 @@ int qcg_autotune(qcg_context* c, const double* dD, int iters, int* chosen) {
     HIPCHK(c, hipSetDevice(c->device));
     const int ncand = kTuneNCand;
     auto tile_of = [&](int k, int cls) { return tune_tile(c, k, cls); };
+    for (int i = 0; i < 21; ++i) c->fuse_cls[i] = true;  // the per-class timings run fused launches
     std::vector<double> best(42, 1e300);
     std::vector<int> pick(42, 0);
     const bool prof = c->profiling;
@@ -659,6 +660,25 @@ int qcg_autotune_refine(qcg_context* c, const double* dD, int builds, int* chose
             } else {
                 c->tile[cls][0] = cur;
                 c->tile_sig += 1;
+            }
+        }
+        // mixed precision: the class's FP32 items in a launch of their own, with the FP32 part's
+        // tile, instead of the fused launch at the FP64 part's register count
+        if (!rc && c->precision == QCG_PREC_MIXED && c->item_cnt[cls][1] > 0) {
+            c->fuse_cls[cls] = false;
+            c->tile_sig += 1;
+            if (!(rc = measure())) {
+                double got = ms;
+                bool keep = got < 0.995 * best;
+                if (keep && !(rc = measure())) {
+                    keep = ms < 0.995 * best;
+                    got = std::min(got, ms);
+                }
+                if (!rc && keep) best = got;
+                else {
+                    c->fuse_cls[cls] = true;
+                    c->tile_sig += 1;
+                }
             }
         }
     }
