@@ -215,7 +215,8 @@ def main():
         log(f"cpu baseline: {cpu['value']} builds/s on {nthr} threads")
 
     # B3LYP leg: one J+K+XC build of (H2O)16/6-31G* on the literal 75 x 302 unpruned grid at its
-    # converged KS density, all-FP64 and mixed (the XC part is FP64 in both).
+    # converged KS density, all-FP64 and mixed (mixed: FP32 quartets below the cutoff and the XC
+    # grid products in FP32 on MFMA); J/K tiles tuned per precision as for decane.
     ks = None
     if not args.no_b3lyp:
         bw = workloads.water16_631gs()
@@ -229,6 +230,9 @@ def main():
               "nbf": bw.nbf, "grid_points": npts, "scf_energy": sw["energy_total"], "scf_iterations": sw["iterations"]}
         for prec in ("fp64", "mixed"):
             ew.set_options(prec, args.fp32_cutoff)
+            if not args.no_autotune:
+                ew.autotune(dDw.data_ptr(), 2)
+                ew.autotune_refine(dDw.data_ptr(), 10)
             with torch.cuda.stream(stream):
                 for _ in range(2):
                     ew.build_jk_device(dDw.data_ptr(), dJw.data_ptr(), dKw.data_ptr(), stream.cuda_stream)
