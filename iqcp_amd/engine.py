@@ -430,6 +430,10 @@ class BatchEngine:
     def get_tiles(self):
         return Engine.get_tiles(self)
 
+    def autotune(self, dD_ptr: int, iters: int = 2):
+        """Per-class tile choice on a packed device density (all molecules), as Engine.autotune."""
+        return Engine.autotune(self, dD_ptr, iters)
+
     def _check(self, rc):
         if rc:
             raise QcgError(rc, self.L.qcg_last_error(self.ctx).decode())

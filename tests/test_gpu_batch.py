@@ -35,6 +35,22 @@ def test_batch_jk_tiles_equal_single(tile):
         np.testing.assert_allclose(K, jk.K, atol=1e-12, rtol=0)
 
 
+def test_batch_autotune_keeps_parity():
+    """Tiles autotuned on a packed batch density: batched J/K still equal the single-molecule ones."""
+    import torch
+    bases = [bench_molecule(m, b) for m, b in (("NH3", "6-31+g*"), ("H2O", "6-31g*"), ("CH4", "6-31g"))]
+    be = BatchEngine(bases)
+    Ds = [random_density(b.nbf, seed=30 + i, nocc=b.n_occ) for i, b in enumerate(bases)]
+    dD = torch.tensor(be.pack(Ds), device=torch.device("cuda:0"))
+    tiles = be.autotune(dD.data_ptr(), 1)
+    assert len(tiles) == 21 and be.get_tiles() == [[tuple(t) for t in pair] for pair in tiles]
+    Js, Ks = be.build_jk(Ds)
+    for b, D, J, K in zip(bases, Ds, Js, Ks):
+        jk = Engine(b).build_jk(D)
+        np.testing.assert_allclose(J, jk.J, atol=1e-12, rtol=0)
+        np.testing.assert_allclose(K, jk.K, atol=1e-12, rtol=0)
+
+
 @pytest.mark.parametrize("use_graph", [True, False])
 def test_batch_scf_equals_single(use_graph):
     bases = [bench_molecule(m, b) for m, b in (("H2O", "6-31g*"), ("CH4", "6-31g"), ("HF", "6-31g"))]
