@@ -68,44 +68,46 @@ int qcg_batch_rhf_scf(qcg_context* c, const qcg_scf_config* cfg, int use_graph, 
     if (bad) return fail(c, QCG_INVALID_ARG, "batch: odd electron count or singular overlap");
     const size_t ntot = (size_t)c->mat_total;
     std::vector<double> Dp(ntot), Jp(ntot), Kp(ntot);
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // graph, exec and events are released on every return path below (GraphHold), after the
+    // context stream has drained
+    GraphHold gh;
+    gh.st = c->stream;
     if (use_graph) {
         int rc = prepare_jk(c);  // plan and streams exist before capture: run_jk then only enqueues
         if (rc) return rc;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         rc = run_jk(c, c->d_D, c->d_J, c->d_K, c->stream);
-        hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        hipError_t e = hipStreamEndCapture(c->stream, &gh.graph);
         if (rc) {
-            if (graph) (void)hipGraphDestroy(graph);
             (void)hipGetLastError();  // do not leave the capture error for the next call to find
             return rc;
         }
         HIPCHK(c, e);
-        HIPCHK(c, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        HIPCHK(c, hipGraphInstantiate(&gh.exec, gh.graph, nullptr, nullptr, 0));
     }
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    HIPCHK(c, hipEventCreate(&e0));
-    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventCreate(&gh.ev[0]));
+    HIPCHK(c, hipEventCreate(&gh.ev[1]));
     double jk_ms = 0.0;
     int rc = QCG_OK;
     for (;;) {
         for (int m = 0; m < nmol; ++m)
             std::memcpy(&Dp[c->mat_off_h[m]], runs[m].D.data(), runs[m].D.size() * sizeof(double));
         HIPCHK(c, hipMemcpyAsync(c->d_D, Dp.data(), ntot * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipEventRecord(e0, c->stream));
+        HIPCHK(c, hipEventRecord(gh.ev[0], c->stream));
         if (use_graph) {
-            HIPCHK(c, hipGraphLaunch(exec, c->stream));
+            HIPCHK(c, stream_begin(c, c->stream));
+            HIPCHK(c, hipGraphLaunch(gh.exec, c->stream));
+            HIPCHK(c, stream_done(c, c->stream));
         } else {
             rc = run_jk(c, c->d_D, c->d_J, c->d_K, c->stream);
             if (rc) break;
         }
-        HIPCHK(c, hipEventRecord(e1, c->stream));
+        HIPCHK(c, hipEventRecord(gh.ev[1], c->stream));
         HIPCHK(c, hipMemcpyAsync(Jp.data(), c->d_J, ntot * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipMemcpyAsync(Kp.data(), c->d_K, ntot * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         float ms = 0.f;
-        HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+        HIPCHK(c, hipEventElapsedTime(&ms, gh.ev[0], gh.ev[1]));
         jk_ms += ms;
         int active = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : active)
@@ -127,10 +129,6 @@ int qcg_batch_rhf_scf(qcg_context* c, const qcg_scf_config* cfg, int use_graph, 
         }
         if (active == 0) break;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
     if (rc) return rc;
     bool all = true;
     for (int m = 0; m < nmol; ++m) {

@@ -59,12 +59,22 @@ struct qcg_context {
     hipGraph_t fock_graph = nullptr;
     hipGraphExec_t fock_exec = nullptr;
     double fock_key[10] = {};
+    // ev_done (below) is recorded after every launch of the exec, so waiting on it before the
+    // exec is destroyed covers a launch still in flight (without touching the caller's stream,
+    // which may no longer exist)
     void drop_fock_graph() {
+        if (fock_exec && ev_done) (void)hipEventSynchronize(ev_done);
         if (fock_exec) (void)hipGraphExecDestroy(fock_exec);
         if (fock_graph) (void)hipGraphDestroy(fock_graph);
         fock_exec = nullptr;
         fock_graph = nullptr;
     }
+    // Stream order of the entry points that share the accumulators (d_Jacc, d_Kacc, d_grad) and
+    // the fork/join events: each one, on its stream st, first waits for the previous one's work
+    // (ev_done, recorded on last_stream) and records ev_done at its end, so calls issued on
+    // different streams of one context run one after the other on the device.
+    hipEvent_t ev_done = nullptr;
+    hipStream_t last_stream = nullptr;
     // per-class profiling of the J/K build: events around each class launch
     bool profiling = false;
     hipEvent_t ev[2 * 21 * 2 + 2] = {};
@@ -175,6 +185,50 @@ inline int fail(qcg_context* c, int code, const std::string& msg) {
     if (c) c->err = msg;
     return code;
 }
+
+// True when st is inside an open stream capture (then nothing may allocate, copy synchronously or
+// create runtime objects: such calls would end the capture with an error).
+inline bool capturing(hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+// Order the work about to be enqueued on st after the context's previous device work, then
+// (stream_done) mark its end.  No-ops inside a capture: a captured sequence is ordered when its
+// graph is launched.
+inline hipError_t stream_begin(qcg_context* c, hipStream_t st) {
+    if (capturing(st)) return hipSuccess;
+    if (!c->ev_done) {
+        hipError_t e = hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
+    if (c->last_stream && c->last_stream != st) return hipStreamWaitEvent(st, c->ev_done, 0);
+    return hipSuccess;
+}
+inline hipError_t stream_done(qcg_context* c, hipStream_t st) {
+    if (capturing(st) || !c->ev_done) return hipSuccess;
+    c->last_stream = st;
+    return hipEventRecord(c->ev_done, st);
+}
+
+// Graph, exec and timing events of one captured launch sequence, released on every exit path
+// (after the stream the exec was launched on has drained).
+struct GraphHold {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipStream_t st = nullptr;
+    GraphHold() = default;
+    GraphHold(const GraphHold&) = delete;
+    GraphHold& operator=(const GraphHold&) = delete;
+    ~GraphHold() {
+        if (st) (void)hipStreamSynchronize(st);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
 
 // Integral-direct J/K into Jacc/Kacc then J = Jacc + Jacc^T, K = Kacc + Kacc^T (dJ/dK may be null).
 int run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStream_t st);

@@ -60,6 +60,7 @@ int qcg::eri_grad_launch(qcg_context* c, const double* dD, double hf_frac, doubl
         rc = onee_setup(c);
         if (rc) return rc;
     }
+    HIPCHK(c, stream_begin(c, st));
     HIPCHK(c, hipEventRecord(c->ev_fork, st));
     for (int i = 0; i < qcg_context::kStreams; ++i) HIPCHK(c, hipStreamWaitEvent(c->aux[i], c->ev_fork, 0));
     // heaviest classes first: the d-bra classes with many components, by component count
@@ -107,6 +108,7 @@ int qcg::eri_grad_launch(qcg_context* c, const double* dD, double hf_frac, doubl
         HIPCHK(c, hipEventRecord(c->ev_join[i], c->aux[i]));
         HIPCHK(c, hipStreamWaitEvent(st, c->ev_join[i], 0));
     }
+    HIPCHK(c, stream_done(c, st));
     return QCG_OK;
 }
 

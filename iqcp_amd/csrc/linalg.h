@@ -1,12 +1,13 @@
 // Small dense FP64 linear algebra for the host SCF driver (row-major, n x n).
-//   sym_eigen : Householder tridiagonalisation + implicit QL with Wilkinson shifts; eigenvalues
-//               ascending, eigenvectors in columns (the role of nalgebra SymmetricEigen +
-//               sorted_eigen, scf/mod.rs:743-778).
+//   sym_eigen : Householder tridiagonalisation + implicit symmetric QR with Wilkinson shifts
+//               (Golub & Van Loan 8.3); eigenvalues ascending, eigenvectors in columns (the role
+//               of nalgebra SymmetricEigen + sorted_eigen, scf/mod.rs:743-778).
 //   lu_solve  : Gaussian elimination with partial pivoting; returns false on a zero pivot
 //               (nalgebra's LU solve returning None, used by DIIS, scf/mod.rs:1189-1262).
 #pragma once
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <numeric>
 #include <vector>
 
@@ -29,110 +30,108 @@ inline void matmul(const double* A, const double* B, double* C, int n, bool ta =
     std::copy(tmp.begin(), tmp.end(), C);
 }
 
-// A (symmetric, destroyed) -> w (ascending), V (columns = eigenvectors), row-major.
+// A (symmetric) -> w (ascending), V (columns = eigenvectors), row-major.
+// Golub & Van Loan, Matrix Computations (4th ed.), section 8.3: Householder reduction to
+// tridiagonal form (Algorithm 8.3.1) with the reflections accumulated into Q, then the implicit
+// symmetric QR step with a Wilkinson shift (Algorithm 8.3.2) on the unreduced trailing block
+// until every off-diagonal element is negligible.  The transformation is kept transposed (QT,
+// one eigenvector per row) so that every reflection and Givens rotation updates contiguous rows.
 inline void sym_eigen(std::vector<double> A, int n, std::vector<double>& w, std::vector<double>& V) {
-    // Householder reduction of A to tridiagonal form, accumulating Q in A (classic scheme).
-    std::vector<double> d(n), e(n);
     auto a = [&](int i, int j) -> double& { return A[(size_t)i * n + j]; };
-    for (int i = n - 1; i > 0; --i) {
-        const int l = i - 1;
-        double h = 0.0;
-        if (l > 0) {
-            double scale = 0.0;
-            for (int k = 0; k <= l; ++k) scale += std::fabs(a(i, k));
-            if (scale == 0.0) {
-                e[i] = a(i, l);
-            } else {
-                for (int k = 0; k <= l; ++k) {
-                    a(i, k) /= scale;
-                    h += a(i, k) * a(i, k);
-                }
-                double f = a(i, l);
-                double g = f >= 0.0 ? -std::sqrt(h) : std::sqrt(h);
-                e[i] = scale * g;
-                h -= f * g;
-                a(i, l) = f - g;
-                f = 0.0;
-                for (int j = 0; j <= l; ++j) {
-                    a(j, i) = a(i, j) / h;
-                    g = 0.0;
-                    for (int k = 0; k <= j; ++k) g += a(j, k) * a(i, k);
-                    for (int k = j + 1; k <= l; ++k) g += a(k, j) * a(i, k);
-                    e[j] = g / h;
-                    f += e[j] * a(i, j);
-                }
-                const double hh = f / (h + h);
-                for (int j = 0; j <= l; ++j) {
-                    f = a(i, j);
-                    e[j] = g = e[j] - hh * f;
-                    for (int k = 0; k <= j; ++k) a(j, k) -= (f * e[k] + g * a(i, k));
-                }
-            }
-        } else {
-            e[i] = a(i, l);
+    std::vector<double> QT((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) QT[(size_t)i * n + i] = 1.0;
+    std::vector<double> v(n), p(n), q(n), u(n);
+    // ---- tridiagonalisation: reflection k zeroes A[k+2.., k] and A[k, k+2..]
+    for (int k = 0; k + 2 < n; ++k) {
+        const int m = n - k - 1;  // length of x = A[k+1.., k]
+        double xn2 = 0.0;
+        for (int i = 1; i < m; ++i) xn2 += a(k + 1 + i, k) * a(k + 1 + i, k);
+        if (xn2 == 0.0) continue;  // already tridiagonal in this column
+        const double x0 = a(k + 1, k);
+        const double mu = std::sqrt(x0 * x0 + xn2);
+        // v = x - alpha e1 with alpha = -sign(x0) |x| (no cancellation); P = I - beta v v^T
+        const double alpha = x0 >= 0.0 ? -mu : mu;
+        v[0] = x0 - alpha;
+        for (int i = 1; i < m; ++i) v[i] = a(k + 1 + i, k);
+        const double beta = 1.0 / (mu * (mu + std::fabs(x0)));  // 2 / (v^T v)
+        // trailing block B = A[k+1.., k+1..]: B <- P B P = B - v w^T - w v^T,
+        // p = beta B v, w = p - (beta p^T v / 2) v
+        for (int i = 0; i < m; ++i) {
+            double s = 0.0;
+            const double* row = &A[(size_t)(k + 1 + i) * n + k + 1];
+            for (int j = 0; j < m; ++j) s += row[j] * v[j];
+            p[i] = beta * s;
         }
-        d[i] = h;
+        double pv = 0.0;
+        for (int i = 0; i < m; ++i) pv += p[i] * v[i];
+        const double half = 0.5 * beta * pv;
+        for (int i = 0; i < m; ++i) q[i] = p[i] - half * v[i];
+        for (int i = 0; i < m; ++i) {
+            double* row = &A[(size_t)(k + 1 + i) * n + k + 1];
+            for (int j = 0; j < m; ++j) row[j] -= v[i] * q[j] + q[i] * v[j];
+        }
+        a(k + 1, k) = a(k, k + 1) = alpha;
+        for (int i = 1; i < m; ++i) a(k + 1 + i, k) = a(k, k + 1 + i) = 0.0;
+        // QT <- P QT on rows k+1.. (QT holds (P_k ... P_0) so that QT A0 QT^T = T):
+        // u = beta v^T QT[k+1.., :], then QT[k+1+i, :] -= v_i u
+        std::fill(u.begin(), u.end(), 0.0);
+        for (int i = 0; i < m; ++i) {
+            const double* row = &QT[(size_t)(k + 1 + i) * n];
+            for (int j = 0; j < n; ++j) u[j] += v[i] * row[j];
+        }
+        for (int i = 0; i < m; ++i) {
+            double* row = &QT[(size_t)(k + 1 + i) * n];
+            const double f = beta * v[i];
+            for (int j = 0; j < n; ++j) row[j] -= f * u[j];
+        }
     }
-    d[0] = 0.0;
-    e[0] = 0.0;
-    for (int i = 0; i < n; ++i) {
-        const int l = i - 1;
-        if (d[i] != 0.0) {
-            for (int j = 0; j <= l; ++j) {
-                double g = 0.0;
-                for (int k = 0; k <= l; ++k) g += a(i, k) * a(k, j);
-                for (int k = 0; k <= l; ++k) a(k, j) -= g * a(k, i);
+    std::vector<double> d(n), e(n > 0 ? n : 1, 0.0);  // diagonal, e[i] couples i and i+1
+    for (int i = 0; i < n; ++i) d[i] = a(i, i);
+    for (int i = 0; i + 1 < n; ++i) e[i] = a(i + 1, i);
+    // ---- implicit symmetric QR with Wilkinson shifts
+    const double eps = std::numeric_limits<double>::epsilon();
+    int hi = n - 1;
+    int guard = 0;
+    while (hi > 0 && guard < 64 * n) {
+        // deflate negligible couplings at the bottom
+        if (std::fabs(e[hi - 1]) <= eps * (std::fabs(d[hi - 1]) + std::fabs(d[hi]))) {
+            e[hi - 1] = 0.0;
+            --hi;
+            continue;
+        }
+        int lo = hi - 1;  // start of the unreduced block ending at hi
+        while (lo > 0 && std::fabs(e[lo - 1]) > eps * (std::fabs(d[lo - 1]) + std::fabs(d[lo]))) --lo;
+        if (lo > 0) e[lo - 1] = 0.0;
+        ++guard;
+        // Wilkinson shift: the eigenvalue of the trailing 2x2 closer to d[hi]
+        const double dd = 0.5 * (d[hi - 1] - d[hi]);
+        const double eh = e[hi - 1];
+        const double r = std::hypot(dd, eh);
+        const double shift = d[hi] - eh * eh / (dd + (dd >= 0.0 ? r : -r));
+        // bulge chase with Givens rotations on rows/columns (k, k+1), k = lo .. hi-1
+        double x = d[lo] - shift, z = e[lo];
+        for (int k = lo; k < hi; ++k) {
+            const double h = std::hypot(x, z);
+            const double c = h == 0.0 ? 1.0 : x / h, s = h == 0.0 ? 0.0 : z / h;
+            if (k > lo) e[k - 1] = h;
+            // T <- G^T T G on the 2x2 block (k, k+1) and the bulge
+            const double dk = d[k], dk1 = d[k + 1], ek = e[k];
+            d[k] = c * c * dk + 2.0 * c * s * ek + s * s * dk1;
+            d[k + 1] = s * s * dk - 2.0 * c * s * ek + c * c * dk1;
+            e[k] = c * s * (dk1 - dk) + (c * c - s * s) * ek;
+            if (k + 1 < hi) {
+                x = e[k];
+                z = s * e[k + 1];
+                e[k + 1] *= c;
+            }
+            double* r0 = &QT[(size_t)k * n];
+            double* r1 = &QT[(size_t)(k + 1) * n];
+            for (int j = 0; j < n; ++j) {
+                const double u0 = r0[j], u1 = r1[j];
+                r0[j] = c * u0 + s * u1;
+                r1[j] = c * u1 - s * u0;
             }
         }
-        d[i] = a(i, i);
-        a(i, i) = 1.0;
-        for (int j = 0; j <= l; ++j) a(j, i) = a(i, j) = 0.0;
-    }
-    // Implicit QL on the tridiagonal (d, e) with the transformation accumulated into A.
-    for (int i = 1; i < n; ++i) e[i - 1] = e[i];
-    if (n > 0) e[n - 1] = 0.0;
-    for (int l = 0; l < n; ++l) {
-        int iter = 0;
-        int m;
-        do {
-            for (m = l; m < n - 1; ++m) {
-                const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
-                if (std::fabs(e[m]) <= 1e-16 * dd) break;
-            }
-            if (m != l) {
-                if (++iter > 60) break;
-                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-                double r = std::hypot(g, 1.0);
-                g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? std::fabs(r) : -std::fabs(r)));
-                double s = 1.0, c = 1.0, p = 0.0;
-                int i;
-                for (i = m - 1; i >= l; --i) {
-                    double f = s * e[i];
-                    const double b = c * e[i];
-                    e[i + 1] = (r = std::hypot(f, g));
-                    if (r == 0.0) {
-                        d[i + 1] -= p;
-                        e[m] = 0.0;
-                        break;
-                    }
-                    s = f / r;
-                    c = g / r;
-                    g = d[i + 1] - p;
-                    r = (d[i] - g) * s + 2.0 * c * b;
-                    d[i + 1] = g + (p = s * r);
-                    g = c * r - b;
-                    for (int k = 0; k < n; ++k) {
-                        f = a(k, i + 1);
-                        a(k, i + 1) = s * a(k, i) + c * f;
-                        a(k, i) = c * a(k, i) - s * f;
-                    }
-                }
-                if (r == 0.0 && i >= l) continue;
-                d[l] -= p;
-                e[l] = g;
-                e[m] = 0.0;
-            }
-        } while (m != l);
     }
     std::vector<int> idx(n);
     std::iota(idx.begin(), idx.end(), 0);
@@ -141,7 +140,8 @@ inline void sym_eigen(std::vector<double> A, int n, std::vector<double>& w, std:
     V.assign((size_t)n * n, 0.0);
     for (int c = 0; c < n; ++c) {
         w[c] = d[idx[c]];
-        for (int r = 0; r < n; ++r) V[(size_t)r * n + c] = a(r, idx[c]);
+        const double* row = &QT[(size_t)idx[c] * n];
+        for (int r = 0; r < n; ++r) V[(size_t)r * n + c] = row[r];
     }
 }
 

@@ -70,7 +70,11 @@ int qcg_create(qcg_context** out, int device) {
 void qcg_destroy(qcg_context* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
+    c->drop_fock_graph();  // waits for the exec's last launch
     (void)hipStreamSynchronize(c->stream);
+    if (c->ev_done) (void)hipEventSynchronize(c->ev_done);  // the last entry point's work on any stream
+    for (int i = 0; i < qcg_context::kStreams; ++i)
+        if (c->aux[i]) (void)hipStreamSynchronize(c->aux[i]);
     for (void* p : c->owned) (void)hipFree(p);
     c->owned.clear();
     if (c->ev_made)
@@ -82,7 +86,7 @@ void qcg_destroy(qcg_context* c) {
             (void)hipStreamDestroy(c->aux[i]);
         }
     }
-    c->drop_fock_graph();
+    if (c->ev_done) (void)hipEventDestroy(c->ev_done);
     if (c->blas) (void)rocblas_destroy_handle(c->blas);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -333,6 +337,11 @@ int qcg_screen_counts(qcg_context* c, int64_t* n_kept, int64_t* n_fp32) {
 
 }  // extern "C"
 
+static bool plan_fresh(const qcg_context* c) {
+    const double key[4] = {(double)c->precision, c->fp32_cutoff, c->schwarz_thr, (double)c->tile_sig};
+    return c->plan_sig == 1 && std::memcmp(key, c->plan_key, sizeof(key)) == 0 && c->ev_fork != nullptr;
+}
+
 static int build_plan(qcg_context* c) {
     const bool mixed = c->precision == QCG_PREC_MIXED && c->fp32_cutoff > c->schwarz_thr;
     const double key[4] = {(double)c->precision, c->fp32_cutoff, c->schwarz_thr, (double)c->tile_sig};
@@ -398,8 +407,17 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
     const int nbf = c->max_nbf;
     const size_t n2 = (size_t)c->mat_total;
     const int do_j = dJ != nullptr, do_k = dK != nullptr;
-    int rc = prepare_jk(c);
-    if (rc) return rc;
+    const bool in_capture = capturing(st);
+    if (in_capture) {
+        // a stale plan would free, allocate and copy synchronously inside the capture, which ends
+        // it with an error: callers prepare the plan (prepare_jk) before they begin a capture
+        if (!plan_fresh(c) || (c->profiling && !c->ev_made))
+            return fail(c, QCG_INVALID_ARG, "run_jk: work plan not prepared before the stream capture");
+    } else {
+        int rc = prepare_jk(c);
+        if (rc) return rc;
+    }
+    HIPCHK(c, stream_begin(c, st));
     if (do_j) HIPCHK(c, launch_fill(c->d_Jacc, n2, 0.0, st));
     if (do_k) HIPCHK(c, launch_fill(c->d_Kacc, n2, 0.0, st));
     const bool mixed = c->precision == QCG_PREC_MIXED && c->fp32_cutoff > c->schwarz_thr;
@@ -487,6 +505,7 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
         }
     if (c->nmol == 1) HIPCHK(c, launch_symmetrize(c->d_Jacc, c->d_Kacc, dJ, dK, nbf, st));
     else HIPCHK(c, launch_symmetrize_batch(c->d_Jacc, c->d_Kacc, dJ, dK, c->nmol, c->d_mat_off, c->d_mol_nbf, nbf, st));
+    HIPCHK(c, stream_done(c, st));
     return QCG_OK;
 }
 
@@ -590,20 +609,21 @@ int qcg_autotune_refine(qcg_context* c, const double* dD, int builds, int* chose
     double ms = 0.0;
     // Timed as the device Fock builds run: qcg_build_fock_device replaying its captured graph on
     // a stream of its own (recaptured for every tile change), with a zero H into a scratch F.
-    // (An earlier variant that captured run_jk on the context's stream crashed the host process
-    // once, cause not found; it is not used.)
+    // The work plan is rebuilt before each capture (qcg_build_fock_device calls prepare_jk before
+    // hipStreamBeginCapture); run_jk refuses to rebuild it inside a capture.
     const size_t n2 = (size_t)c->basis.nbf * c->basis.nbf;
     double *dH = nullptr, *dF = nullptr;
     hipStream_t ts = nullptr;
     const bool graph = c->use_graph;
     auto cleanup = [&]() {
+        c->drop_fock_graph();  // waits for its launches on ts, before ts goes away
         if (ts) {
             (void)hipStreamSynchronize(ts);
+            if (c->last_stream == ts) c->last_stream = nullptr;  // drained: nothing to order after
             (void)hipStreamDestroy(ts);
         }
         c->release(dH);
         c->release(dF);
-        c->drop_fock_graph();
         c->use_graph = graph;
         c->profiling = prof;
     };
@@ -818,12 +838,15 @@ int qcg_build_fock_device(qcg_context* c, const double* dH, const double* dD, do
             HIPCHK(c, hipGraphInstantiate(&c->fock_exec, g, nullptr, nullptr, 0));
             std::memcpy(c->fock_key, key, sizeof(key));
         }
+        HIPCHK(c, stream_begin(c, st));
         HIPCHK(c, hipGraphLaunch(c->fock_exec, st));
+        HIPCHK(c, stream_done(c, st));
         return QCG_OK;
     }
     int rc = run_jk(c, dD, c->d_J, c->d_K, st);
     if (rc) return rc;
     HIPCHK(c, launch_fock_combine(dH, c->d_Jacc, c->d_Kacc, dF, c->basis.nbf, k_factor, st));
+    HIPCHK(c, stream_done(c, st));
     return QCG_OK;
 }
 

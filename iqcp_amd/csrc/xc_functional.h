@@ -1,107 +1,124 @@
-// Closed-shell B3LYP (B3LYP5 = VWN5 variant) pointwise exchange-correlation, FP64, host/device.
+// Closed-shell B3LYP (the VWN5 variant, B3LYP5) at one point, FP64, host and device.
 //
-// Same formulas as qc-core's dft/functionals.rs: SlaterExchange::eval (:179), Vwn5Correlation::eval
-// (:324), Becke88Exchange::eval_gga (:691, spin-scaled, includes Slater), LypCorrelation::eval_gga
-// (:1170), B3lyp::eval_xc_gga (:1625+) with a = 0.20, b = 0.72, c = 0.81:
-//   exc   = 0.08 e_Slater + 0.72 e_B88 + 0.19 e_VWN5 + 0.81 e_LYP
-//   vrho  = same weights on the potentials, vsigma = 0.72 vs_B88 + 0.81 vs_LYP
-// rho <= 1e-20 gives zeros (DENSITY_THRESHOLD, :31).  exc is energy per electron.
+// Written from the published functionals, each derivative worked out for this file:
+//   Slater (Dirac) exchange, unpolarised:  f = -Cx rho^(4/3), Cx = (3/4)(3/pi)^(1/3)
+//   VWN5 correlation (Vosko, Wilk, Nusair, Can. J. Phys. 58, 1200 (1980), eq. 4.4 with the
+//     paramagnetic "5" parameters), in x = sqrt(rs):
+//       ec(x) = A [ ln(x^2/X) + (2b/Q) atn - (b x0/X0) ( ln((x-x0)^2/X) + (2(b+2x0)/Q) atn ) ]
+//     X = x^2 + b x + c, X0 = X(x0), Q = sqrt(4c - b^2), atn = atan(Q/(2x+b));  since
+//     (2x+b)^2 + Q^2 = 4X, d atn/dx = -Q/(2X), so every term of dec/dx is a ratio over X;
+//     v = ec - (x/6) dec/dx.
+//   Becke 88 exchange (A. D. Becke, Phys. Rev. A 38, 3098 (1988)) per spin channel with
+//     rho_s = rho/2, g_s = |grad rho_s|^2 = sigma/4, x_s = sqrt(g_s) rho_s^(-4/3):
+//       f = 2 [ -Cs rho_s^(4/3) - beta g_s rho_s^(-4/3) / d(x_s) ],  d = 1 + 6 beta x asinh(x),
+//     Cs = 2^(1/3) Cx; the gradient term written with g_s rho_s^(-4/3) = rho_s^(4/3) x_s^2 so the
+//     sigma derivative needs no division by |grad rho| (it is finite at sigma = 0).
+//   LYP correlation (Lee, Yang, Parr, Phys. Rev. B 37, 785 (1988)) in the closed form of
+//     Miehlich, Savin, Stoll, Preuss (Chem. Phys. Lett. 157, 200 (1989)), closed shell:
+//     with u = rho^(-1/3):
+//       f = -a rho/(1+du) - a b CF W + a b (3 + 7 delta)/72 Y sigma
+//     W = rho e^(-cu)/(1+du) (= omega rho^(14/3)), Y = e^(-cu) rho^(-5/3)/(1+du) (= omega rho^2),
+//     delta = c u + d u/(1+du); logarithmic derivatives: dlnW/drho = (1 + delta/3)/rho,
+//     dlnY/drho = (delta - 5)/(3 rho), ddelta/drho = -(u/(3 rho)) (c + d/(1+du)^2).
+//   B3LYP5 = 0.08 Slater + 0.72 B88 + 0.19 VWN5 + 0.81 LYP (+ 0.20 exact exchange elsewhere), the
+//   combination of qc-core's B3lyp::eval_xc_gga (dft/functionals.rs:1625).
+// Densities at or below 1e-20 give zeros (the reference's DENSITY_THRESHOLD, functionals.rs:31).
+// exc is the energy per electron; vrho = d(rho exc)/d rho, vsigma = d(rho exc)/d sigma.
 #pragma once
 #include "qcg_common.h"
 
 namespace qcg {
 
 constexpr double kXcRhoThreshold = 1e-20;
-constexpr double kSlaterCx = 0.7385587663820224;        // (3/4)(3/pi)^(1/3)
-constexpr double kSlater43Cx = 0.9847450218426965;      // (4/3) C_x
 constexpr double kB3lypA = 0.20, kB3lypB = 0.72, kB3lypC = 0.81;
+
+namespace xcf {
+constexpr double kCx = 0.73855876638202240588;        // (3/4)(3/pi)^(1/3)
+constexpr double kTwoThird = 1.25992104989487316477;  // 2^(1/3)
+// VWN5 paramagnetic parameters and the two derived constants
+constexpr double kVA = 0.0310907, kVx0 = -0.10498, kVb = 3.72744, kVc = 12.9352;
+constexpr double kVX0 = kVx0 * kVx0 + kVb * kVx0 + kVc;
+constexpr double kVQ = 6.15199081975908023217;  // sqrt(4c - b^2)
+// Becke 88
+constexpr double kBeta = 0.0042;
+// LYP
+constexpr double kLa = 0.04918, kLb = 0.132, kLc = 0.2533, kLd = 0.349;
+constexpr double kCF = 2.87123400018819181594;  // (3/10)(3 pi^2)^(2/3)
+}  // namespace xcf
 
 QCG_HD void xc_slater(double rho, double& e, double& v) {
     if (rho <= kXcRhoThreshold) { e = v = 0.0; return; }
-    const double r3 = cbrt(rho);
-    e = -kSlaterCx * r3;
-    v = -kSlater43Cx * r3;
+    e = -xcf::kCx * cbrt(rho);
+    v = (4.0 / 3.0) * e;
 }
 
 QCG_HD void xc_vwn5(double rho, double& e, double& v) {
-    constexpr double A = 0.0310907, X0 = -0.10498, B = 3.72744, C = 12.9352;
-    constexpr double Q = 6.15199081975908, XX0 = 12.5549141492;
+    using namespace xcf;
     if (rho <= kXcRhoThreshold) { e = v = 0.0; return; }
     const double rs = cbrt(3.0 / (4.0 * kPi * rho));
     const double x = sqrt(rs);
-    const double xs = x * x;
-    const double X = xs + B * x + C;
-    const double t2b = 2.0 * x + B;
-    const double at = atan(Q / t2b);
-    const double xm = x - X0;
-    const double ec = A * (log(xs / X) + 2.0 * B / Q * at -
-                           B * X0 / XX0 * (log(xm * xm / X) + 2.0 * (B + 2.0 * X0) / Q * at));
-    const double den = t2b * t2b + Q * Q;
-    const double dat = -2.0 * Q / den;
-    const double dedx = A * ((2.0 / x - t2b / X) + 2.0 * B / Q * dat -
-                             B * X0 / XX0 * (2.0 / xm - t2b / X + 2.0 * (B + 2.0 * X0) / Q * dat));
-    e = ec;
-    v = ec - (rs / 3.0) * (dedx / (2.0 * x));
+    const double X = x * x + kVb * x + kVc;
+    const double iX = 1.0 / X;
+    const double atn = atan(kVQ / (2.0 * x + kVb));
+    const double xm = x - kVx0;
+    const double k0 = kVb * kVx0 / kVX0;  // weight of the x0 bracket
+    const double k1 = 2.0 * kVb / kVQ, k2 = 2.0 * (kVb + 2.0 * kVx0) / kVQ;
+    e = kVA * (log(x * x * iX) + k1 * atn - k0 * (log(xm * xm * iX) + k2 * atn));
+    // d/dx: ln(x^2/X) -> 2/x - (2x+b)/X; (2b/Q) atn -> -b/X; ln((x-x0)^2/X) -> 2/(x-x0) - (2x+b)/X;
+    // (2(b+2x0)/Q) atn -> -(b+2x0)/X
+    const double dX = (2.0 * x + kVb) * iX;
+    const double de = kVA * ((2.0 / x - dX - kVb * iX) - k0 * (2.0 / xm - dX - (kVb + 2.0 * kVx0) * iX));
+    v = e - (x / 6.0) * de;
 }
 
 QCG_HD void xc_b88(double rho, double sigma, double& e, double& vr, double& vs) {
-    constexpr double beta = 0.0042;
+    using namespace xcf;
     if (rho <= kXcRhoThreshold) { e = vr = vs = 0.0; return; }
-    const double ra = 0.5 * rho;
-    const double sa = fmax(sigma, 0.0) * 0.25;
-    const double ra13 = cbrt(ra);
-    const double ra43 = ra * ra13;
-    const double two13 = 1.2599210498948731648;
-    const double es_a = -two13 * kSlaterCx * ra13;
-    const double g = sqrt(sa);
-    if (g < 1e-30) {
-        xc_slater(rho, e, vr);
-        vs = 0.0;
-        return;
-    }
-    const double x = g / ra43;
-    const double x2 = x * x;
+    const double rs = 0.5 * rho;                       // rho_s
+    const double gs = 0.25 * (sigma > 0.0 ? sigma : 0.0);  // |grad rho_s|^2
+    const double r13 = cbrt(rs);
+    const double r43 = rs * r13;
+    const double im43 = 1.0 / r43;                     // rho_s^(-4/3)
+    const double x = sqrt(gs) * im43;
     const double ash = asinh(x);
-    const double den = 1.0 + 6.0 * beta * x * ash;
-    const double de_a = -beta * ra13 * x2 / den;
-    e = es_a + de_a;
-    const double dden = 6.0 * beta * (ash + x / sqrt(x2 + 1.0));
-    const double dgdx = (2.0 * x * den - x2 * dden) / (den * den);
-    const double des = -two13 * kSlaterCx / (3.0 * ra13 * ra13);
-    const double dxdr = -4.0 / 3.0 * x / ra;
-    const double dde = -beta * (x2 / den / (3.0 * ra13 * ra13) + ra13 * dgdx * dxdr);
-    vr = e + rho * (des + dde) * 0.5;
-    const double dxds = 1.0 / (2.0 * g * ra43);
-    vs = rho * (-beta * ra13 * dgdx * dxds) * 0.25;
+    const double d = 1.0 + 6.0 * kBeta * x * ash;
+    const double dp = 6.0 * kBeta * (ash + x / sqrt(1.0 + x * x));  // d'(x)
+    const double id = 1.0 / d;
+    const double cs = kTwoThird * kCx;
+    const double grad_term = kBeta * gs * im43 * id;   // beta g_s rho_s^(-4/3) / d
+    // f = 2 (-cs rho_s^(4/3) - grad_term); exc = f / rho = f / (2 rho_s)
+    e = (-cs * r43 - grad_term) / rs;
+    // vrho = d f / d rho = d g / d rho_s with g = -cs rho_s^(4/3) - grad_term;
+    // d grad_term / d rho_s = -(4/3) grad_term (1 - x d'/d) / rho_s
+    vr = -(4.0 / 3.0) * cs * r13 + (4.0 / 3.0) * grad_term * (1.0 - x * dp * id) / rs;
+    // vsigma = d f / d sigma = (1/2) d g / d g_s;  d grad_term / d g_s = beta rho_s^(-4/3) (1 - x d'/(2d)) / d
+    vs = -0.5 * kBeta * im43 * id * (1.0 - 0.5 * x * dp * id);
 }
 
 QCG_HD void xc_lyp(double rho, double sigma, double& e, double& vr, double& vs) {
-    constexpr double A = 0.04918, B = 0.132, C = 0.2533, D = 0.349, CF = 2.8712340001881918;
+    using namespace xcf;
     if (rho <= kXcRhoThreshold) { e = vr = vs = 0.0; return; }
-    const double s = fmax(sigma, 0.0);
-    const double r13 = cbrt(rho), r23 = r13 * r13, rm13 = 1.0 / r13;
-    const double r2 = rho * rho, r3 = r2 * rho;
-    const double rm113 = 1.0 / (r3 * r23);
-    const double drm = D * rm13;
-    const double inv = 1.0 / (1.0 + drm);
-    const double crm = C * rm13;
-    const double om = exp(-crm) * inv * rm113;
-    const double del = crm + drm * inv;
-    const double t1 = -A * rho * inv;
-    const double r143 = r2 * r2 * r23;
-    const double t2 = -A * B * CF * om * r143;
-    const double bc = A * B * om * r2 * (7.0 * del + 3.0) / 72.0;
-    e = (t1 + t2 + bc * s) / rho;
-    vs = bc;
-    const double h = 1.0 + drm;
-    const double dt1 = -A * (1.0 + 4.0 / 3.0 * drm) / (h * h);
-    const double dom = om * (rm13 / (3.0 * rho) * (C + D * inv) - 11.0 / (3.0 * rho));
-    const double r113 = r3 * r23;
-    const double dt2 = -A * B * CF * (dom * r143 + om * 14.0 / 3.0 * r113);
-    const double r43 = rho * r13;
-    const double ddel = -C * rm13 / (3.0 * rho) + D * inv / (3.0 * r43) * (-1.0 + drm * inv);
-    const double dq = ((dom * r2 + om * 2.0 * rho) * (7.0 * del + 3.0) + om * r2 * 7.0 * ddel) / 72.0;
-    vr = dt1 + dt2 + A * B * dq * s;
+    const double s = sigma > 0.0 ? sigma : 0.0;
+    const double u = 1.0 / cbrt(rho);          // rho^(-1/3)
+    const double h = 1.0 + kLd * u;
+    const double ih = 1.0 / h;
+    const double ex = exp(-kLc * u);
+    const double delta = kLc * u + kLd * u * ih;
+    const double ir = 1.0 / rho;
+    const double W = rho * ex * ih;            // omega rho^(14/3)
+    const double Y = ex * ih * u * u * u * u * u;  // omega rho^2 = e^(-cu) rho^(-5/3) / h
+    const double f1 = -kLa * rho * ih;
+    const double f2 = -kLa * kLb * kCF * W;
+    const double gq = kLa * kLb * (3.0 + 7.0 * delta) / 72.0;  // sigma coefficient / Y
+    const double f3 = gq * Y * s;
+    e = (f1 + f2 + f3) * ir;
+    vs = gq * Y;
+    // d f1 / d rho = -a (1 + (4/3) d u) / h^2
+    const double df1 = -kLa * (1.0 + (4.0 / 3.0) * kLd * u) * ih * ih;
+    const double df2 = f2 * (1.0 + delta / 3.0) * ir;
+    const double ddelta = -(u * ir / 3.0) * (kLc + kLd * ih * ih);
+    const double df3 = s * (gq * Y * (delta - 5.0) * ir / 3.0 + kLa * kLb * (7.0 / 72.0) * Y * ddelta);
+    vr = df1 + df2 + df3;
 }
 
 QCG_HD void xc_b3lyp(double rho, double sigma, double& exc, double& vrho, double& vsigma) {

@@ -223,3 +223,32 @@ def xc_component(which, rho, sigma):
     exc, vr, vs = np.zeros_like(rho), np.zeros_like(rho), np.zeros_like(rho)
     lib().or_xc_component(k, ctypes.c_int64(rho.size), _d(rho), _d(sigma), _d(exc), _d(vr), _d(vs))
     return exc, vr, vs
+
+
+def jk_direct(b, D, thr=1e-12, nthreads=None):
+    """J, K from the unique Schwarz-surviving shell quartets without the stored tensor
+    (or_jk_direct); returns (J, K, kept unique shell quartets)."""
+    n = b.nbf
+    J, K = np.zeros((n, n)), np.zeros((n, n))
+    L = lib()
+    L.or_jk_direct.restype = ctypes.c_int64
+    nthr = int(nthreads or min(16, os.cpu_count() or 1))
+    kept = L.or_jk_direct(*_basis_args(b), _d(np.ascontiguousarray(D, dtype=np.float64)), ctypes.c_double(thr),
+                          nthr, _d(J), _d(K))
+    return J, K, int(kept)
+
+
+def screen_counts(b, thr=1e-12, cut=None):
+    """Unique shell quartets (pair-of-pairs, 8-fold) kept by the reference's Schwarz predicate
+    Q_ij * Q_kl >= thr (integrals/eri/mod.rs:1008-1023), evaluated as that exact product, and
+    (cut given) how many of those have Q_ij * Q_kl < cut."""
+    Q = schwarz(b)
+    q = Q[np.tril_indices(b.nshell)]
+    kept = low = 0
+    for i in range(len(q)):
+        prod = q[i] * q[: i + 1]
+        keep = prod >= thr
+        kept += int(np.count_nonzero(keep))
+        if cut is not None:
+            low += int(np.count_nonzero(keep & (prod < cut)))
+    return kept if cut is None else (kept, low)

@@ -16,6 +16,7 @@
 //   or_eri_compressed  integrals/eri/mod.rs:992-1070                          pinned: PySCF STO-3G tensors
 //   or_build_fock      scf/mod.rs:825-881                                     pinned: H2 PySCF J/K/F (scf/fock.rs:192-229) + numpy einsum
 //   or_build_jk        dft/ks_scf.rs:1306 / :1342 / :1369                     pinned: numpy einsum on the full tensor
+//   or_jk_direct       eri_compressed + build_jk without the stored tensor     pinned: equals or_build_jk on or_eri_compressed
 //   or_one_electron    integrals/overlap.rs:647, kinetic.rs, nuclear.rs:226   pinned: PySCF S and h_core of the STO-3G presets
 //                      (binomial Gaussian moments; V by Rys over or_rys_roots)  (content/presets), closed-form s-s values
 #include <cmath>
@@ -633,6 +634,80 @@ int or_build_jk(const double* D, const double* eri, int nbf, double* J, double* 
 }  // extern "C"
 
 extern "C" {
+// J and K straight from the unique shell quartets, without the stored tensor: the integrals of
+// or_eri_compressed (same shell_eri, same Schwarz predicate Q_ij Q_kl >= thr,
+// integrals/eri/mod.rs:1008-1023) contracted as build_jk does (dft/ks_scf.rs:1369-1394),
+// J_pq = sum_rs (pq|rs) D_rs and K_pr = sum_qs (pq|rs) D_qs over the full index range.  Each
+// unique basis-function quartet (i >= j, k >= l, pair(ij) >= pair(kl)) is computed once and
+// expanded over its distinct index permutations, so every element of the full tensor enters
+// exactly once.  Shell quartets run on nthreads OpenMP threads with private J/K, summed at the
+// end.  Used where the compressed tensor would not fit a test (decane 6-31G*: 1.43 GB);
+// pinned against or_build_jk on the stored tensor.  Returns the number of unique shell quartets
+// kept.
+int64_t or_jk_direct(int nshell, const int* l, const int* nprim, const int* prim_off, const double* exps,
+                     const double* coefs, const double* centers, const int* bf_off, int nbf, const double* D,
+                     double thr, int nthreads, double* J, double* K) {
+    OrBasis B = make_basis(nshell, l, nprim, prim_off, exps, coefs, centers, bf_off, nbf);
+    std::vector<double> Q((size_t)nshell * nshell);
+    or_schwarz(nshell, l, nprim, prim_off, exps, coefs, centers, bf_off, nbf, Q.data());
+    // unique shell pairs (si >= sj) in pair-index order
+    std::vector<int> ps, qs;
+    for (int si = 0; si < nshell; ++si)
+        for (int sj = 0; sj <= si; ++sj) {
+            ps.push_back(si);
+            qs.push_back(sj);
+        }
+    const int np = (int)ps.size();
+    const size_t n2 = (size_t)nbf * nbf;
+    for (size_t i = 0; i < n2; ++i) J[i] = K[i] = 0.0;
+    int64_t kept = 0;
+#pragma omp parallel num_threads(nthreads) reduction(+ : kept)
+    {
+        std::vector<double> Jt(n2, 0.0), Kt(n2, 0.0), blk(1296);
+#pragma omp for schedule(dynamic, 1)
+        for (int P = np - 1; P >= 0; --P) {
+            const int si = ps[P], sj = qs[P];
+            for (int Qp = 0; Qp <= P; ++Qp) {
+                const int sk = ps[Qp], sl = qs[Qp];
+                if (Q[si * nshell + sj] * Q[sk * nshell + sl] < thr) continue;
+                ++kept;
+                shell_eri_impl(B, si, sj, sk, sl, blk.data());
+                const int ni = ncart(l[si]), nj = ncart(l[sj]), nk = ncart(l[sk]), nl = ncart(l[sl]);
+                for (int a = 0; a < ni; ++a)
+                    for (int b = 0; b < nj; ++b) {
+                        const int i = bf_off[si] + a, j = bf_off[sj] + b;
+                        if (si == sj && i < j) continue;
+                        for (int c = 0; c < nk; ++c)
+                            for (int d = 0; d < nl; ++d) {
+                                const int k = bf_off[sk] + c, ll = bf_off[sl] + d;
+                                if (sk == sl && k < ll) continue;
+                                if (P == Qp && pidx(i, j) < pidx(k, ll)) continue;
+                                const double v = blk[((a * nj + b) * nk + c) * nl + d];
+                                int perm[8][4] = {{i, j, k, ll}, {j, i, k, ll}, {i, j, ll, k}, {j, i, ll, k},
+                                                  {k, ll, i, j}, {ll, k, i, j}, {k, ll, j, i}, {ll, k, j, i}};
+                                for (int t = 0; t < 8; ++t) {
+                                    bool dup = false;
+                                    for (int u = 0; u < t && !dup; ++u)
+                                        dup = perm[u][0] == perm[t][0] && perm[u][1] == perm[t][1] &&
+                                              perm[u][2] == perm[t][2] && perm[u][3] == perm[t][3];
+                                    if (dup) continue;
+                                    const int p = perm[t][0], q = perm[t][1], r = perm[t][2], s = perm[t][3];
+                                    Jt[(size_t)p * nbf + q] += D[(size_t)r * nbf + s] * v;
+                                    Kt[(size_t)p * nbf + r] += D[(size_t)q * nbf + s] * v;
+                                }
+                            }
+                    }
+            }
+        }
+#pragma omp critical
+        for (size_t i = 0; i < n2; ++i) {
+            J[i] += Jt[i];
+            K[i] += Kt[i];
+        }
+    }
+    return kept;
+}
+
 // build_fock (scf/mod.rs:825-881) with the mu loop spread over nthreads OpenMP threads (the
 // (mu, nu) elements are independent, so the result is identical to or_build_fock).  Used as
 // the timed CPU baseline.

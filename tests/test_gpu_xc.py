@@ -29,6 +29,17 @@ def test_gpu_b3lyp_pointwise():
         np.testing.assert_allclose(a, c, rtol=1e-13, atol=1e-300)
 
 
+def test_gpu_b3lyp_pointwise_vs_mpmath():
+    """The device functional (written from the published forms) against 40-digit mpmath values
+    of the same forms (tests/golden/make_b3lyp_mpmath.py); the oracle restates the reference's
+    code, so this and the libxc vectors are the checks independent of both."""
+    d = json.loads((GOLDEN / "b3lyp_mpmath.json").read_text())
+    eng = Engine(bench_molecule("H2", "sto-3g"))
+    got = eng.xc_eval(d["rho"], d["sigma"])
+    for k, a in zip(("exc", "vrho", "vsigma"), got):
+        np.testing.assert_allclose(a, d[k], rtol=1e-13, atol=0, err_msg=k)
+
+
 @pytest.mark.parametrize("mol,bas,pruning", [("H2O", "6-31g*", 1), ("CH4", "6-31g*", 0), ("NH3", "6-31+g*", 1)])
 def test_gpu_grid_matches_oracle(mol, bas, pruning):
     b = bench_molecule(mol, bas)

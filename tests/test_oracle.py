@@ -241,3 +241,22 @@ def test_build_fock_vs_einsum():
     np.testing.assert_allclose(Ko, K, atol=1e-12)
     F = oracle.build_fock(H, D, eri, b.nbf)
     np.testing.assert_allclose(F, H + J - 0.5 * K, atol=1e-12)
+
+
+@pytest.mark.parametrize("mol,bas", [("H2O", "6-31g*"), ("NH3", "6-31+g*")])
+def test_jk_direct_equals_build_jk_on_stored_tensor(mol, bas):
+    """or_jk_direct (no stored tensor; used at the decane size) against or_build_jk over
+    or_eri_compressed, and its kept-quartet count against the exact-product count."""
+    from helpers import random_density
+    b = bench_molecule(mol, bas)
+    D = random_density(b.nbf, seed=3)
+    eri, _ = oracle.eri_compressed(b)
+    Jo, Ko = oracle.build_jk(D, eri, b.nbf)
+    J, K, kept = oracle.jk_direct(b, D, nthreads=4)
+    np.testing.assert_allclose(J, Jo, atol=1e-13 * max(1.0, np.abs(Jo).max()), rtol=0)
+    np.testing.assert_allclose(K, Ko, atol=1e-13 * max(1.0, np.abs(Ko).max()), rtol=0)
+    assert kept == oracle.screen_counts(b)
+    # a looser threshold keeps fewer quartets, and the count is what the direct build used
+    J2, K2, kept2 = oracle.jk_direct(b, D, thr=1e-3, nthreads=4)
+    assert kept2 == oracle.screen_counts(b, 1e-3) < kept
+    assert kept2 == kept - oracle.screen_counts(b, 1e-12, 1e-3)[1]

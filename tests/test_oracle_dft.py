@@ -29,6 +29,15 @@ def test_b3lyp5_vs_libxc():
     np.testing.assert_allclose(vs, ref["vsigma"], atol=ref["tol"], rtol=0)
 
 
+def test_b3lyp5_vs_mpmath():
+    """40-digit mpmath evaluation of the published forms, derivatives by mpmath differentiation
+    (tests/golden/make_b3lyp_mpmath.py): independent of both floating-point implementations."""
+    d = json.loads((GOLDEN / "b3lyp_mpmath.json").read_text())
+    got = oracle.b3lyp(np.array(d["rho"]), np.array(d["sigma"]))
+    for k, a in zip(("exc", "vrho", "vsigma"), got):
+        np.testing.assert_allclose(a, d[k], rtol=1e-13, atol=0, err_msg=k)
+
+
 def test_b3lyp_derivatives_by_finite_difference():
     """vrho = d(rho exc)/d rho, vsigma = d(rho exc)/d sigma (independent of the libxc vectors)."""
     rng = np.random.default_rng(1)
