@@ -212,20 +212,42 @@ def main():
         import oracle
         eng.set_options("fp64")
         eri = eng.eri_compressed()  # the tensor qc-core's build_fock reads (eri_compressed, eri/mod.rs:992)
-        nthr = min(16, os.cpu_count() or 1)
-        n_done, t = 0, time.perf_counter()
-        Fc = None
-        while n_done == 0 or time.perf_counter() - t < args.cpu_seconds:
-            Fc = oracle.build_fock_omp(H, D, eri, b.nbf, nthr)
-            n_done += 1
-        el = time.perf_counter() - t
+        # all the host threads this process may use (16 on the GPU box), and one thread (the
+        # reference's published protocol is single-threaded, BENCHMARKS.md:23-36); half the
+        # time budget each
+        try:
+            avail = len(os.sched_getaffinity(0))
+        except AttributeError:
+            avail = os.cpu_count() or 1
+        nthr = max(1, min(16, avail))
+
+        def cpu_rate(threads, seconds):
+            n_done, t0_ = 0, time.perf_counter()
+            Fc_ = None
+            while n_done == 0 or time.perf_counter() - t0_ < seconds:
+                Fc_ = oracle.build_fock_omp(H, D, eri, b.nbf, threads)
+                n_done += 1
+            return n_done, time.perf_counter() - t0_, Fc_
+
+        n_all, el, Fc = cpu_rate(nthr, 0.5 * args.cpu_seconds)
+        n_one, el1, _ = cpu_rate(1, 0.5 * args.cpu_seconds)
         del eri
-        cpu = {"value": round(n_done / el, 4), "unit": "Fock builds/s", "cores": nthr, "kind": "port",
-               "sample": f"{n_done} build_fock call(s) (scf/mod.rs:825 restated in oracle/) over the "
+        model = "unknown"
+        try:
+            for line in Path("/proc/cpuinfo").read_text().splitlines():
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+        cpu = {"value": round(n_all / el, 4), "unit": "Fock builds/s", "cores": nthr, "kind": "port",
+               "value_1_thread": round(n_one / el1, 4), "cpu_model": model,
+               "sample": f"{n_all} build_fock call(s) on {nthr} threads and {n_one} on 1 thread "
+                         f"(scf/mod.rs:825 restated in oracle/) over the "
                          f"{b.nbf * (b.nbf + 1) // 2 * (b.nbf * (b.nbf + 1) // 2 + 1) // 2}-element compressed "
-                         f"ERI tensor of the same decane geometry at the same density, {el:.1f}s",
+                         f"ERI tensor of the same decane geometry at the same density, {el:.1f}s + {el1:.1f}s",
                "max_abs_F_vs_gpu_fp64": float(np.abs(Fc - F64).max())}
-        log(f"cpu baseline: {cpu['value']} builds/s on {nthr} threads")
+        log(f"cpu baseline: {cpu['value']} builds/s on {nthr} threads, {cpu['value_1_thread']} on 1 thread ({model})")
 
     # B3LYP leg: one J+K+XC build of (H2O)16/6-31G* on the literal 75 x 302 unpruned grid at its
     # converged KS density, all-FP64 and mixed (mixed: FP32 quartets below the cutoff and the XC

@@ -64,6 +64,24 @@ typedef struct qcg_scf_result {
     double energy_1e;    /* KS only: Tr DH */
 } qcg_scf_result;
 
+/* One SCF iteration, ScfIteration (scf/mod.rs:627-650): iteration 0 is the guess density's build
+ * (delta_e and rms_density_change NaN there, the reference's None); diis_applied: the density of
+ * this iteration came from a DIIS-extrapolated Fock matrix.  Two fields have no counterpart in
+ * qc-core, which builds in FP64 only: fp32_build = 1 when this iteration's Fock build ran mixed
+ * precision, and error_bound = qcg_error_bound's bound on that build's two-electron energy error
+ * (0 for FP64 builds). */
+typedef struct qcg_scf_iteration {
+    int iteration;
+    int converged;
+    int diis_applied;
+    int fp32_build;
+    double energy_total;
+    double energy_electronic;
+    double delta_e;
+    double rms_density_change;
+    double error_bound;
+} qcg_scf_iteration;
+
 /* ---- context ---------------------------------------------------------------------------- */
 int qcg_create(qcg_context** ctx, int device);
 void qcg_destroy(qcg_context* ctx);
@@ -108,6 +126,20 @@ int qcg_schwarz_bounds(qcg_context* ctx, double* Q);
 /* Integer screening outputs: unique shell quartets (8-fold) with Q_ij*Q_kl >= threshold, and how
  * many of those run in FP32 under QCG_PREC_MIXED. */
 int qcg_screen_counts(qcg_context* ctx, int64_t* n_kept, int64_t* n_fp32);
+/* No counterpart in qc-core (it computes every quartet in FP64): an a-priori bound on the energy
+ * error of a mixed-precision J/K build at density D against the all-FP64 build of the same
+ * quartets,
+ *   |E2(mixed) - E2(FP64)| <= eps32 * sum over FP32 quartets (ab|cd) of
+ *                             Q_ab Q_cd deg [4 S_ab S_cd + hf_frac (S_ac S_bd + S_ad S_bc)],
+ * E2 = 1/2 Tr D (J - hf_frac/2 K), Q the Schwarz bounds (|(ij|kl)| <= Q_ab Q_cd per component),
+ * S_xy the sum of |D| over the shell block (x, y), deg the 8-fold symmetry weight of the quartet,
+ * and eps32 = 2^-16 the relative error allowed per FP32 quartet (a single quartet measures below
+ * 2^-20).  *dE_bound = 0 in FP64 mode.  Single molecules only (QCG_INVALID_ARG after
+ * qcg_set_basis_batch). */
+int qcg_error_bound(qcg_context* ctx, const double* D, double hf_frac, double* dE_bound);
+/* The per-iteration trace of the context's last qcg_rhf_scf / qcg_ks_scf (ScfOutput.trace,
+ * scf/mod.rs:652-700): copies min(cap, length) entries to out, *n = the trace length. */
+int qcg_scf_history(const qcg_context* ctx, qcg_scf_iteration* out, int cap, int* n);
 /* eri_compressed (integrals/eri/mod.rs:992): out has npair(npair+1)/2 doubles, npair = nbf(nbf+1)/2,
  * index eri_index (scf/mod.rs:608-613). */
 int qcg_eri_compressed(qcg_context* ctx, double* out);

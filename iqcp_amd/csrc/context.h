@@ -50,7 +50,9 @@ struct qcg_context {
     int* d_ket_cnt = nullptr;
     int64_t mat_total = 0;
     // work buffers
-    double *d_D = nullptr, *d_J = nullptr, *d_K = nullptr, *d_Jacc = nullptr, *d_Kacc = nullptr;
+    double *d_D = nullptr, *d_J = nullptr, *d_K = nullptr;
+    // J and K accumulators before symmetrisation, fixed-point (fixacc.h): 2 words per element
+    fix_t *d_Jacc = nullptr, *d_Kacc = nullptr;
     double *d_H = nullptr, *d_F = nullptr;
     std::vector<void*> owned;
     std::vector<PrimPair<double>> h_pp;  // host copy of the primitive pairs (flop model counts)
@@ -75,6 +77,8 @@ struct qcg_context {
     // different streams of one context run one after the other on the device.
     hipEvent_t ev_done = nullptr;
     hipStream_t last_stream = nullptr;
+    // per-iteration trace of the last qcg_rhf_scf / qcg_ks_scf (qcg_scf_history)
+    std::vector<qcg_scf_iteration> scf_history;
     // per-class profiling of the J/K build: events around each class launch
     bool profiling = false;
     hipEvent_t ev[2 * 21 * 2 + 2] = {};
@@ -111,6 +115,7 @@ struct qcg_context {
     std::vector<int4> items;  // J/K work items over the ket segments segs (cut_items)
     std::vector<int2> segs;
     int item_off[21][2] = {}, item_cnt[21][2] = {};
+    int bra_pp[kNumPairClasses] = {};  // per pair class: most primitive pairs of one item's bra
     int4* d_items = nullptr;
     int2* d_segs = nullptr;
     double plan_key[4] = {-1.0, -1.0, -1.0, -1.0};

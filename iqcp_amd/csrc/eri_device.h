@@ -161,12 +161,13 @@ QCG_HD void for_n(F&& body) {
 
 // Accumulate the contracted quartet (bra pair P | ket pair Q) into eri[NQ] (row-major
 // [a][b][c][d] over Cartesian components; the primitive-dependent normalisation is folded
-// into ck, the component normalisation is NOT applied here).  TAB_N: rys_tab already points at
-// the coefficients of this class's Rys order (e.g. a copy staged in LDS).
+// into ck, the component normalisation is NOT applied here).  ppb: the bra's primitive pairs
+// (P.npp of them, from index 0: a copy staged in LDS or pp + P.pp_off), ppk: the ket's (Q.npp).
+// TAB_N: rys_tab already points at the coefficients of this class's Rys order (e.g. a copy
+// staged in LDS).
 template <int LA, int LB, int LC, int LD, typename R, bool TAB_N = false>
-QCG_HD void contract_quartet(const PairRec& P, const PairRec& Q,
-                                                 const PrimPair<R>* __restrict__ pp,
-                                                 const R* __restrict__ rys_tab, R* eri) {
+QCG_HD void contract_quartet_pp(const PairRec& P, const PairRec& Q, const PrimPair<R>* __restrict__ ppb,
+                                const PrimPair<R>* __restrict__ ppk, const R* __restrict__ rys_tab, R* eri) {
     using S = EriShape<LA, LB, LC, LD>;
     constexpr int N = S::NROOTS;
     const R abx = (R)P.abx, aby = (R)P.aby, abz = (R)P.abz;
@@ -175,15 +176,16 @@ QCG_HD void contract_quartet(const PairRec& P, const PairRec& Q,
     const R Cx = (R)Q.ax, Cy = (R)Q.ay, Cz = (R)Q.az;
     const R two_pi_52 = (R)(2.0 * kPi * kPi * 1.7724538509055160273);
     // ket primitives outer: in the J/K kernels the bra pair is uniform over the block, so the inner
-    // loop's bra primitive loads are uniform (scalar) and each lane loads its ket primitive once
+    // loop's bra primitive reads are uniform (an LDS broadcast) and each lane loads its ket
+    // primitive once
     for (int j = 0; j < Q.npp; ++j) {
-        const PrimPair<R> c = pp[Q.pp_off + j];
+        const PrimPair<R> c = ppk[j];
         const R q = c.p;
         const R hq = c.hp;
         const R QCx = c.px - Cx, QCy = c.py - Cy, QCz = c.pz - Cz;
         for (int i = 0; i < P.npp; ++i) {
-            const PrimPair<R> a = pp[P.pp_off + i];
-            if (a.kab * c.kab < (R)kPrimScreen) continue;
+            const PrimPair<R> a = ppb[i];
+            if (prim_kab(a) * prim_kab(c) < kPrimScreen) continue;
             const R p = a.p;
             const R hp = a.hp;
             // 1/(p+q) = rs^2: one reciprocal square root per primitive quartet
@@ -231,6 +233,13 @@ QCG_HD void contract_quartet(const PairRec& P, const PairRec& Q,
             }
         }
     }
+}
+
+// contract_quartet_pp with both pairs' primitives read from the one array pp.
+template <int LA, int LB, int LC, int LD, typename R, bool TAB_N = false>
+QCG_HD void contract_quartet(const PairRec& P, const PairRec& Q, const PrimPair<R>* __restrict__ pp,
+                             const R* __restrict__ rys_tab, R* eri) {
+    contract_quartet_pp<LA, LB, LC, LD, R, TAB_N>(P, Q, pp + P.pp_off, pp + Q.pp_off, rys_tab, eri);
 }
 
 }  // namespace qcg

@@ -29,6 +29,14 @@
 #ifndef QCG_CB
 #error "compile with -DQCG_CB=<bra pair class>"
 #endif
+// Per-lane kernel options: the bra's primitive pairs staged in LDS, the D rows of the bra
+// functions staged in LDS next to the K rows.
+#ifndef QCG_JK_BRA_LDS
+#define QCG_JK_BRA_LDS 1
+#endif
+#ifndef QCG_JK_DROWS
+#define QCG_JK_DROWS 1
+#endif
 
 namespace qcg {
 
@@ -49,44 +57,69 @@ constexpr int rys_lds_doubles() { return N <= 3 ? rys_tab_size<N>() : 1; }
 
 // One work item = (bra pair, run of its surviving kets over the ket segments, KetCursor); the
 // host builds the items with the exact screening predicate, so no quartet is tested twice or
-// skipped.  rys_n: the
-// coefficients of this class's Rys order (rys_stage); kl: krow != 0: the four K terms of the item
-// (all in the rows of the bra functions a and b) are summed in LDS rows kl[a][*], kl[NA + b][*]
-// and flushed once per block.
+// skipped.  rys_n: the coefficients of this class's Rys order (rys_stage); bp: LDS room for the
+// bra's primitive pairs; rows: krow != 0: LDS room for the D rows of the bra functions a and b
+// ((NA + NB) x nbf doubles, read by the K digestion) followed by their K rows (fixed-point pairs,
+// fixacc.h) in which the four K terms of the item (all in the rows of a and b) are summed and then
+// flushed once per block.  J and K go to the fixed-point accumulators Jacc/Kacc, so the result does
+// not depend on the order in which blocks and lanes arrive.
 template <int LA, int LB, int LC, int LD, typename R>
 __device__ __forceinline__ void jk_item(
     const int4 it, const int2* __restrict__ segs, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket, int diag,
     const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_n, const double* __restrict__ D,
-    double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
-    const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf, int krow, double* kl) {
+    fix_t* __restrict__ Jacc, fix_t* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
+    const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf, int krow, PrimPair<R>* bp, double* rows) {
+#if !QCG_JK_BRA_LDS
+    (void)bp;
+#endif
     using S = EriShape<LA, LB, LC, LD>;
     constexpr bool U = S::kUnroll;
+    constexpr int NR = S::NA + S::NB;
     const PairRec P = item_bra(bra, it);
     const int pidx = P.pos;
     ket += ket_off[P.mol];
     D += mat_off[P.mol];
-    Jacc += mat_off[P.mol];
-    Kacc += mat_off[P.mol];
+    Jacc += 2 * mat_off[P.mol];
+    Kacc += 2 * mat_off[P.mol];
     const int nbf = mol_nbf[P.mol];
     const int tid = threadIdx.x;
-    const int nrow = krow ? (S::NA + S::NB) * nbf : 0;
-    for (int i = tid; i < nrow; i += blockDim.x) kl[i] = 0.0;
+#if QCG_JK_BRA_LDS
+    for (int i = tid; i < P.npp; i += blockDim.x) bp[i] = pp[P.pp_off + i];
+    const PrimPair<R>* ppb = bp;
+#else
+    const PrimPair<R>* ppb = pp + P.pp_off;
+#endif
+    // krow bit 0: K rows in LDS, bit 1: D rows in LDS after them
+    const bool kr = krow & 1, dr = krow & 2;
+    const int nrow = NR * nbf;
+    fix_t* kl = reinterpret_cast<fix_t*>(rows);
+    double* dl = rows + (kr ? 2 * nrow : 0);
+    for (int i = tid; i < nrow; i += blockDim.x) {
+        if (dr) {
+            const int r = i / nbf, col = i - r * nbf;
+            dl[i] = D[(size_t)(r < S::NA ? P.bfa + r : P.bfb + (r - S::NA)) * nbf + col];
+        }
+        if (kr) {
+            kl[2 * i] = 0;
+            kl[2 * i + 1] = 0;
+        }
+    }
     __syncthreads();
-    // LDS rows take a workgroup-scope atomic, HBM an agent-scope one: two distinct operations, so
-    // the compiler never merges them into one flat atomic on a selected pointer
+    // D element (row function a + r of the bra's shell a, or b + r of shell b; column col)
+    auto drow = [&](bool row_b, int r, int col) -> double {
+        if (dr) return dl[(row_b ? S::NA + r : r) * nbf + col];
+        return D[(size_t)((row_b ? P.bfb : P.bfa) + r) * nbf + col];
+    };
     auto kadd = [&](bool row_b, int r, int col, double v) {
-        if (krow)
-            __hip_atomic_fetch_add(&kl[(row_b ? S::NA + r : r) * nbf + col], v, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-        else
-            atomicAdd(&Kacc[(size_t)((row_b ? P.bfb : P.bfa) + r) * nbf + col], v);
+        if (kr) fix_add_lds(kl, (row_b ? S::NA + r : r) * nbf + col, v);
+        else fix_add_global(Kacc, (size_t)((row_b ? P.bfb : P.bfa) + r) * nbf + col, v);
     };
 
     // bra density block scaled by the bra component norms
     double Dps[S::NAB], jp[S::NAB];
     for_n<U, S::NAB>([&](int ab) {
         const int a = ab / S::NB, b = ab % S::NB;
-        Dps[ab] = D[(size_t)(P.bfa + a) * nbf + P.bfb + b] * cnorm(LA, a) * cnorm(LB, b);
+        Dps[ab] = drow(false, a, P.bfb + b) * cnorm(LA, a) * cnorm(LB, b);
         jp[ab] = 0.0;
     });
     const double deg_p = (P.sa == P.sb) ? 0.5 : 1.0;
@@ -97,7 +130,7 @@ __device__ __forceinline__ void jk_item(
         const PairRec Q = ket[q];
         R eri[S::NQ];
         for_n<U, S::NQ>([&](int i) { eri[i] = R(0); });
-        contract_quartet<LA, LB, LC, LD, R, true>(P, Q, pp, rys_n, eri);
+        contract_quartet_pp<LA, LB, LC, LD, R, true>(P, Q, ppb, pp + Q.pp_off, rys_n, eri);
         const double deg = deg_p * ((Q.sa == Q.sb) ? 0.5 : 1.0) * ((diag && q == pidx) ? 0.5 : 1.0);
         if (do_j) {
             double Dqs[S::NCD];
@@ -116,7 +149,7 @@ __device__ __forceinline__ void jk_item(
                 double s = 0.0;
                 for_n<U, S::NAB>([&](int ab) { s += Dps[ab] * (double)eri[ab * S::NCD + cd]; });
                 const int c = cd / S::ND, d = cd % S::ND;
-                atomicAdd(&Jacc[(size_t)(Q.bfa + c) * nbf + Q.bfb + d], fj * cnorm(LC, c) * cnorm(LD, d) * s);
+                fix_add_global(Jacc, (size_t)(Q.bfa + c) * nbf + Q.bfb + d, fj * cnorm(LC, c) * cnorm(LD, d) * s);
             });
         }
         if (do_k) {
@@ -125,19 +158,19 @@ __device__ __forceinline__ void jk_item(
             double Dbd[S::NB * S::ND], Dad[S::NA * S::ND], Dbc[S::NB * S::NC], Dac[S::NA * S::NC];
             for_n<U, S::NB * S::ND>([&](int i) {
                 const int b = i / S::ND, d = i % S::ND;
-                Dbd[i] = D[(size_t)(P.bfb + b) * nbf + Q.bfb + d] * cnorm(LB, b) * cnorm(LD, d);
+                Dbd[i] = drow(true, b, Q.bfb + d) * cnorm(LB, b) * cnorm(LD, d);
             });
             for_n<U, S::NA * S::ND>([&](int i) {
                 const int a = i / S::ND, d = i % S::ND;
-                Dad[i] = D[(size_t)(P.bfa + a) * nbf + Q.bfb + d] * cnorm(LA, a) * cnorm(LD, d);
+                Dad[i] = drow(false, a, Q.bfb + d) * cnorm(LA, a) * cnorm(LD, d);
             });
             for_n<U, S::NB * S::NC>([&](int i) {
                 const int b = i / S::NC, c = i % S::NC;
-                Dbc[i] = D[(size_t)(P.bfb + b) * nbf + Q.bfa + c] * cnorm(LB, b) * cnorm(LC, c);
+                Dbc[i] = drow(true, b, Q.bfa + c) * cnorm(LB, b) * cnorm(LC, c);
             });
             for_n<U, S::NA * S::NC>([&](int i) {
                 const int a = i / S::NC, c = i % S::NC;
-                Dac[i] = D[(size_t)(P.bfa + a) * nbf + Q.bfa + c] * cnorm(LA, a) * cnorm(LC, c);
+                Dac[i] = drow(false, a, Q.bfa + c) * cnorm(LA, a) * cnorm(LC, c);
             });
 #define VIDX(a, b, c, d) ((((a) * S::NB + (b)) * S::NC + (c)) * S::ND + (d))
             for_n<U, S::NA * S::NC>([&](int ac) {
@@ -179,7 +212,7 @@ __device__ __forceinline__ void jk_item(
 #undef VIDX
         }
     }
-    // J_P: wave reduction, one atomic per component per wave
+    // J_P: wave reduction (a fixed shuffle tree), one fixed-point add per component per wave
     if (do_j) {
         for_n<true, S::NAB>([&](int ab) {
             double s = jp[ab];
@@ -187,35 +220,57 @@ __device__ __forceinline__ void jk_item(
             for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
             if ((tid & 63) == 0 && s != 0.0) {
                 const int a = ab / S::NB, b = ab % S::NB;
-                atomicAdd(&Jacc[(size_t)(P.bfa + a) * nbf + P.bfb + b], s);
+                fix_add_global(Jacc, (size_t)(P.bfa + a) * nbf + P.bfb + b, s);
             }
         });
     }
-    if (krow) {
+    if (kr) {
         __syncthreads();
         for (int i = tid; i < nrow; i += blockDim.x) {
-            const double v = kl[i];
-            if (v != 0.0) {
-                const int r = i / nbf, col = i - r * nbf;
-                const int row = r < S::NA ? P.bfa + r : P.bfb + (r - S::NA);
-                atomicAdd(&Kacc[(size_t)row * nbf + col], v);
-            }
+            const int r = i / nbf, col = i - r * nbf;
+            const int row = r < S::NA ? P.bfa + r : P.bfb + (r - S::NA);
+            fix_add_pair_global(Kacc, (size_t)row * nbf + col, kl[2 * i], kl[2 * i + 1]);
         }
     }
+}
+
+// LDS layout of the per-lane kernels' rows for nbf functions (the largest molecule's): K rows
+// as fixed-point pairs, (NA + NB) x nbf x 16 bytes, when they fit the budget left by the static
+// LDS (Rys coefficients, bra primitives), then the D rows, (NA + NB) x nbf x 8 bytes, when those
+// fit too.  Returns the krow flags (bit 0 K rows, bit 1 D rows) and the dynamic LDS bytes.
+constexpr size_t kLdsBudget = 65536;
+template <int LA, int LB, int LC, int LD>
+inline int jk_row_flags(int nbf, int do_k, int bra_pp, size_t* bytes) {
+    using S = EriShape<LA, LB, LC, LD>;
+    const size_t fixed = (size_t)rys_lds_doubles<S::NROOTS>() * sizeof(double) + (size_t)bra_pp * sizeof(PrimPair<double>);
+    const size_t kb = (size_t)(S::NA + S::NB) * nbf * 2 * sizeof(fix_t), db = (size_t)(S::NA + S::NB) * nbf * sizeof(double);
+    int f = 0;
+    size_t n = 0;
+    if (do_k && fixed + kb <= kLdsBudget) {
+        f |= 1;
+        n += kb;
+    }
+    if (QCG_JK_DROWS && fixed + n + db <= kLdsBudget) {
+        f |= 2;
+        n += db;
+    }
+    *bytes = n + (size_t)bra_pp * sizeof(PrimPair<double>);
+    return f;
 }
 
 template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(256) void jk_kernel(
     const int4* __restrict__ items, const int2* __restrict__ segs, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket, int diag,
     const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_tab, const double* __restrict__ D,
-    double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
-    const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf, int krow) {
+    fix_t* __restrict__ Jacc, fix_t* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
+    const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf, int krow, int bra_pp) {
     constexpr int NR = EriShape<LA, LB, LC, LD>::NROOTS;
-    extern __shared__ double kl[];
+    extern __shared__ double dyn[];  // bra primitive pairs (bra_pp), then the K / D rows
     __shared__ double rys_raw[rys_lds_doubles<NR>()];
     const R* rys_n = rys_stage<R, NR>(rys_tab, reinterpret_cast<R*>(rys_raw));
     jk_item<LA, LB, LC, LD, R>(items[blockIdx.x], segs, bra, ket, diag, pp, rys_n, D, Jacc, Kacc, do_j, do_k, ket_off,
-                               mat_off, mol_nbf, krow, kl);
+                               mat_off, mol_nbf, krow, reinterpret_cast<PrimPair<R>*>(dyn),
+                               dyn + bra_pp * sizeof(PrimPair<R>) / sizeof(double));
 }
 
 // Mixed precision in one launch: items [0, n64) are FP64 quartets, the rest FP32 (one branch per
@@ -225,19 +280,20 @@ __global__ __launch_bounds__(256) void jk_kernel_mp(
     const int4* __restrict__ items, const int2* __restrict__ segs, int n64, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket,
     int diag, const PrimPair<double>* __restrict__ pp, const PrimPair<float>* __restrict__ ppf,
     const double* __restrict__ rys_tab, const float* __restrict__ rys_tabf, const double* __restrict__ D,
-    double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
-    const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf, int krow) {
+    fix_t* __restrict__ Jacc, fix_t* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
+    const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf, int krow, int bra_pp) {
     constexpr int NR = EriShape<LA, LB, LC, LD>::NROOTS;
-    extern __shared__ double kl[];
+    extern __shared__ double dyn[];  // bra primitive pairs (bra_pp FP64 slots), then the K / D rows
     __shared__ double rys_raw[rys_lds_doubles<NR>()];
+    double* rows = dyn + bra_pp * sizeof(PrimPair<double>) / sizeof(double);
     if ((int)blockIdx.x < n64) {
         const double* rys_n = rys_stage<double, NR>(rys_tab, rys_raw);
         jk_item<LA, LB, LC, LD, double>(items[blockIdx.x], segs, bra, ket, diag, pp, rys_n, D, Jacc, Kacc, do_j, do_k,
-                                        ket_off, mat_off, mol_nbf, krow, kl);
+                                        ket_off, mat_off, mol_nbf, krow, reinterpret_cast<PrimPair<double>*>(dyn), rows);
     } else {
         const float* rys_n = rys_stage<float, NR>(rys_tabf, reinterpret_cast<float*>(rys_raw));
         jk_item<LA, LB, LC, LD, float>(items[blockIdx.x], segs, bra, ket, diag, ppf, rys_n, D, Jacc, Kacc, do_j, do_k,
-                                       ket_off, mat_off, mol_nbf, krow, kl);
+                                       ket_off, mat_off, mol_nbf, krow, reinterpret_cast<PrimPair<float>*>(dyn), rows);
     }
 }
 
@@ -264,7 +320,7 @@ template <int LA, int LB, int LC, int LD, typename R>
 __device__ __forceinline__ void jk_wave_item(
     const int4 it, const int2* __restrict__ segs, int lane, double* V, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket,
     int diag, const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_tab, const double* __restrict__ D,
-    double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
+    fix_t* __restrict__ Jacc, fix_t* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
     const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf) {
     using S = EriShape<LA, LB, LC, LD>;
     using W = JkWave<LA, LB, LC, LD, R>;
@@ -283,8 +339,8 @@ __device__ __forceinline__ void jk_wave_item(
     const int pidx = P.pos;
     ket += ket_off[P.mol];
     D += mat_off[P.mol];
-    Jacc += mat_off[P.mol];
-    Kacc += mat_off[P.mol];
+    Jacc += 2 * mat_off[P.mol];
+    Kacc += 2 * mat_off[P.mol];
     const size_t nbf = (size_t)mol_nbf[P.mol];
 
     constexpr const CompTab<LA, LB, LC, LD>& ct = kCompTab<LA, LB, LC, LD>;
@@ -349,7 +405,7 @@ __device__ __forceinline__ void jk_wave_item(
             for (int o = lane; o < NCD; o += 64) {
                 double s = 0.0;
                 for (int ab = 0; ab < S::NAB; ++ab) s += Dab[ab] * V[ab * NCD + o];
-                atomicAdd(&Jacc[(Q.bfa + o / ND) * nbf + Q.bfb + o % ND], 2.0 * s);
+                fix_add_global(Jacc, (Q.bfa + o / ND) * nbf + Q.bfb + o % ND, 2.0 * s);
             }
         }
         if (do_k) {
@@ -358,40 +414,40 @@ __device__ __forceinline__ void jk_wave_item(
                 double s = 0.0;
                 for (int b = 0; b < NB; ++b)
                     for (int d = 0; d < ND; ++d) s += Dbd[b * ND + d] * VAT(a, b, c, d);
-                atomicAdd(&Kacc[(P.bfa + a) * nbf + Q.bfa + c], s);
+                fix_add_global(Kacc, (P.bfa + a) * nbf + Q.bfa + c, s);
             }
             for (int o = lane; o < NB * NC; o += 64) {
                 const int b = o / NC, c = o % NC;
                 double s = 0.0;
                 for (int a = 0; a < NA; ++a)
                     for (int d = 0; d < ND; ++d) s += Dad[a * ND + d] * VAT(a, b, c, d);
-                atomicAdd(&Kacc[(P.bfb + b) * nbf + Q.bfa + c], s);
+                fix_add_global(Kacc, (P.bfb + b) * nbf + Q.bfa + c, s);
             }
             for (int o = lane; o < NA * ND; o += 64) {
                 const int a = o / ND, d = o % ND;
                 double s = 0.0;
                 for (int b = 0; b < NB; ++b)
                     for (int c = 0; c < NC; ++c) s += Dbc[b * NC + c] * VAT(a, b, c, d);
-                atomicAdd(&Kacc[(P.bfa + a) * nbf + Q.bfb + d], s);
+                fix_add_global(Kacc, (P.bfa + a) * nbf + Q.bfb + d, s);
             }
             for (int o = lane; o < NB * ND; o += 64) {
                 const int b = o / ND, d = o % ND;
                 double s = 0.0;
                 for (int a = 0; a < NA; ++a)
                     for (int c = 0; c < NC; ++c) s += Dac[a * NC + c] * VAT(a, b, c, d);
-                atomicAdd(&Kacc[(P.bfb + b) * nbf + Q.bfb + d], s);
+                fix_add_global(Kacc, (P.bfb + b) * nbf + Q.bfb + d, s);
             }
         }
 #undef VAT
     }
-    if (do_j && lane < S::NAB && jp != 0.0) atomicAdd(&Jacc[(P.bfa + lane / NB) * nbf + P.bfb + lane % NB], jp);
+    if (do_j && lane < S::NAB && jp != 0.0) fix_add_global(Jacc, (P.bfa + lane / NB) * nbf + P.bfb + lane % NB, jp);
 }
 
 template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(256) void jk_wave_kernel(
     const int4* __restrict__ items, const int2* __restrict__ segs, int nitems, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket,
     int diag, const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_tab, const double* __restrict__ D,
-    double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
+    fix_t* __restrict__ Jacc, fix_t* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
     const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf) {
     constexpr int WD = JkWave<LA, LB, LC, LD, R>::WAVE_D;
     __shared__ double smem[4 * WD];
@@ -408,7 +464,7 @@ __global__ __launch_bounds__(256) void jk_wave_kernel_mp(
     const int4* __restrict__ items, const int2* __restrict__ segs, int nitems, int n64, const PairRec* __restrict__ bra,
     const PairRec* __restrict__ ket, int diag, const PrimPair<double>* __restrict__ pp,
     const PrimPair<float>* __restrict__ ppf, const double* __restrict__ rys_tab, const float* __restrict__ rys_tabf,
-    const double* __restrict__ D, double* __restrict__ Jacc, double* __restrict__ Kacc, int do_j, int do_k,
+    const double* __restrict__ D, fix_t* __restrict__ Jacc, fix_t* __restrict__ Kacc, int do_j, int do_k,
     const int* __restrict__ ket_off, const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf) {
     constexpr int WD64 = JkWave<LA, LB, LC, LD, double>::WAVE_D, WD32 = JkWave<LA, LB, LC, LD, float>::WAVE_D;
     constexpr int WD = WD64 > WD32 ? WD64 : WD32;
@@ -465,8 +521,6 @@ __global__ __launch_bounds__(256) void eri_tensor_kernel(
 #define QCG_LA(c) ((c) == 0 ? 0 : (c) <= 2 ? 1 : 2)
 #define QCG_LB(c) ((c) == 0 || (c) == 1 || (c) == 3 ? 0 : ((c) == 2 || (c) == 4) ? 1 : 2)
 
-constexpr size_t kKrowMaxBytes = 32768;
-
 // Classes that have a per-lane kernel: up to kJkLaneMaxNQ Cartesian components (above that the
 // wave-cooperative kernel always wins and the per-lane form is register-starved).
 template <typename S>
@@ -487,13 +541,14 @@ static hipError_t launch_jk_t(const JkLaunch& a, hipStream_t st) {
         }
     }
     if constexpr (kLaneForm<S>) {
-        // K rows of the bra functions in LDS when they fit (a.nbf is the largest molecule's nbf)
-        const size_t krow_bytes = (size_t)(S::NA + S::NB) * a.nbf * sizeof(double);
-        const int krow = (a.do_k && krow_bytes <= kKrowMaxBytes) ? 1 : 0;
+        // K and D rows of the bra functions in LDS when they fit (a.nbf is the largest molecule's nbf)
+        size_t dyn_bytes = 0;
+        const int bra_pp = QCG_JK_BRA_LDS ? a.bra_pp : 0;
+        const int krow = jk_row_flags<LA, LB, LC, LD>(a.nbf, a.do_k, bra_pp, &dyn_bytes);
         hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, R>), dim3(a.nitems), dim3(a.block > 0 ? a.block : 64),
-                           krow ? krow_bytes : 0, st, a.items, a.segs, a.bra, a.ket, a.diag, (const PrimPair<R>*)a.pp,
+                           dyn_bytes, st, a.items, a.segs, a.bra, a.ket, a.diag, (const PrimPair<R>*)a.pp,
                            (const R*)a.rys, a.D, a.Jacc, a.Kacc, a.do_j, a.do_k, a.ket_off, a.mat_off, a.mol_nbf,
-                           krow);
+                           krow, bra_pp);
         return hipGetLastError();
     } else {
         return hipErrorInvalidValue;  // not reached: large classes take the wave-cooperative form above
@@ -518,13 +573,14 @@ static hipError_t launch_jk_mp_t(const JkLaunch& a, hipStream_t st) {
         }
     }
     if constexpr (kLaneForm<S>) {
-        const size_t krow_bytes = (size_t)(S::NA + S::NB) * a.nbf * sizeof(double);
-        const int krow = (a.do_k && krow_bytes <= kKrowMaxBytes) ? 1 : 0;
+        size_t dyn_bytes = 0;
+        const int bra_pp = QCG_JK_BRA_LDS ? a.bra_pp : 0;
+        const int krow = jk_row_flags<LA, LB, LC, LD>(a.nbf, a.do_k, bra_pp, &dyn_bytes);
         hipLaunchKernelGGL((jk_kernel_mp<LA, LB, LC, LD>), dim3(a.nitems), dim3(a.block > 0 ? a.block : 64),
-                           krow ? krow_bytes : 0, st, a.items, a.segs, a.n64, a.bra, a.ket, a.diag,
+                           dyn_bytes, st, a.items, a.segs, a.n64, a.bra, a.ket, a.diag,
                            (const PrimPair<double>*)a.pp, (const PrimPair<float>*)a.ppf, (const double*)a.rys,
                            (const float*)a.rysf, a.D, a.Jacc, a.Kacc, a.do_j, a.do_k, a.ket_off, a.mat_off, a.mol_nbf,
-                           krow);
+                           krow, bra_pp);
         return hipGetLastError();
     } else {
         return hipErrorInvalidValue;  // not reached: large classes take the wave-cooperative form above

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "fixacc.h"
 #include "qcg_types.h"
 
 namespace qcg {
@@ -53,6 +54,11 @@ inline void cut_items_split(std::vector<int4>& items, const std::vector<int2>& s
 
 inline QCG_HD int item_kets(const int4& it) { return it.w & kItemMaxKets; }
 
+// Most bra primitive pairs one per-lane J/K item may have: the kernel stages them in LDS (a block's
+// bra pair is uniform, so the inner primitive loop reads them as LDS broadcasts instead of scalar
+// loads).  The planner cuts heavier bras into primitive ranges of at most this many.
+constexpr int kBraPPMax = 64;
+
 // Largest class (Cartesian components) with a per-lane J/K kernel; larger classes always run the
 // wave-cooperative one, classes from 27 components up to this limit may run either (tile block 0).
 #ifndef QCG_JK_LANE_MAX_NQ
@@ -97,8 +103,8 @@ struct JkLaunch {
     const void* rys;    // Rys table (double or float)
     const double* D;    // density, nbf x nbf, symmetric
     int nbf;
-    double* Jacc;
-    double* Kacc;
+    fix_t* Jacc;        // fixed-point accumulators (fixacc.h), 2 words per element
+    fix_t* Kacc;
     double thr_lo;      // process quartets with thr_lo <= Q_P Q_Q < thr_hi
     double thr_hi;
     int do_j, do_k;
@@ -116,6 +122,7 @@ struct JkLaunch {
     const int2* segs;
     int nitems;
     int block;
+    int bra_pp;         // most primitive pairs of any bra of the items (<= kBraPPMax): LDS staging
     // mixed precision in one launch (n64 >= 0): the first n64 items are FP64, the rest FP32 with
     // the FP32 primitive pairs ppf and Rys table rysf; n64 < 0: one precision, given by the caller
     int n64 = -1;
@@ -210,11 +217,11 @@ inline hipError_t launch_grad(int cb, int ck, const GradLaunch& a, hipStream_t s
 hipError_t launch_schwarz(int c, const PairRec* pairs, int n, const PrimPair<double>* pp, const double* rys,
                           double* Q, hipStream_t st);
 hipError_t launch_fill(double* x, size_t n, double v, hipStream_t st);
-hipError_t launch_symmetrize(const double* Jacc, const double* Kacc, double* J, double* K, int nbf, hipStream_t st);
+hipError_t launch_symmetrize(const fix_t* Jacc, const fix_t* Kacc, double* J, double* K, int nbf, hipStream_t st);
 // batched J = Jacc + Jacc^T, K = Kacc + Kacc^T over block-diagonal packed matrices
-hipError_t launch_symmetrize_batch(const double* Jacc, const double* Kacc, double* J, double* K, int nmol,
+hipError_t launch_symmetrize_batch(const fix_t* Jacc, const fix_t* Kacc, double* J, double* K, int nmol,
                                    const int64_t* mat_off, const int* mol_nbf, int max_nbf, hipStream_t st);
-hipError_t launch_fock_combine(const double* H, const double* Jacc, const double* Kacc, double* F, int nbf,
+hipError_t launch_fock_combine(const double* H, const fix_t* Jacc, const fix_t* Kacc, double* F, int nbf,
                                double kfac, hipStream_t st);
 hipError_t launch_boys_batch(int m, int n, const double* T, double* F, int* status, hipStream_t st);
 hipError_t launch_rys_batch(int nroots, int n, const double* T, const double* tab, double* x, double* w,

@@ -55,48 +55,56 @@ hipError_t launch_fill(double* x, size_t n, double v, hipStream_t st) {
     return hipGetLastError();
 }
 
+// Element (i, j) of A + A^T for a fixed-point accumulator: the integer words are summed before
+// the conversion, so the result is exactly symmetric.
+__device__ __forceinline__ double fix_sym(const fix_t* __restrict__ a, size_t ij, size_t ji) {
+    return fix_value(a[2 * ij] + a[2 * ji], a[2 * ij + 1] + a[2 * ji + 1]);
+}
+
 // J = Jacc + Jacc^T, K = Kacc + Kacc^T
-__global__ void symmetrize_kernel(const double* __restrict__ Jacc, const double* __restrict__ Kacc,
+__global__ void symmetrize_kernel(const fix_t* __restrict__ Jacc, const fix_t* __restrict__ Kacc,
                                   double* __restrict__ J, double* __restrict__ K, int nbf) {
     const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nbf) return;
-    if (J) J[(size_t)i * nbf + j] = Jacc[(size_t)i * nbf + j] + Jacc[(size_t)j * nbf + i];
-    if (K) K[(size_t)i * nbf + j] = Kacc[(size_t)i * nbf + j] + Kacc[(size_t)j * nbf + i];
+    const size_t ij = (size_t)i * nbf + j, ji = (size_t)j * nbf + i;
+    if (J) J[ij] = fix_sym(Jacc, ij, ji);
+    if (K) K[ij] = fix_sym(Kacc, ij, ji);
 }
 
-__global__ void symmetrize_batch_kernel(const double* __restrict__ Jacc, const double* __restrict__ Kacc,
+__global__ void symmetrize_batch_kernel(const fix_t* __restrict__ Jacc, const fix_t* __restrict__ Kacc,
                                         double* __restrict__ J, double* __restrict__ K,
                                         const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf) {
     const int m = blockIdx.z, n = mol_nbf[m];
     const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || j >= n) return;
     const size_t o = (size_t)mat_off[m];
-    if (J) J[o + (size_t)i * n + j] = Jacc[o + (size_t)i * n + j] + Jacc[o + (size_t)j * n + i];
-    if (K) K[o + (size_t)i * n + j] = Kacc[o + (size_t)i * n + j] + Kacc[o + (size_t)j * n + i];
+    const size_t ij = o + (size_t)i * n + j, ji = o + (size_t)j * n + i;
+    if (J) J[ij] = fix_sym(Jacc, ij, ji);
+    if (K) K[ij] = fix_sym(Kacc, ij, ji);
 }
 
-hipError_t launch_symmetrize_batch(const double* Jacc, const double* Kacc, double* J, double* K, int nmol,
+hipError_t launch_symmetrize_batch(const fix_t* Jacc, const fix_t* Kacc, double* J, double* K, int nmol,
                                    const int64_t* mat_off, const int* mol_nbf, int max_nbf, hipStream_t st) {
     hipLaunchKernelGGL(symmetrize_batch_kernel, dim3((max_nbf + 127) / 128, max_nbf, nmol), dim3(128), 0, st, Jacc,
                        Kacc, J, K, mat_off, mol_nbf);
     return hipGetLastError();
 }
 
-hipError_t launch_symmetrize(const double* Jacc, const double* Kacc, double* J, double* K, int nbf, hipStream_t st) {
+hipError_t launch_symmetrize(const fix_t* Jacc, const fix_t* Kacc, double* J, double* K, int nbf, hipStream_t st) {
     hipLaunchKernelGGL(symmetrize_kernel, dim3((nbf + 127) / 128, nbf), dim3(128), 0, st, Jacc, Kacc, J, K, nbf);
     return hipGetLastError();
 }
 
 // F = H + J - kfac K  (build_fock, scf/mod.rs:825-881 with kfac = 1/2; hybrid form dft/ks_scf.rs:1960)
-__global__ void fock_combine_kernel(const double* __restrict__ H, const double* __restrict__ Jacc,
-                                    const double* __restrict__ Kacc, double* __restrict__ F, int nbf, double kfac) {
+__global__ void fock_combine_kernel(const double* __restrict__ H, const fix_t* __restrict__ Jacc,
+                                    const fix_t* __restrict__ Kacc, double* __restrict__ F, int nbf, double kfac) {
     const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nbf) return;
     const size_t ij = (size_t)i * nbf + j, ji = (size_t)j * nbf + i;
-    F[ij] = H[ij] + (Jacc[ij] + Jacc[ji]) - kfac * (Kacc[ij] + Kacc[ji]);
+    F[ij] = H[ij] + fix_sym(Jacc, ij, ji) - kfac * fix_sym(Kacc, ij, ji);
 }
 
-hipError_t launch_fock_combine(const double* H, const double* Jacc, const double* Kacc, double* F, int nbf,
+hipError_t launch_fock_combine(const double* H, const fix_t* Jacc, const fix_t* Kacc, double* F, int nbf,
                                double kfac, hipStream_t st) {
     hipLaunchKernelGGL(fock_combine_kernel, dim3((nbf + 127) / 128, nbf), dim3(128), 0, st, H, Jacc, Kacc, F, nbf,
                        kfac);

@@ -97,7 +97,7 @@ __device__ __forceinline__ void wave_tables(const PairRec& P, const PairRec& Q, 
         vrr2d<NB_, NK_, R>(g, PA - x * qf * PQ, QC + x * pf * PQ, b00, b10, b01);
         R sc = R(1);
         if (axis == 2)
-            sc = (a.kab * c.kab < (R)kPrimScreen) ? R(0) : w * (two_pi_52 * rs * a.ckp * c.ckp);
+            sc = (prim_kab(a) * prim_kab(c) < kPrimScreen) ? R(0) : w * (two_pi_52 * rs * a.ckp * c.ckp);
         R* row = tbl + axis * HT * ES + pql * N + r;
         hrr1d_put<TA, TB, TC, TD, R>(g, AB, CD, [&](int h, R v) { row[h * ES] = v * sc; });
     }

@@ -239,8 +239,9 @@ int qcg_set_basis_batch(qcg_context* c, int nmol, const int* mol_nshell, const i
     std::vector<PrimPair<float>> ppf(pp.size());
     for (size_t i = 0; i < pp.size(); ++i) {
         ppf[i].p = (float)pp[i].p; ppf[i].px = (float)pp[i].px; ppf[i].py = (float)pp[i].py;
-        ppf[i].pz = (float)pp[i].pz; ppf[i].kab = (float)pp[i].kab; ppf[i].ckp = (float)pp[i].ckp;
-        ppf[i].hp = (float)pp[i].hp; ppf[i].ea = (float)pp[i].ea;
+        ppf[i].pz = (float)pp[i].pz; ppf[i].ckp = (float)pp[i].ckp;
+        ppf[i].hp = (float)pp[i].hp;
+        set_prim_kab(ppf[i], pp[i].kab);  // FP64 bits in kab + ea: the FP64 screening decisions
     }
     c->h_pp = pp;
     HIPCHK(c, c->alloc(&c->d_pp, pp.size()));
@@ -302,8 +303,8 @@ int qcg_set_basis_batch(qcg_context* c, int nmol, const int* mol_nshell, const i
     HIPCHK(c, c->alloc(&c->d_D, n2));
     HIPCHK(c, c->alloc(&c->d_J, n2));
     HIPCHK(c, c->alloc(&c->d_K, n2));
-    HIPCHK(c, c->alloc(&c->d_Jacc, n2));
-    HIPCHK(c, c->alloc(&c->d_Kacc, n2));
+    HIPCHK(c, c->alloc(&c->d_Jacc, 2 * n2));
+    HIPCHK(c, c->alloc(&c->d_Kacc, 2 * n2));
     HIPCHK(c, c->alloc(&c->d_H, n2));
     HIPCHK(c, c->alloc(&c->d_F, n2));
     c->has_basis = true;
@@ -346,6 +347,16 @@ static int build_plan(qcg_context* c) {
     const bool mixed = c->precision == QCG_PREC_MIXED && c->fp32_cutoff > c->schwarz_thr;
     const double key[4] = {(double)c->precision, c->fp32_cutoff, c->schwarz_thr, (double)c->tile_sig};
     if (c->plan_sig == 1 && std::memcmp(key, c->plan_key, sizeof(key)) == 0) return QCG_OK;
+    // a work item encodes its bra's primitive range in 8 bits (cut_items_split)
+    for (const PairRec& pr : c->pairs)
+        if (pr.npp > kItemMaxPrim)
+            return fail(c, QCG_INVALID_ARG, "shell pair with more than 255 primitive pairs (at most 15 primitives per shell)");
+    for (int cb = 0; cb < kNumPairClasses; ++cb) {
+        int m = 1;
+        for (int i = c->class_off[cb]; i < c->class_off[cb + 1]; ++i) m = std::max(m, c->pairs[i].npp);
+        const int split = (c->prim_split > 0 && c->prim_split < kBraPPMax) ? c->prim_split : kBraPPMax;
+        c->bra_pp[cb] = std::min(m, split);
+    }
     c->items.clear();
     c->segs.clear();
     const int nmol = c->nmol;
@@ -370,8 +381,10 @@ static int build_plan(qcg_context* c) {
                             const int lo = part == 0 ? bs : sp, hi = part == 0 ? sp : e;
                             if (hi > lo) c->segs.push_back(make_int2(lo, hi - lo));
                         });
-                        cut_items_split(c->items, c->segs, g0, pbase + p, ch,
-                                        c->pairs[c->class_off[cb] + pbase + p].npp, c->prim_split);
+                        // bras above kBraPPMax primitive pairs are always cut into ranges
+                        const int npp = c->pairs[c->class_off[cb] + pbase + p].npp;
+                        const int split = (c->prim_split > 0 && c->prim_split < kBraPPMax) ? c->prim_split : kBraPPMax;
+                        cut_items_split(c->items, c->segs, g0, pbase + p, ch, npp, split);
                     }
                 }
                 c->item_cnt[cls][part] = (int)c->items.size() - c->item_off[cls][part];
@@ -418,8 +431,9 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
         if (rc) return rc;
     }
     HIPCHK(c, stream_begin(c, st));
-    if (do_j) HIPCHK(c, launch_fill(c->d_Jacc, n2, 0.0, st));
-    if (do_k) HIPCHK(c, launch_fill(c->d_Kacc, n2, 0.0, st));
+    // the fixed-point words are zeroed as doubles: +0.0 is the all-zero bit pattern
+    if (do_j) HIPCHK(c, launch_fill(reinterpret_cast<double*>(c->d_Jacc), 2 * n2, 0.0, st));
+    if (do_k) HIPCHK(c, launch_fill(reinterpret_cast<double*>(c->d_Kacc), 2 * n2, 0.0, st));
     const bool mixed = c->precision == QCG_PREC_MIXED && c->fp32_cutoff > c->schwarz_thr;
     if (c->profiling && !c->ev_made) {
         for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
@@ -482,6 +496,7 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
             a.segs = c->d_segs;
             a.nitems = c->item_cnt[cls][part];
             a.block = c->tile[cls][part].block;
+            a.bra_pp = c->bra_pp[cb];
             a.pp = part ? (const void*)c->d_ppf : (const void*)c->d_pp;
             a.rys = part ? (const void*)c->d_rys_f : (const void*)c->d_rys;
             if (fused) {
@@ -517,7 +532,8 @@ namespace {
 
 // Autotuner candidates {threads per block, kets per thread}; block 0 = one wave per work item of
 // `reps` kets (wave-cooperative kernel, classes with at least 27 components only).
-constexpr int kTuneCand[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {256, 1}, {64, 8}, {0, 1}, {0, 2}, {0, 4}};
+constexpr int kTuneCand[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {256, 1}, {64, 8},
+                                {128, 4}, {256, 2}, {0, 1}, {0, 2}, {0, 4}};
 constexpr int kTuneNCand = sizeof(kTuneCand) / sizeof(kTuneCand[0]);
 
 // Candidate k mapped onto what class cls has: classes with fewer than 27 components have only the
@@ -991,6 +1007,7 @@ int refine_fp64(void* u) {
     static_cast<RhfUser*>(u)->c->precision = QCG_PREC_FP64;
     return QCG_OK;
 }
+int rhf_bound_cb(void* u, const double* D, double* b) { return qcg_error_bound(static_cast<RhfUser*>(u)->c, D, 1.0, b); }
 }  // namespace
 
 int qcg_rhf_scf(qcg_context* c, const qcg_scf_config* cfg, const double* D_init, qcg_scf_result* res, double* D,
@@ -1011,7 +1028,7 @@ int qcg_rhf_scf(qcg_context* c, const qcg_scf_config* cfg, const double* D_init,
     qcg_scf_result r{};
     const int prec = c->precision;
     int rc = scf_driver(B.nbf, nelec / 2, S.data(), H.data(), nuclear_repulsion(B), *cfg, rhf_fock_cb, &u, D_init, &r,
-                        D, C, eps, F, prec == QCG_PREC_MIXED ? refine_fp64 : nullptr);
+                        D, C, eps, F, prec == QCG_PREC_MIXED ? refine_fp64 : nullptr, rhf_bound_cb, &c->scf_history);
     c->precision = prec;
     if (res) *res = r;
     if (rc == QCG_NOT_CONVERGED) return fail(c, rc, "ScfError::NotConverged");

@@ -20,6 +20,22 @@ struct alignas(8 * sizeof(R)) PrimPair {
     R p, px, py, pz, kab, ckp, hp, ea;
 };
 
+// k_ab of a primitive pair in FP64 for the 1e-15 primitive screens (k_ab k_cd < 1e-15 skips the
+// primitive quartet, integrals/eri/mod.rs:611).  The FP32 copies carry the FP64 value's bits in their
+// kab (low word) and ea (high word) fields, ea being read by the FP64 gradient kernels only, so the
+// FP32 quartets take exactly the FP64 screening decisions (those the planner and the flop model
+// count).
+QCG_HD double prim_kab(const PrimPair<double>& a) { return a.kab; }
+QCG_HD double prim_kab(const PrimPair<float>& a) {
+    const unsigned long long lo = __builtin_bit_cast(unsigned int, a.kab), hi = __builtin_bit_cast(unsigned int, a.ea);
+    return __builtin_bit_cast(double, lo | (hi << 32));
+}
+inline void set_prim_kab(PrimPair<float>& a, double kab) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, kab);
+    a.kab = __builtin_bit_cast(float, (unsigned int)(b & 0xffffffffu));
+    a.ea = __builtin_bit_cast(float, (unsigned int)(b >> 32));
+}
+
 // A shell pair with l(sa) >= l(sb) (the two shells are swapped if needed; integrals are
 // invariant to that relabelling).  Lists of these are grouped by pair class and sorted by
 // Schwarz bound Q, descending.

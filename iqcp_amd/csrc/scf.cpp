@@ -101,10 +101,30 @@ bool ScfRun::criteria_met() const {
     return last_dE < cfg.energy_tol && last_rms < cfg.density_tol && last_g < std::sqrt(cfg.energy_tol);
 }
 
-bool ScfRun::accept_fock(const double* F_, double e_elec, bool may_converge) {
+bool ScfRun::accept_fock(const double* F_, double e_elec, bool may_converge, double bound) {
     const size_t n2 = (size_t)nbf * nbf;
     F.assign(F_, F_ + n2);
     const double e_new = e_elec + e_nuc;
+    qcg_scf_iteration rec{};
+    rec.iteration = iter;
+    rec.diis_applied = iter > 0 && last_diis;
+    rec.fp32_build = may_converge ? 0 : 1;
+    rec.energy_total = e_new;
+    rec.energy_electronic = e_elec;
+    rec.delta_e = rec.rms_density_change = std::nan("");
+    rec.error_bound = bound;
+    struct Push {  // records the iteration on every return path below
+        ScfRun* r;
+        qcg_scf_iteration* rec;
+        ~Push() {
+            if (r->iter > 0) {
+                rec->delta_e = r->last_dE;
+                rec->rms_density_change = r->last_rms;
+            }
+            rec->converged = r->converged ? 1 : 0;
+            r->history.push_back(*rec);
+        }
+    } push{this, &rec};
     if (iter > 0) {
         F_last = F;
         have_last = true;
@@ -175,6 +195,7 @@ void ScfRun::step() {
         diis.e.push_back(Err);
         if (diis.f.size() >= 2) Fdiag = diis.extrapolate();
     }
+    last_diis = diis_now && diis.f.size() >= 2;
     if (shift) {
         std::vector<double> SDS(n2);
         matmul(S.data(), DS.data(), SDS.data(), n);
@@ -196,17 +217,25 @@ void ScfRun::fill_result(ScfResult* res) const {
 
 int scf_driver(int nbf, int nocc, const double* S, const double* H, double e_nuc, const ScfConfig& cfg,
                FockFn fock, void* user, const double* D_init, ScfResult* res, double* D_out, double* C_out,
-               double* eps_out, double* F_out, RefineFn refine) {
+               double* eps_out, double* F_out, RefineFn refine, BoundFn bound,
+               std::vector<qcg_scf_iteration>* history) {
     ScfRun run;
+    if (history) history->clear();
     int rc = run.init(nbf, nocc, S, H, e_nuc, cfg, D_init);
     if (rc) return rc;
     std::vector<double> F((size_t)nbf * nbf);
     bool exact = refine == nullptr;
     for (;;) {
-        double e_elec = 0.0;
+        double e_elec = 0.0, b = 0.0;
         rc = fock(user, run.D.data(), F.data(), &e_elec);
         if (rc) return rc;
-        if (run.accept_fock(F.data(), e_elec, exact)) break;
+        if (!exact && bound) {
+            rc = bound(user, run.D.data(), &b);
+            if (rc) return rc;
+        }
+        const bool fin = run.accept_fock(F.data(), e_elec, exact, b);
+        if (history) *history = run.history;
+        if (fin) break;
         if (!exact && run.iter > 0 && (run.last_rms < kRefineRms || run.criteria_met())) {
             rc = refine(user);
             if (rc) return rc;

@@ -403,6 +403,11 @@ int ks_refine_fp64(void* u) {
     static_cast<KsUser*>(u)->c->precision = QCG_PREC_FP64;
     return QCG_OK;
 }
+// the two-electron part's bound (exact-exchange fraction a)
+int ks_bound_cb(void* u, const double* D, double* b) {
+    KsUser* k = static_cast<KsUser*>(u);
+    return qcg_error_bound(k->c, D, k->a, b);
+}
 
 }  // namespace
 
@@ -425,7 +430,7 @@ extern "C" int qcg_ks_scf(qcg_context* c, const qcg_scf_config* cfg, const doubl
     qcg_scf_result r{};
     const int prec = c->precision;
     int rc = scf_driver(B.nbf, nelec / 2, S.data(), H.data(), nuclear_repulsion(B), *cfg, ks_fock_cb, &u, D_init, &r,
-                        D, C, eps, F, prec == QCG_PREC_MIXED ? ks_refine_fp64 : nullptr);
+                        D, C, eps, F, prec == QCG_PREC_MIXED ? ks_refine_fp64 : nullptr, ks_bound_cb, &c->scf_history);
     c->precision = prec;
     r.energy_xc = u.e_xc;
     r.energy_j = u.e_j;

@@ -40,7 +40,7 @@ EXPORTS = [
     "qcg_set_basis_batch", "qcg_batch_build_jk", "qcg_batch_rhf_scf", "qcg_autotune",
     "qcg_set_tiles", "qcg_get_tiles", "qcg_xc_gradient", "qcg_ks_gradient", "qcg_set_graph",
     "qcg_set_prim_split", "qcg_set_xc_mode", "qcg_build_fock_stored", "qcg_rhf_scf_stored",
-    "qcg_autotune_refine",
+    "qcg_autotune_refine", "qcg_error_bound", "qcg_scf_history",
 ]
 
 
@@ -79,6 +79,17 @@ class ClassStat(ctypes.Structure):
                 ("prim_quartets_fp64", ctypes.c_int64), ("prim_quartets_fp32", ctypes.c_int64),
                 ("flops_fp64", ctypes.c_double), ("flops_fp32", ctypes.c_double),
                 ("ms_fp64", ctypes.c_double), ("ms_fp32", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ScfIteration(ctypes.Structure):
+    """qcg_scf_iteration = ScfIteration (scf/mod.rs:627-650) plus the mixed-precision fields."""
+    _fields_ = [("iteration", ctypes.c_int), ("converged", ctypes.c_int), ("diis_applied", ctypes.c_int),
+                ("fp32_build", ctypes.c_int), ("energy_total", ctypes.c_double),
+                ("energy_electronic", ctypes.c_double), ("delta_e", ctypes.c_double),
+                ("rms_density_change", ctypes.c_double), ("error_bound", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -190,6 +201,21 @@ class Engine:
         f32 = ctypes.c_int64()
         self._check(self.L.qcg_screen_counts(self.ctx, ctypes.byref(kept), ctypes.byref(f32)))
         return kept.value, f32.value
+
+    def error_bound(self, D, hf_frac: float = 1.0) -> float:
+        """Bound on |E2(mixed) - E2(FP64)| of a J/K build at D (0 in FP64 mode)."""
+        b = ctypes.c_double()
+        Dc = np.ascontiguousarray(D, dtype=np.float64)
+        self._check(self.L.qcg_error_bound(self.ctx, _d(Dc), ctypes.c_double(hf_frac), ctypes.byref(b)))
+        return b.value
+
+    def scf_history(self):
+        """Per-iteration trace of the last rhf_scf / ks_scf (list of dicts)."""
+        n = ctypes.c_int()
+        self._check(self.L.qcg_scf_history(self.ctx, None, 0, ctypes.byref(n)))
+        arr = (ScfIteration * max(1, n.value))()
+        self._check(self.L.qcg_scf_history(self.ctx, arr, n.value, ctypes.byref(n)))
+        return [arr[i].as_dict() for i in range(n.value)]
 
     def eri_compressed(self) -> np.ndarray:
         npair = self.nbf * (self.nbf + 1) // 2
@@ -429,6 +455,13 @@ class BatchEngine:
 
     def get_tiles(self):
         return Engine.get_tiles(self)
+
+    def class_stats(self):
+        """Per quartet-class counts and flop model over all molecules of the batch."""
+        arr = (ClassStat * 21)()
+        n = ctypes.c_int()
+        self._check(self.L.qcg_class_stats(self.ctx, arr, ctypes.byref(n)))
+        return [arr[i].as_dict() for i in range(n.value)]
 
     def autotune(self, dD_ptr: int, iters: int = 2):
         """Per-class tile choice on a packed device density (all molecules), as Engine.autotune."""

@@ -56,6 +56,10 @@ def test_decane_fock_parity_at_config_size():
         E[prec] = _energy(D, H, F)
     assert abs(E["fp64"] - Eo) < 1e-9
     assert abs(E["mixed"] - E["fp64"]) < 1e-6
+    # the stated error of the mixed build bounds the measured one, and is itself below 1 uHa
+    eng.set_options("mixed", 1e-6)
+    bound = eng.error_bound(D, 1.0)
+    assert abs(E["mixed"] - E["fp64"]) <= bound < 1e-6, (E["mixed"] - E["fp64"], bound)
 
 
 def _displaced(b, a, x, h, basis):
@@ -106,13 +110,16 @@ def test_water16_b3lyp_at_config_size():
         nel += float(np.sum(w[i:i + 100000] * np.einsum("gi,gi->g", chi @ D, chi)))
     assert abs(nel - 2 * b.n_occ) < 2e-3, nel
     # J + K + XC energy at the converged density: mixed precision against all-FP64
-    e2 = {}
+    e2, exc = {}, {}
     for prec in ("fp64", "mixed"):
         eng.set_options(prec, 1e-6)
         jk = eng.build_jk(D)
-        V, exc = eng.xc(D)
-        e2[prec] = 0.5 * np.sum(D * jk.J) - 0.25 * 0.20 * np.sum(D * jk.K) + exc
-    assert abs(e2["mixed"] - e2["fp64"]) < 1e-6, e2
+        V, exc[prec] = eng.xc(D)
+        e2[prec] = 0.5 * np.sum(D * jk.J) - 0.25 * 0.20 * np.sum(D * jk.K)
+    assert abs(e2["mixed"] + exc["mixed"] - e2["fp64"] - exc["fp64"]) < 1e-6, (e2, exc)
+    # two-electron part: within the stated bound of the mixed build
+    bound = eng.error_bound(D, 0.20)
+    assert abs(e2["mixed"] - e2["fp64"]) <= bound < 1e-6, (e2["mixed"] - e2["fp64"], bound)
     # V_xc / E_xc against the oracle on a sample of whole 512-point blocks of the grid
     rng = np.random.default_rng(0)
     blocks = rng.choice(n // 512, size=24, replace=False)

@@ -203,6 +203,44 @@ def test_gpu_fock_graph_replay():
             eng.set_graph(False)
 
 
+def test_gpu_fock_bitwise_reproducible():
+    """J and K are accumulated in fixed point (order-independent integer sums), so with the tiles
+    held fixed every build of the same density gives the same bits: 20 graph replays of the device
+    Fock build (21 class kernels on four streams) and eager builds in between, FP64 and mixed.
+    The reference merges its per-thread Fock parts in a fixed order for the same reason
+    (integrals/eri/mod.rs:1366-1380)."""
+    import torch
+    b = bench_molecule("C6H6", "6-31g*")
+    eng = _eng(b)
+    dev = torch.device("cuda:0")
+    H = np.diag(np.linspace(-2, 1, b.nbf))
+    D = random_density(b.nbf, seed=3)
+    eri, _ = oracle.eri_compressed(b)
+    Fo = oracle.build_fock(H, D, eri, b.nbf)
+    st = torch.cuda.Stream(device=dev)
+    dH, dD = torch.tensor(H, device=dev), torch.tensor(D, device=dev)
+    dF = torch.empty_like(dD)
+    for prec, tol in (("fp64", 1e-11), ("mixed", 1e-8)):
+        eng.set_options(prec, 1e-6)
+        eng.set_graph(True)
+        first = None
+        for rep in range(20):
+            dF.fill_(7.0)
+            eng.build_fock_device(dH.data_ptr(), dD.data_ptr(), dF.data_ptr(), 0.5, st.cuda_stream)
+            torch.cuda.synchronize()
+            F = dF.cpu().numpy()
+            if first is None:
+                first = F
+                assert np.abs(F - Fo).max() < tol, prec
+                assert np.array_equal(F, F.T)  # integer words summed before the conversion
+            assert np.array_equal(F, first), (prec, rep, np.abs(F - first).max())
+            if rep % 5 == 4:
+                eng.build_jk(random_density(b.nbf, seed=rep))
+        eng.set_graph(False)
+        for rep in range(3):
+            assert np.array_equal(eng.build_fock(H, D), first), (prec, "eager", rep)
+
+
 def test_gpu_autotune_keeps_parity():
     """After the autotuner picks a tile per (class, precision part), J/K still match the oracle,
     and every chosen tile is one of the candidates."""
@@ -212,7 +250,8 @@ def test_gpu_autotune_keeps_parity():
     eng = _eng(b)
     D = random_density(b.nbf, seed=11)
     dD = torch.tensor(D, device=torch.device("cuda:0"))
-    cands = {(64, 1), (64, 2), (64, 4), (128, 1), (256, 1), (64, 8), (0, 1), (0, 2), (0, 4), (0, 8)}
+    cands = {(64, 1), (64, 2), (64, 4), (128, 1), (256, 1), (64, 8), (128, 4), (256, 2), (0, 1), (0, 2), (0, 4),
+             (0, 8)}
     Jo, Ko = oracle.build_jk(D, eri, b.nbf)
     for prec in ("fp64", "mixed"):
         eng.set_options(prec, 1e-6)
@@ -304,3 +343,84 @@ def test_gpu_rhf_presets_heh_plus():
     d, b = preset("heh_plus_sto3g")
     out = _eng(b).rhf_scf(ScfConfig.tight())
     assert abs(out["energy_total"] - d["reference"]["energy"]) < 1e-7
+
+
+# ------------------------------------------------------------------ mixed-precision error statement
+def _shell_of_bf(b):
+    s = np.zeros(b.nbf, dtype=int)
+    for sh, (off, l) in enumerate(zip(b.shell_bf_off, b.shell_l)):
+        s[off:off + (l + 1) * (l + 2) // 2] = sh
+    return s
+
+
+def test_gpu_fp32_quartet_error_against_schwarz():
+    """Single integrals in both precisions: with the FP32 cutoff above every Schwarz product all
+    quartets run in FP32, and a density that is one symmetric unit pair (k, l) turns J into
+    2 (mu nu|k l).  Each FP32 integral's error stays below 2^-20 Q_ab Q_cd (the bound of
+    qcg_error_bound allows 2^-16).  H2O 6-31G* (s, p and d shells), k, l over the first function
+    of every shell pair."""
+    b = bench_molecule("H2O", "6-31g*")
+    eng = _eng(b)
+    Q = eng.schwarz_bounds()
+    sh = _shell_of_bf(b)
+    worst = 0.0
+    for x in range(b.nshell):
+        for y in range(x + 1):
+            k, l = b.shell_bf_off[x], b.shell_bf_off[y]
+            D = np.zeros((b.nbf, b.nbf))
+            D[k, l] = D[l, k] = 1.0
+            eng.set_options("fp64")
+            J64 = eng.build_jk(D, want_k=False).J
+            eng.set_options("mixed", 1e30)
+            J32 = eng.build_jk(D, want_k=False).J
+            scale = (1.0 if k == l else 2.0) * Q[np.ix_(sh, sh)] * Q[x, y]
+            keep = scale > 1e-12
+            worst = max(worst, float(np.max(np.abs(J32 - J64)[keep] / scale[keep])))
+    assert worst < 2.0 ** -20, worst
+
+
+@pytest.mark.parametrize("cutoff", [1e-6, 1e30])
+def test_gpu_error_bound_holds(cutoff):
+    """qcg_error_bound bounds the two-electron energy error of a mixed build against the FP64 build
+    at the same density (RHF and hybrid exchange weights), for the default cutoff and with every
+    quartet in FP32; zero in FP64 mode."""
+    b = bench_molecule("C6H6", "6-31g*")
+    eng = _eng(b)
+    D = random_density(b.nbf, seed=7, nocc=b.n_occ)
+    jk64 = eng.build_jk(D)
+    assert eng.error_bound(D) == 0.0
+    eng.set_options("mixed", cutoff)
+    jk32 = eng.build_jk(D)
+    for a in (1.0, 0.2):
+        e64 = 0.5 * np.sum(D * (jk64.J - 0.5 * a * jk64.K))
+        e32 = 0.5 * np.sum(D * (jk32.J - 0.5 * a * jk32.K))
+        bound = eng.error_bound(D, a)
+        assert abs(e32 - e64) <= bound, (a, abs(e32 - e64), bound)
+        if cutoff == 1e-6:
+            assert bound < 1e-6  # the default split keeps a 1 uHa statement
+
+
+def test_gpu_scf_history():
+    """The iteration trace mirrors ScfIteration (scf/mod.rs:1407-1531): iteration 0 is the guess
+    build (no delta_e / rms, no DIIS), the last entry is the converged one with the returned
+    energy; a mixed run's early iterations are FP32 builds with a positive error bound and its last
+    ones FP64 builds."""
+    b = bench_molecule("H2O", "6-31g*")
+    for prec in ("fp64", "mixed"):
+        eng = _eng(b)
+        eng.set_options(prec, 1e-6)
+        out = eng.rhf_scf(ScfConfig.tight())
+        h = eng.scf_history()
+        assert len(h) == out["iterations"] + 1
+        assert [r["iteration"] for r in h] == list(range(len(h)))
+        assert np.isnan(h[0]["delta_e"]) and np.isnan(h[0]["rms_density_change"]) and not h[0]["diis_applied"]
+        assert h[-1]["converged"] == 1 and sum(r["converged"] for r in h) == 1
+        assert h[-1]["energy_total"] == out["energy_total"]
+        assert all(r["diis_applied"] for r in h[2:])
+        for r in h[1:]:
+            assert r["delta_e"] >= 0.0 and r["rms_density_change"] >= 0.0
+        if prec == "fp64":
+            assert not any(r["fp32_build"] for r in h) and all(r["error_bound"] == 0.0 for r in h)
+        else:
+            assert h[0]["fp32_build"] == 1 and h[0]["error_bound"] > 0.0
+            assert h[-1]["fp32_build"] == 0
