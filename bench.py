@@ -51,6 +51,21 @@ def hbm_traffic(name):
     return None if k is None else k["bytes_per_launch"]
 
 
+def grad_model_flops(stats):
+    """Algorithmic flops of one ERI-gradient evaluation from the J/K class counts (same Schwarz and
+    primitive screens): per surviving primitive quartet n_g [80 + 15 (la+lb+2)(lc+ld+2)
+    + 13 N_q (4 + 6 (lb+ld+1))] with n_g = (L+2)//2 + 1 roots, plus 144 N_q per contracted quartet."""
+    tot = 0.0
+    for x in stats:
+        la, lb, lc, ld = x["la"], x["lb"], x["lc"], x["ld"]
+        nq = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
+        ng = (la + lb + lc + ld + 2) // 2 + 1
+        per_prim = ng * (80 + 15 * (la + lb + 2) * (lc + ld + 2) + 13 * nq * (4 + 6 * (lb + ld + 1)))
+        tot += per_prim * (x["prim_quartets_fp64"] + x["prim_quartets_fp32"])
+        tot += 144 * nq * (x["quartets_fp64"] + x["quartets_fp32"])
+    return tot
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -268,6 +283,17 @@ def main():
             if not args.no_autotune:
                 ew.autotune(dDw.data_ptr(), 2)
                 ew.autotune_refine(dDw.data_ptr(), 10)
+            # the XC build alone (HIP events on the stream it runs on)
+            with torch.cuda.stream(stream):
+                ew.xc_device(dDw.data_ptr(), dVw.data_ptr(), dEw.data_ptr(), stream.cuda_stream)
+                nx = max(1, args.steps // 4)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(nx):
+                    ew.xc_device(dDw.data_ptr(), dVw.data_ptr(), dEw.data_ptr(), stream.cuda_stream)
+                e1.record(stream)
+                torch.cuda.synchronize()
+            ks[f"{prec}_xc_ms"] = round(e0.elapsed_time(e1) / nx, 3)
             with torch.cuda.stream(stream):
                 for _ in range(2):
                     ew.build_jk_device(dDw.data_ptr(), dJw.data_ptr(), dKw.data_ptr(), stream.cuda_stream)
@@ -285,6 +311,16 @@ def main():
             ks[f"{prec}_ms_per_build"] = round(1e3 * el_k / nk, 3)
             ks[f"{prec}_two_electron_plus_xc_energy"] = Ek
         ks["speedup_mixed_vs_fp64"] = round(ks["fp64_ms_per_build"] / ks["mixed_ms_per_build"], 3)
+        # XC roofline: 4 G nbf^2 (the two grid GEMMs) + 16 G nbf (rho, grad rho, aow rows) + 300 G
+        # (the functional) flops per build on MFMA (FP32 products in the mixed option, FP64 otherwise);
+        # algorithmic bytes: Phi and grad Phi read once (4 G nbf elements) + D, V (2 nbf^2 doubles)
+        xc_fl = 4.0 * npts * bw.nbf ** 2 + 16.0 * npts * bw.nbf + 300.0 * npts
+        for prec, peak, esz in (("mixed", 157.3, 4), ("fp64", 78.6, 8)):
+            t_ = ks[f"{prec}_xc_ms"] * 1e-3
+            byts = 4.0 * npts * bw.nbf * esz + 2.0 * bw.nbf ** 2 * 8
+            ks[f"{prec}_xc_roofline"] = {"bound": "mfma", "achieved": round(xc_fl / t_ / 1e12, 3), "peak": peak,
+                                         "unit": "TFLOP/s", "frac": round(xc_fl / t_ / 1e12 / peak, 4),
+                                         "flops_model": xc_fl, "algorithmic_GBps": round(byts / t_ / 1e9, 1)}
         ks["dE_mixed_vs_fp64_Ha"] = ks["mixed_two_electron_plus_xc_energy"] - ks["fp64_two_electron_plus_xc_energy"]
         ew.set_options("fp64")
         del ew
@@ -308,6 +344,7 @@ def main():
         t = time.perf_counter()
         res_b, _, jk_ms = be.rhf_scf(cfg_b)
         el_b = time.perf_counter() - t
+        fl_b = sum(x["flops_fp64"] + x["flops_fp32"] for x in be.class_stats())
         iters = [r["iterations"] for r in res_b]
         batch = {"workload": "64 n-alkanes C4H10..C12H26 RHF/6-31G, lockstep batched SCF (level shift 0.1 Ha)",
                  "tiles": "default" if args.no_autotune else "autotuned on an untimed first SCF's density",
@@ -316,6 +353,10 @@ def main():
                  "wall_s": round(el_b, 3), "jk_ms_total": round(jk_ms, 2),
                  "jk_ms_per_iteration": round(jk_ms / max(iters), 3),
                  "molecule_scf_per_s": round(len(bb) / el_b, 3)}
+        t_b = jk_ms / max(iters) * 1e-3
+        batch["roofline"] = {"bound": "valu", "achieved": round(fl_b / t_b / 1e12, 3), "peak": PEAK["f64"],
+                             "unit": "TFLOP/s", "frac": round(fl_b / t_b / 1e12 / PEAK["f64"], 4),
+                             "flops_model_per_build": fl_b}
         del be
         log(f"batch: {batch['molecules']} molecules, {batch['batch_iterations']} iterations, {batch['wall_s']} s, "
             f"{batch['jk_ms_per_iteration']} ms per batched J/K build")
@@ -338,12 +379,17 @@ def main():
         for _ in range(ng):
             eo.eri_gradient(so["D"], 1.0)
         el_e = time.perf_counter() - t
+        fl_g = grad_model_flops(eo.class_stats())
+        t_g = el_e / ng
+        grad_roof = {"bound": "valu", "achieved": round(fl_g / t_g / 1e12, 3), "peak": PEAK["f64"],
+                     "unit": "TFLOP/s", "frac": round(fl_g / t_g / 1e12 / PEAK["f64"], 4),
+                     "flops_model_per_eval": fl_g, "of": "the ERI gradient part"}
         grad = {"workload": "RHF/6-31+G* n-octane C8H18 analytic nuclear gradient at the converged density",
                 "nbf": bo.nbf, "nshell": bo.nshell, "evals": ng,
                 "gradient_evals_per_s": round(ng / el_g, 3), "ms_per_gradient": round(1e3 * el_g / ng, 3),
                 "eri_gradient_ms": round(1e3 * el_e / ng, 3),
                 "max_abs_gradient": float(np.abs(g).max()),
-                "translational_sum": float(np.abs(g.sum(axis=0)).max())}
+                "translational_sum": float(np.abs(g.sum(axis=0)).max()), "roofline": grad_roof}
         log(f"gradient: {grad['ms_per_gradient']} ms/eval (ERI part {grad['eri_gradient_ms']} ms) nbf={bo.nbf}")
 
     out = {

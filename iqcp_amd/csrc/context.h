@@ -51,8 +51,15 @@ struct qcg_context {
     int64_t mat_total = 0;
     // work buffers
     double *d_D = nullptr, *d_J = nullptr, *d_K = nullptr;
-    // J and K accumulators before symmetrisation, fixed-point (fixacc.h): 2 words per element
+    // J and K accumulators before symmetrisation, fixed-point (fixacc.h): one word per element
     fix_t *d_Jacc = nullptr, *d_Kacc = nullptr;
+    // their scales per molecule (4 doubles each), recomputed at the start of every build from the
+    // shell-pair Schwarz matrices (qsh, packed per molecule), the shell of every function and D
+    double *d_fix = nullptr, *d_qsh = nullptr, *d_rowsum = nullptr, *d_qmax = nullptr;
+    int64_t* d_qsh_off = nullptr;
+    int *d_mol_ns = nullptr, *d_bf_shell = nullptr, *d_bf_off = nullptr, *d_row_mol = nullptr;
+    int *d_shell_first = nullptr, *d_shell_first_off = nullptr;
+    FixData fixdata{};
     double *d_H = nullptr, *d_F = nullptr;
     std::vector<void*> owned;
     std::vector<PrimPair<double>> h_pp;  // host copy of the primitive pairs (flop model counts)
@@ -170,7 +177,10 @@ struct qcg_context {
     void free_basis() {
         drop_fock_graph();
         for (void** p : {(void**)&d_pairs, (void**)&d_pp, (void**)&d_ppf, (void**)&d_D, (void**)&d_J, (void**)&d_K,
-                         (void**)&d_Jacc, (void**)&d_Kacc, (void**)&d_H, (void**)&d_F, (void**)&d_mol_nbf,
+                         (void**)&d_Jacc, (void**)&d_Kacc, (void**)&d_fix, (void**)&d_qsh, (void**)&d_rowsum,
+                         (void**)&d_qsh_off, (void**)&d_mol_ns, (void**)&d_bf_shell, (void**)&d_bf_off,
+                         (void**)&d_qmax, (void**)&d_row_mol, (void**)&d_shell_first, (void**)&d_shell_first_off,
+                         (void**)&d_H, (void**)&d_F, (void**)&d_mol_nbf,
                          (void**)&d_mat_off, (void**)&d_ket_off, (void**)&d_ket_cnt, (void**)&d_items,
                          (void**)&d_segs, (void**)&d_grad_items, (void**)&d_grad_segs, (void**)&d_grad,
                          (void**)&d_W, (void**)&d_onee_int, (void**)&d_onee_dbl, (void**)&d_onee_work}) {

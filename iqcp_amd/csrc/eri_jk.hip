@@ -58,17 +58,19 @@ constexpr int rys_lds_doubles() { return N <= 3 ? rys_tab_size<N>() : 1; }
 // One work item = (bra pair, run of its surviving kets over the ket segments, KetCursor); the
 // host builds the items with the exact screening predicate, so no quartet is tested twice or
 // skipped.  rys_n: the coefficients of this class's Rys order (rys_stage); bp: LDS room for the
-// bra's primitive pairs; rows: krow != 0: LDS room for the D rows of the bra functions a and b
-// ((NA + NB) x nbf doubles, read by the K digestion) followed by their K rows (fixed-point pairs,
-// fixacc.h) in which the four K terms of the item (all in the rows of a and b) are summed and then
-// flushed once per block.  J and K go to the fixed-point accumulators Jacc/Kacc, so the result does
-// not depend on the order in which blocks and lanes arrive.
-template <int LA, int LB, int LC, int LD, typename R>
+// bra's primitive pairs; rows: LDS room for the K rows of the bra functions a and b (krow bit 0:
+// (NA + NB) x nbf fixed-point words, fixacc.h, in which the four K terms of the item, all in the
+// rows of a and b, are summed and then flushed once per block) followed by their D rows (bit 1:
+// (NA + NB) x nbf doubles, read by the K digestion).  J and K go to the fixed-point accumulators
+// Jacc/Kacc with the scales fix[4 mol], fix[4 mol + 1], so the result does not depend on the order
+// in which blocks and lanes arrive.
+template <int LA, int LB, int LC, int LD, typename R, int ROWS>
 __device__ __forceinline__ void jk_item(
     const int4 it, const int2* __restrict__ segs, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket, int diag,
     const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_n, const double* __restrict__ D,
-    fix_t* __restrict__ Jacc, fix_t* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
-    const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf, int krow, PrimPair<R>* bp, double* rows) {
+    fix_t* __restrict__ Jacc, fix_t* __restrict__ Kacc, const double* __restrict__ fix, int do_j, int do_k,
+    const int* __restrict__ ket_off, const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf,
+    PrimPair<R>* bp, double* rows) {
 #if !QCG_JK_BRA_LDS
     (void)bp;
 #endif
@@ -79,9 +81,10 @@ __device__ __forceinline__ void jk_item(
     const int pidx = P.pos;
     ket += ket_off[P.mol];
     D += mat_off[P.mol];
-    Jacc += 2 * mat_off[P.mol];
-    Kacc += 2 * mat_off[P.mol];
+    Jacc += mat_off[P.mol];
+    Kacc += mat_off[P.mol];
     const int nbf = mol_nbf[P.mol];
+    const double sJ = fix[4 * P.mol], sK = fix[4 * P.mol + 1];  // fixed-point scales of J and K
     const int tid = threadIdx.x;
 #if QCG_JK_BRA_LDS
     for (int i = tid; i < P.npp; i += blockDim.x) bp[i] = pp[P.pp_off + i];
@@ -89,30 +92,29 @@ __device__ __forceinline__ void jk_item(
 #else
     const PrimPair<R>* ppb = pp + P.pp_off;
 #endif
-    // krow bit 0: K rows in LDS, bit 1: D rows in LDS after them
-    const bool kr = krow & 1, dr = krow & 2;
+    // ROWS bit 0: K rows in LDS, bit 1: D rows in LDS after them (compile-time, so each
+    // instantiation holds one addressing path only)
+    constexpr bool kr = ROWS & 1, dr = ROWS & 2;
     const int nrow = NR * nbf;
     fix_t* kl = reinterpret_cast<fix_t*>(rows);
-    double* dl = rows + (kr ? 2 * nrow : 0);
-    for (int i = tid; i < nrow; i += blockDim.x) {
-        if (dr) {
-            const int r = i / nbf, col = i - r * nbf;
-            dl[i] = D[(size_t)(r < S::NA ? P.bfa + r : P.bfb + (r - S::NA)) * nbf + col];
+    double* dl = rows + (kr ? nrow : 0);
+    if constexpr (ROWS != 0)
+        for (int i = tid; i < nrow; i += blockDim.x) {
+            if constexpr (dr) {
+                const int r = i / nbf, col = i - r * nbf;
+                dl[i] = D[(size_t)(r < S::NA ? P.bfa + r : P.bfb + (r - S::NA)) * nbf + col];
+            }
+            if constexpr (kr) kl[i] = 0;
         }
-        if (kr) {
-            kl[2 * i] = 0;
-            kl[2 * i + 1] = 0;
-        }
-    }
     __syncthreads();
     // D element (row function a + r of the bra's shell a, or b + r of shell b; column col)
     auto drow = [&](bool row_b, int r, int col) -> double {
-        if (dr) return dl[(row_b ? S::NA + r : r) * nbf + col];
-        return D[(size_t)((row_b ? P.bfb : P.bfa) + r) * nbf + col];
+        if constexpr (dr) return dl[(row_b ? S::NA + r : r) * nbf + col];
+        else return D[(size_t)((row_b ? P.bfb : P.bfa) + r) * nbf + col];
     };
     auto kadd = [&](bool row_b, int r, int col, double v) {
-        if (kr) fix_add_lds(kl, (row_b ? S::NA + r : r) * nbf + col, v);
-        else fix_add_global(Kacc, (size_t)((row_b ? P.bfb : P.bfa) + r) * nbf + col, v);
+        if constexpr (kr) fix_add_lds(kl, (row_b ? S::NA + r : r) * nbf + col, v, sK);
+        else fix_add_global(Kacc, (size_t)((row_b ? P.bfb : P.bfa) + r) * nbf + col, v, sK);
     };
 
     // bra density block scaled by the bra component norms
@@ -149,7 +151,7 @@ __device__ __forceinline__ void jk_item(
                 double s = 0.0;
                 for_n<U, S::NAB>([&](int ab) { s += Dps[ab] * (double)eri[ab * S::NCD + cd]; });
                 const int c = cd / S::ND, d = cd % S::ND;
-                fix_add_global(Jacc, (size_t)(Q.bfa + c) * nbf + Q.bfb + d, fj * cnorm(LC, c) * cnorm(LD, d) * s);
+                fix_add_global(Jacc, (size_t)(Q.bfa + c) * nbf + Q.bfb + d, fj * cnorm(LC, c) * cnorm(LD, d) * s, sJ);
             });
         }
         if (do_k) {
@@ -220,22 +222,22 @@ __device__ __forceinline__ void jk_item(
             for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
             if ((tid & 63) == 0 && s != 0.0) {
                 const int a = ab / S::NB, b = ab % S::NB;
-                fix_add_global(Jacc, (size_t)(P.bfa + a) * nbf + P.bfb + b, s);
+                fix_add_global(Jacc, (size_t)(P.bfa + a) * nbf + P.bfb + b, s, sJ);
             }
         });
     }
-    if (kr) {
+    if constexpr (kr) {
         __syncthreads();
         for (int i = tid; i < nrow; i += blockDim.x) {
             const int r = i / nbf, col = i - r * nbf;
             const int row = r < S::NA ? P.bfa + r : P.bfb + (r - S::NA);
-            fix_add_pair_global(Kacc, (size_t)row * nbf + col, kl[2 * i], kl[2 * i + 1]);
+            fix_add_word_global(Kacc, (size_t)row * nbf + col, kl[i]);
         }
     }
 }
 
 // LDS layout of the per-lane kernels' rows for nbf functions (the largest molecule's): K rows
-// as fixed-point pairs, (NA + NB) x nbf x 16 bytes, when they fit the budget left by the static
+// as fixed-point words, (NA + NB) x nbf x 8 bytes, when they fit the budget left by the static
 // LDS (Rys coefficients, bra primitives), then the D rows, (NA + NB) x nbf x 8 bytes, when those
 // fit too.  Returns the krow flags (bit 0 K rows, bit 1 D rows) and the dynamic LDS bytes.
 constexpr size_t kLdsBudget = 65536;
@@ -243,14 +245,14 @@ template <int LA, int LB, int LC, int LD>
 inline int jk_row_flags(int nbf, int do_k, int bra_pp, size_t* bytes) {
     using S = EriShape<LA, LB, LC, LD>;
     const size_t fixed = (size_t)rys_lds_doubles<S::NROOTS>() * sizeof(double) + (size_t)bra_pp * sizeof(PrimPair<double>);
-    const size_t kb = (size_t)(S::NA + S::NB) * nbf * 2 * sizeof(fix_t), db = (size_t)(S::NA + S::NB) * nbf * sizeof(double);
+    const size_t kb = (size_t)(S::NA + S::NB) * nbf * sizeof(fix_t), db = (size_t)(S::NA + S::NB) * nbf * sizeof(double);
     int f = 0;
     size_t n = 0;
     if (do_k && fixed + kb <= kLdsBudget) {
         f |= 1;
         n += kb;
     }
-    if (QCG_JK_DROWS && fixed + n + db <= kLdsBudget) {
+    if (QCG_JK_DROWS && (f & 1) && fixed + n + db <= kLdsBudget) {
         f |= 2;
         n += db;
     }
@@ -258,42 +260,46 @@ inline int jk_row_flags(int nbf, int do_k, int bra_pp, size_t* bytes) {
     return f;
 }
 
-template <int LA, int LB, int LC, int LD, typename R>
+template <int LA, int LB, int LC, int LD, typename R, int ROWS>
 __global__ __launch_bounds__(256) void jk_kernel(
     const int4* __restrict__ items, const int2* __restrict__ segs, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket, int diag,
     const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_tab, const double* __restrict__ D,
-    fix_t* __restrict__ Jacc, fix_t* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
-    const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf, int krow, int bra_pp) {
+    fix_t* __restrict__ Jacc, fix_t* __restrict__ Kacc, const double* __restrict__ fix, int do_j, int do_k,
+    const int* __restrict__ ket_off, const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf,
+    int bra_pp) {
     constexpr int NR = EriShape<LA, LB, LC, LD>::NROOTS;
     extern __shared__ double dyn[];  // bra primitive pairs (bra_pp), then the K / D rows
     __shared__ double rys_raw[rys_lds_doubles<NR>()];
     const R* rys_n = rys_stage<R, NR>(rys_tab, reinterpret_cast<R*>(rys_raw));
-    jk_item<LA, LB, LC, LD, R>(items[blockIdx.x], segs, bra, ket, diag, pp, rys_n, D, Jacc, Kacc, do_j, do_k, ket_off,
-                               mat_off, mol_nbf, krow, reinterpret_cast<PrimPair<R>*>(dyn),
+    jk_item<LA, LB, LC, LD, R, ROWS>(items[blockIdx.x], segs, bra, ket, diag, pp, rys_n, D, Jacc, Kacc, fix, do_j, do_k,
+                                     ket_off, mat_off, mol_nbf, reinterpret_cast<PrimPair<R>*>(dyn),
                                dyn + bra_pp * sizeof(PrimPair<R>) / sizeof(double));
 }
 
 // Mixed precision in one launch: items [0, n64) are FP64 quartets, the rest FP32 (one branch per
 // block); saves the second launch per class and its tail.
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, int ROWS>
 __global__ __launch_bounds__(256) void jk_kernel_mp(
     const int4* __restrict__ items, const int2* __restrict__ segs, int n64, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket,
     int diag, const PrimPair<double>* __restrict__ pp, const PrimPair<float>* __restrict__ ppf,
     const double* __restrict__ rys_tab, const float* __restrict__ rys_tabf, const double* __restrict__ D,
-    fix_t* __restrict__ Jacc, fix_t* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
-    const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf, int krow, int bra_pp) {
+    fix_t* __restrict__ Jacc, fix_t* __restrict__ Kacc, const double* __restrict__ fix, int do_j, int do_k,
+    const int* __restrict__ ket_off, const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf,
+    int bra_pp) {
     constexpr int NR = EriShape<LA, LB, LC, LD>::NROOTS;
     extern __shared__ double dyn[];  // bra primitive pairs (bra_pp FP64 slots), then the K / D rows
     __shared__ double rys_raw[rys_lds_doubles<NR>()];
     double* rows = dyn + bra_pp * sizeof(PrimPair<double>) / sizeof(double);
     if ((int)blockIdx.x < n64) {
         const double* rys_n = rys_stage<double, NR>(rys_tab, rys_raw);
-        jk_item<LA, LB, LC, LD, double>(items[blockIdx.x], segs, bra, ket, diag, pp, rys_n, D, Jacc, Kacc, do_j, do_k,
-                                        ket_off, mat_off, mol_nbf, krow, reinterpret_cast<PrimPair<double>*>(dyn), rows);
+        jk_item<LA, LB, LC, LD, double, ROWS>(items[blockIdx.x], segs, bra, ket, diag, pp, rys_n, D, Jacc, Kacc, fix,
+                                              do_j, do_k, ket_off, mat_off, mol_nbf,
+                                              reinterpret_cast<PrimPair<double>*>(dyn), rows);
     } else {
         const float* rys_n = rys_stage<float, NR>(rys_tabf, reinterpret_cast<float*>(rys_raw));
-        jk_item<LA, LB, LC, LD, float>(items[blockIdx.x], segs, bra, ket, diag, ppf, rys_n, D, Jacc, Kacc, do_j, do_k,
-                                       ket_off, mat_off, mol_nbf, krow, reinterpret_cast<PrimPair<float>*>(dyn), rows);
+        jk_item<LA, LB, LC, LD, float, ROWS>(items[blockIdx.x], segs, bra, ket, diag, ppf, rys_n, D, Jacc, Kacc, fix,
+                                             do_j, do_k, ket_off, mat_off, mol_nbf,
+                                             reinterpret_cast<PrimPair<float>*>(dyn), rows);
     }
 }
 
@@ -320,8 +326,8 @@ template <int LA, int LB, int LC, int LD, typename R>
 __device__ __forceinline__ void jk_wave_item(
     const int4 it, const int2* __restrict__ segs, int lane, double* V, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket,
     int diag, const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_tab, const double* __restrict__ D,
-    fix_t* __restrict__ Jacc, fix_t* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
-    const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf) {
+    fix_t* __restrict__ Jacc, fix_t* __restrict__ Kacc, const double* __restrict__ fix, int do_j, int do_k,
+    const int* __restrict__ ket_off, const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf) {
     using S = EriShape<LA, LB, LC, LD>;
     using W = JkWave<LA, LB, LC, LD, R>;
     using TB = typename W::T;
@@ -339,9 +345,10 @@ __device__ __forceinline__ void jk_wave_item(
     const int pidx = P.pos;
     ket += ket_off[P.mol];
     D += mat_off[P.mol];
-    Jacc += 2 * mat_off[P.mol];
-    Kacc += 2 * mat_off[P.mol];
+    Jacc += mat_off[P.mol];
+    Kacc += mat_off[P.mol];
     const size_t nbf = (size_t)mol_nbf[P.mol];
+    const double sJ = fix[4 * P.mol], sK = fix[4 * P.mol + 1];  // fixed-point scales of J and K
 
     constexpr const CompTab<LA, LB, LC, LD>& ct = kCompTab<LA, LB, LC, LD>;
     int ox[KPL], oy[KPL], oz[KPL];
@@ -405,7 +412,7 @@ __device__ __forceinline__ void jk_wave_item(
             for (int o = lane; o < NCD; o += 64) {
                 double s = 0.0;
                 for (int ab = 0; ab < S::NAB; ++ab) s += Dab[ab] * V[ab * NCD + o];
-                fix_add_global(Jacc, (Q.bfa + o / ND) * nbf + Q.bfb + o % ND, 2.0 * s);
+                fix_add_global(Jacc, (Q.bfa + o / ND) * nbf + Q.bfb + o % ND, 2.0 * s, sJ);
             }
         }
         if (do_k) {
@@ -414,48 +421,48 @@ __device__ __forceinline__ void jk_wave_item(
                 double s = 0.0;
                 for (int b = 0; b < NB; ++b)
                     for (int d = 0; d < ND; ++d) s += Dbd[b * ND + d] * VAT(a, b, c, d);
-                fix_add_global(Kacc, (P.bfa + a) * nbf + Q.bfa + c, s);
+                fix_add_global(Kacc, (P.bfa + a) * nbf + Q.bfa + c, s, sK);
             }
             for (int o = lane; o < NB * NC; o += 64) {
                 const int b = o / NC, c = o % NC;
                 double s = 0.0;
                 for (int a = 0; a < NA; ++a)
                     for (int d = 0; d < ND; ++d) s += Dad[a * ND + d] * VAT(a, b, c, d);
-                fix_add_global(Kacc, (P.bfb + b) * nbf + Q.bfa + c, s);
+                fix_add_global(Kacc, (P.bfb + b) * nbf + Q.bfa + c, s, sK);
             }
             for (int o = lane; o < NA * ND; o += 64) {
                 const int a = o / ND, d = o % ND;
                 double s = 0.0;
                 for (int b = 0; b < NB; ++b)
                     for (int c = 0; c < NC; ++c) s += Dbc[b * NC + c] * VAT(a, b, c, d);
-                fix_add_global(Kacc, (P.bfa + a) * nbf + Q.bfb + d, s);
+                fix_add_global(Kacc, (P.bfa + a) * nbf + Q.bfb + d, s, sK);
             }
             for (int o = lane; o < NB * ND; o += 64) {
                 const int b = o / ND, d = o % ND;
                 double s = 0.0;
                 for (int a = 0; a < NA; ++a)
                     for (int c = 0; c < NC; ++c) s += Dac[a * NC + c] * VAT(a, b, c, d);
-                fix_add_global(Kacc, (P.bfb + b) * nbf + Q.bfb + d, s);
+                fix_add_global(Kacc, (P.bfb + b) * nbf + Q.bfb + d, s, sK);
             }
         }
 #undef VAT
     }
-    if (do_j && lane < S::NAB && jp != 0.0) fix_add_global(Jacc, (P.bfa + lane / NB) * nbf + P.bfb + lane % NB, jp);
+    if (do_j && lane < S::NAB && jp != 0.0) fix_add_global(Jacc, (P.bfa + lane / NB) * nbf + P.bfb + lane % NB, jp, sJ);
 }
 
 template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(256) void jk_wave_kernel(
     const int4* __restrict__ items, const int2* __restrict__ segs, int nitems, const PairRec* __restrict__ bra, const PairRec* __restrict__ ket,
     int diag, const PrimPair<R>* __restrict__ pp, const R* __restrict__ rys_tab, const double* __restrict__ D,
-    fix_t* __restrict__ Jacc, fix_t* __restrict__ Kacc, int do_j, int do_k, const int* __restrict__ ket_off,
-    const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf) {
+    fix_t* __restrict__ Jacc, fix_t* __restrict__ Kacc, const double* __restrict__ fix, int do_j, int do_k,
+    const int* __restrict__ ket_off, const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf) {
     constexpr int WD = JkWave<LA, LB, LC, LD, R>::WAVE_D;
     __shared__ double smem[4 * WD];
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int item = blockIdx.x * 4 + wid;
     if (item >= nitems) return;  // no block-wide barriers in the item: waves are independent
     jk_wave_item<LA, LB, LC, LD, R>(items[item], segs, lane, smem + wid * WD, bra, ket, diag, pp, rys_tab, D, Jacc, Kacc,
-                                    do_j, do_k, ket_off, mat_off, mol_nbf);
+                                    fix, do_j, do_k, ket_off, mat_off, mol_nbf);
 }
 
 // Mixed precision in one launch: items [0, n64) FP64, the rest FP32 (one branch per wavefront).
@@ -464,8 +471,9 @@ __global__ __launch_bounds__(256) void jk_wave_kernel_mp(
     const int4* __restrict__ items, const int2* __restrict__ segs, int nitems, int n64, const PairRec* __restrict__ bra,
     const PairRec* __restrict__ ket, int diag, const PrimPair<double>* __restrict__ pp,
     const PrimPair<float>* __restrict__ ppf, const double* __restrict__ rys_tab, const float* __restrict__ rys_tabf,
-    const double* __restrict__ D, fix_t* __restrict__ Jacc, fix_t* __restrict__ Kacc, int do_j, int do_k,
-    const int* __restrict__ ket_off, const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf) {
+    const double* __restrict__ D, fix_t* __restrict__ Jacc, fix_t* __restrict__ Kacc, const double* __restrict__ fix,
+    int do_j, int do_k, const int* __restrict__ ket_off, const int64_t* __restrict__ mat_off,
+    const int* __restrict__ mol_nbf) {
     constexpr int WD64 = JkWave<LA, LB, LC, LD, double>::WAVE_D, WD32 = JkWave<LA, LB, LC, LD, float>::WAVE_D;
     constexpr int WD = WD64 > WD32 ? WD64 : WD32;
     __shared__ double smem[4 * WD];
@@ -474,10 +482,10 @@ __global__ __launch_bounds__(256) void jk_wave_kernel_mp(
     if (item >= nitems) return;
     if (item < n64)
         jk_wave_item<LA, LB, LC, LD, double>(items[item], segs, lane, smem + wid * WD, bra, ket, diag, pp, rys_tab, D, Jacc,
-                                             Kacc, do_j, do_k, ket_off, mat_off, mol_nbf);
+                                             Kacc, fix, do_j, do_k, ket_off, mat_off, mol_nbf);
     else
         jk_wave_item<LA, LB, LC, LD, float>(items[item], segs, lane, smem + wid * WD, bra, ket, diag, ppf, rys_tabf, D, Jacc,
-                                            Kacc, do_j, do_k, ket_off, mat_off, mol_nbf);
+                                            Kacc, fix, do_j, do_k, ket_off, mat_off, mol_nbf);
 }
 
 __device__ __forceinline__ int64_t pair_index64(int64_t i, int64_t j) {
@@ -536,7 +544,7 @@ static hipError_t launch_jk_t(const JkLaunch& a, hipStream_t st) {
         if constexpr (JkWave<LA, LB, LC, LD, R>::kEnabled) {
             hipLaunchKernelGGL((jk_wave_kernel<LA, LB, LC, LD, R>), dim3((a.nitems + 3) / 4), dim3(256), 0, st,
                                a.items, a.segs, a.nitems, a.bra, a.ket, a.diag, (const PrimPair<R>*)a.pp, (const R*)a.rys,
-                               a.D, a.Jacc, a.Kacc, a.do_j, a.do_k, a.ket_off, a.mat_off, a.mol_nbf);
+                               a.D, a.Jacc, a.Kacc, a.fix, a.do_j, a.do_k, a.ket_off, a.mat_off, a.mol_nbf);
             return hipGetLastError();
         }
     }
@@ -545,10 +553,16 @@ static hipError_t launch_jk_t(const JkLaunch& a, hipStream_t st) {
         size_t dyn_bytes = 0;
         const int bra_pp = QCG_JK_BRA_LDS ? a.bra_pp : 0;
         const int krow = jk_row_flags<LA, LB, LC, LD>(a.nbf, a.do_k, bra_pp, &dyn_bytes);
-        hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, R>), dim3(a.nitems), dim3(a.block > 0 ? a.block : 64),
-                           dyn_bytes, st, a.items, a.segs, a.bra, a.ket, a.diag, (const PrimPair<R>*)a.pp,
-                           (const R*)a.rys, a.D, a.Jacc, a.Kacc, a.do_j, a.do_k, a.ket_off, a.mat_off, a.mol_nbf,
-                           krow, bra_pp);
+        auto go = [&](auto rows_c) {
+            constexpr int ROWS = decltype(rows_c)::value;
+            hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, R, ROWS>), dim3(a.nitems), dim3(a.block > 0 ? a.block : 64),
+                               dyn_bytes, st, a.items, a.segs, a.bra, a.ket, a.diag, (const PrimPair<R>*)a.pp,
+                               (const R*)a.rys, a.D, a.Jacc, a.Kacc, a.fix, a.do_j, a.do_k, a.ket_off, a.mat_off,
+                               a.mol_nbf, bra_pp);
+        };
+        if (krow == 3) go(std::integral_constant<int, 3>{});
+        else if (krow == 1) go(std::integral_constant<int, 1>{});
+        else go(std::integral_constant<int, 0>{});
         return hipGetLastError();
     } else {
         return hipErrorInvalidValue;  // not reached: large classes take the wave-cooperative form above
@@ -568,7 +582,7 @@ static hipError_t launch_jk_mp_t(const JkLaunch& a, hipStream_t st) {
             hipLaunchKernelGGL((jk_wave_kernel_mp<LA, LB, LC, LD>), dim3((a.nitems + 3) / 4), dim3(256), 0, st,
                                a.items, a.segs, a.nitems, a.n64, a.bra, a.ket, a.diag, (const PrimPair<double>*)a.pp,
                                (const PrimPair<float>*)a.ppf, (const double*)a.rys, (const float*)a.rysf, a.D, a.Jacc,
-                               a.Kacc, a.do_j, a.do_k, a.ket_off, a.mat_off, a.mol_nbf);
+                               a.Kacc, a.fix, a.do_j, a.do_k, a.ket_off, a.mat_off, a.mol_nbf);
             return hipGetLastError();
         }
     }
@@ -576,11 +590,17 @@ static hipError_t launch_jk_mp_t(const JkLaunch& a, hipStream_t st) {
         size_t dyn_bytes = 0;
         const int bra_pp = QCG_JK_BRA_LDS ? a.bra_pp : 0;
         const int krow = jk_row_flags<LA, LB, LC, LD>(a.nbf, a.do_k, bra_pp, &dyn_bytes);
-        hipLaunchKernelGGL((jk_kernel_mp<LA, LB, LC, LD>), dim3(a.nitems), dim3(a.block > 0 ? a.block : 64),
-                           dyn_bytes, st, a.items, a.segs, a.n64, a.bra, a.ket, a.diag,
-                           (const PrimPair<double>*)a.pp, (const PrimPair<float>*)a.ppf, (const double*)a.rys,
-                           (const float*)a.rysf, a.D, a.Jacc, a.Kacc, a.do_j, a.do_k, a.ket_off, a.mat_off, a.mol_nbf,
-                           krow, bra_pp);
+        auto go = [&](auto rows_c) {
+            constexpr int ROWS = decltype(rows_c)::value;
+            hipLaunchKernelGGL((jk_kernel_mp<LA, LB, LC, LD, ROWS>), dim3(a.nitems), dim3(a.block > 0 ? a.block : 64),
+                               dyn_bytes, st, a.items, a.segs, a.n64, a.bra, a.ket, a.diag,
+                               (const PrimPair<double>*)a.pp, (const PrimPair<float>*)a.ppf, (const double*)a.rys,
+                               (const float*)a.rysf, a.D, a.Jacc, a.Kacc, a.fix, a.do_j, a.do_k, a.ket_off, a.mat_off,
+                               a.mol_nbf, bra_pp);
+        };
+        if (krow == 3) go(std::integral_constant<int, 3>{});
+        else if (krow == 1) go(std::integral_constant<int, 1>{});
+        else go(std::integral_constant<int, 0>{});
         return hipGetLastError();
     } else {
         return hipErrorInvalidValue;  // not reached: large classes take the wave-cooperative form above

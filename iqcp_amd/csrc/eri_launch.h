@@ -103,8 +103,9 @@ struct JkLaunch {
     const void* rys;    // Rys table (double or float)
     const double* D;    // density, nbf x nbf, symmetric
     int nbf;
-    fix_t* Jacc;        // fixed-point accumulators (fixacc.h), 2 words per element
+    fix_t* Jacc;        // fixed-point accumulators (fixacc.h), one word per element
     fix_t* Kacc;
+    const double* fix;  // per molecule: scale of J, scale of K, and their inverses (fix_scale_kernel)
     double thr_lo;      // process quartets with thr_lo <= Q_P Q_Q < thr_hi
     double thr_hi;
     int do_j, do_k;
@@ -217,11 +218,36 @@ inline hipError_t launch_grad(int cb, int ck, const GradLaunch& a, hipStream_t s
 hipError_t launch_schwarz(int c, const PairRec* pairs, int n, const PrimPair<double>* pp, const double* rys,
                           double* Q, hipStream_t st);
 hipError_t launch_fill(double* x, size_t n, double v, hipStream_t st);
-hipError_t launch_symmetrize(const fix_t* Jacc, const fix_t* Kacc, double* J, double* K, int nbf, hipStream_t st);
+// What the fixed-point scale kernels read (device pointers, packed per molecule): the Schwarz bounds
+// of the shell pairs as nshell x nshell matrices (qsh, offsets qsh_off), their maximum (qmax), the
+// shell of every function (bf_shell, offsets bf_off into the functions of all molecules), the first
+// function of every shell plus a sentinel (shell_first, offsets shell_first_off), the molecule of
+// every function (row_mol, nrows functions in all) and 2 doubles of scratch per function (rowsum).
+struct FixData {
+    const int64_t* mat_off;
+    const int* mol_nbf;
+    const double* qsh;
+    const int64_t* qsh_off;
+    const int* mol_ns;
+    const int* bf_shell;
+    const int* bf_off;
+    const int* row_mol;
+    int nrows;
+    double* rowsum;
+    const double* qmax;
+    const int* shell_first;
+    const int* shell_first_off;
+};
+// Per molecule m the fixed-point scales fix[4m] (J) and fix[4m + 1] (K) and their inverses from a
+// bound on the accumulated magnitudes at density D: B_J = 2 Qmax sum_ls Q_ls |D_ls|,
+// B_K = 2 Qmax max_s sum_t Q_st sum_(n in t) sum_l |D_nl|.
+hipError_t launch_fix_scale(const double* D, const FixData& f, double* fix, int nmol, hipStream_t st);
+hipError_t launch_symmetrize(const fix_t* Jacc, const fix_t* Kacc, const double* fix, double* J, double* K, int nbf,
+                             hipStream_t st);
 // batched J = Jacc + Jacc^T, K = Kacc + Kacc^T over block-diagonal packed matrices
-hipError_t launch_symmetrize_batch(const fix_t* Jacc, const fix_t* Kacc, double* J, double* K, int nmol,
+hipError_t launch_symmetrize_batch(const fix_t* Jacc, const fix_t* Kacc, const double* fix, double* J, double* K, int nmol,
                                    const int64_t* mat_off, const int* mol_nbf, int max_nbf, hipStream_t st);
-hipError_t launch_fock_combine(const double* H, const fix_t* Jacc, const fix_t* Kacc, double* F, int nbf,
+hipError_t launch_fock_combine(const double* H, const fix_t* Jacc, const fix_t* Kacc, const double* fix, double* F, int nbf,
                                double kfac, hipStream_t st);
 hipError_t launch_boys_batch(int m, int n, const double* T, double* F, int* status, hipStream_t st);
 hipError_t launch_rys_batch(int nroots, int n, const double* T, const double* tab, double* x, double* w,

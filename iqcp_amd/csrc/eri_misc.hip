@@ -55,58 +55,158 @@ hipError_t launch_fill(double* x, size_t n, double v, hipStream_t st) {
     return hipGetLastError();
 }
 
+// Fixed-point scales of one build (fixacc.h).  Every J contribution to element (m, n) is a product
+// (mn|ls) D_ls with |(mn|ls)| <= Q_mn Q_ls (Schwarz, per component), so
+// |Jacc_mn| <= Qmax sum_ls Q_ls |D_ls| (counted twice for the symmetry images: B_J); a K contribution
+// to (m, l) is (mn|ls) D_ns, so |Kacc_ml| <= Qmax sum_t Q_sh(m),t rho_t with rho_t = sum over the
+// functions n of shell t of sum_s |D_ns| (twice: B_K).  Two kernels: one wavefront per row of D for
+// the row sums, then one block per molecule for the shell sums and the scales.  Every reduction runs
+// in a fixed order, so the scales, like the sums they serve, are the same on every build of the same
+// density.
+__global__ __launch_bounds__(256) void fix_rows_kernel(
+    const double* __restrict__ D, const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf,
+    const double* __restrict__ qsh, const int64_t* __restrict__ qsh_off, const int* __restrict__ mol_ns,
+    const int* __restrict__ bf_shell, const int* __restrict__ bf_off, const int* __restrict__ row_mol, int nrows,
+    double* __restrict__ rowsum) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (row >= nrows) return;
+    const int m = row_mol[row], n = row - bf_off[m], nbf = mol_nbf[m], ns = mol_ns[m];
+    const int* bs = bf_shell + bf_off[m];
+    const double* Drow = D + mat_off[m] + (size_t)n * nbf;
+    const double* Qrow = qsh + qsh_off[m] + (size_t)bs[n] * ns;
+    double r = 0.0, j = 0.0;
+    for (int l = lane; l < nbf; l += 64) {
+        const double a = fabs(Drow[l]);
+        r += a;
+        j += Qrow[bs[l]] * a;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        r += __shfl_xor(r, off, 64);
+        j += __shfl_xor(j, off, 64);
+    }
+    if (lane == 0) {
+        rowsum[2 * row] = r;
+        rowsum[2 * row + 1] = j;
+    }
+}
+
+constexpr int kFixMaxShells = 4096;  // shells per molecule the scale kernel handles (rho in LDS)
+
+__global__ __launch_bounds__(1024) void fix_scale_kernel(FixData f, double* __restrict__ fix) {
+    __shared__ double red[1024];
+    __shared__ double rho[kFixMaxShells];
+    const int m = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nbf = f.mol_nbf[m], ns = f.mol_ns[m];
+    const double* Q = f.qsh + f.qsh_off[m];
+    const int* first = f.shell_first + f.shell_first_off[m];
+    const double* rs = f.rowsum + 2 * (size_t)f.bf_off[m];
+    // shell row sums rho_t over the shell's consecutive functions, and sum_n jrow_n
+    double aj = 0.0;
+    for (int n = tid; n < nbf; n += blockDim.x) aj += rs[2 * n + 1];
+    for (int t = tid; t < ns; t += blockDim.x) {
+        double v = 0.0;
+        for (int n = first[t]; n < first[t + 1]; ++n) v += rs[2 * n];
+        rho[t] = v;
+    }
+    red[tid] = aj;
+    __syncthreads();
+    for (int h = 512; h > 0; h >>= 1) {
+        if (tid < h) red[tid] += red[tid + h];
+        __syncthreads();
+    }
+    const double sumJ = red[0];
+    __syncthreads();
+    // max over shells s of sum_t Q_st rho_t: one wavefront per shell s, lanes over t
+    double km = 0.0;
+    for (int sh = wave; sh < ns; sh += 16) {
+        double t = 0.0;
+        for (int u = lane; u < ns; u += 64) t += Q[(size_t)sh * ns + u] * rho[u];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) t += __shfl_xor(t, off, 64);
+        km = fmax(km, t);
+    }
+    red[tid] = km;
+    __syncthreads();
+    for (int h = 512; h > 0; h >>= 1) {
+        if (tid < h) red[tid] = fmax(red[tid], red[tid + h]);
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const double qmax = f.qmax[m];
+        const double sj = fix_scale_for(2.0 * qmax * sumJ), sk = fix_scale_for(2.0 * qmax * red[0]);
+        fix[4 * m] = sj;
+        fix[4 * m + 1] = sk;
+        fix[4 * m + 2] = 1.0 / sj;
+        fix[4 * m + 3] = 1.0 / sk;
+    }
+}
+
+hipError_t launch_fix_scale(const double* D, const FixData& f, double* fix, int nmol, hipStream_t st) {
+    hipLaunchKernelGGL(fix_rows_kernel, dim3((f.nrows + 3) / 4), dim3(256), 0, st, D, f.mat_off, f.mol_nbf, f.qsh,
+                       f.qsh_off, f.mol_ns, f.bf_shell, f.bf_off, f.row_mol, f.nrows, f.rowsum);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(fix_scale_kernel, dim3(nmol), dim3(1024), 0, st, f, fix);
+    return hipGetLastError();
+}
+
 // Element (i, j) of A + A^T for a fixed-point accumulator: the integer words are summed before
 // the conversion, so the result is exactly symmetric.
-__device__ __forceinline__ double fix_sym(const fix_t* __restrict__ a, size_t ij, size_t ji) {
-    return fix_value(a[2 * ij] + a[2 * ji], a[2 * ij + 1] + a[2 * ji + 1]);
+__device__ __forceinline__ double fix_sym(const fix_t* __restrict__ a, size_t ij, size_t ji, double inv) {
+    return fix_value(a[ij] + a[ji], inv);
 }
 
 // J = Jacc + Jacc^T, K = Kacc + Kacc^T
 __global__ void symmetrize_kernel(const fix_t* __restrict__ Jacc, const fix_t* __restrict__ Kacc,
-                                  double* __restrict__ J, double* __restrict__ K, int nbf) {
+                                  const double* __restrict__ fix, double* __restrict__ J, double* __restrict__ K,
+                                  int nbf) {
     const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nbf) return;
     const size_t ij = (size_t)i * nbf + j, ji = (size_t)j * nbf + i;
-    if (J) J[ij] = fix_sym(Jacc, ij, ji);
-    if (K) K[ij] = fix_sym(Kacc, ij, ji);
+    if (J) J[ij] = fix_sym(Jacc, ij, ji, fix[2]);
+    if (K) K[ij] = fix_sym(Kacc, ij, ji, fix[3]);
 }
 
 __global__ void symmetrize_batch_kernel(const fix_t* __restrict__ Jacc, const fix_t* __restrict__ Kacc,
-                                        double* __restrict__ J, double* __restrict__ K,
-                                        const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf) {
+                                        const double* __restrict__ fix, double* __restrict__ J,
+                                        double* __restrict__ K, const int64_t* __restrict__ mat_off,
+                                        const int* __restrict__ mol_nbf) {
     const int m = blockIdx.z, n = mol_nbf[m];
     const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || j >= n) return;
     const size_t o = (size_t)mat_off[m];
     const size_t ij = o + (size_t)i * n + j, ji = o + (size_t)j * n + i;
-    if (J) J[ij] = fix_sym(Jacc, ij, ji);
-    if (K) K[ij] = fix_sym(Kacc, ij, ji);
+    if (J) J[ij] = fix_sym(Jacc, ij, ji, fix[4 * m + 2]);
+    if (K) K[ij] = fix_sym(Kacc, ij, ji, fix[4 * m + 3]);
 }
 
-hipError_t launch_symmetrize_batch(const fix_t* Jacc, const fix_t* Kacc, double* J, double* K, int nmol,
-                                   const int64_t* mat_off, const int* mol_nbf, int max_nbf, hipStream_t st) {
+hipError_t launch_symmetrize_batch(const fix_t* Jacc, const fix_t* Kacc, const double* fix, double* J, double* K,
+                                   int nmol, const int64_t* mat_off, const int* mol_nbf, int max_nbf, hipStream_t st) {
     hipLaunchKernelGGL(symmetrize_batch_kernel, dim3((max_nbf + 127) / 128, max_nbf, nmol), dim3(128), 0, st, Jacc,
-                       Kacc, J, K, mat_off, mol_nbf);
+                       Kacc, fix, J, K, mat_off, mol_nbf);
     return hipGetLastError();
 }
 
-hipError_t launch_symmetrize(const fix_t* Jacc, const fix_t* Kacc, double* J, double* K, int nbf, hipStream_t st) {
-    hipLaunchKernelGGL(symmetrize_kernel, dim3((nbf + 127) / 128, nbf), dim3(128), 0, st, Jacc, Kacc, J, K, nbf);
+hipError_t launch_symmetrize(const fix_t* Jacc, const fix_t* Kacc, const double* fix, double* J, double* K, int nbf,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(symmetrize_kernel, dim3((nbf + 127) / 128, nbf), dim3(128), 0, st, Jacc, Kacc, fix, J, K, nbf);
     return hipGetLastError();
 }
 
 // F = H + J - kfac K  (build_fock, scf/mod.rs:825-881 with kfac = 1/2; hybrid form dft/ks_scf.rs:1960)
 __global__ void fock_combine_kernel(const double* __restrict__ H, const fix_t* __restrict__ Jacc,
-                                    const fix_t* __restrict__ Kacc, double* __restrict__ F, int nbf, double kfac) {
+                                    const fix_t* __restrict__ Kacc, const double* __restrict__ fix,
+                                    double* __restrict__ F, int nbf, double kfac) {
     const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nbf) return;
     const size_t ij = (size_t)i * nbf + j, ji = (size_t)j * nbf + i;
-    F[ij] = H[ij] + fix_sym(Jacc, ij, ji) - kfac * fix_sym(Kacc, ij, ji);
+    F[ij] = H[ij] + fix_sym(Jacc, ij, ji, fix[2]) - kfac * fix_sym(Kacc, ij, ji, fix[3]);
 }
 
-hipError_t launch_fock_combine(const double* H, const fix_t* Jacc, const fix_t* Kacc, double* F, int nbf,
-                               double kfac, hipStream_t st) {
-    hipLaunchKernelGGL(fock_combine_kernel, dim3((nbf + 127) / 128, nbf), dim3(128), 0, st, H, Jacc, Kacc, F, nbf,
+hipError_t launch_fock_combine(const double* H, const fix_t* Jacc, const fix_t* Kacc, const double* fix, double* F,
+                               int nbf, double kfac, hipStream_t st) {
+    hipLaunchKernelGGL(fock_combine_kernel, dim3((nbf + 127) / 128, nbf), dim3(128), 0, st, H, Jacc, Kacc, fix, F, nbf,
                        kfac);
     return hipGetLastError();
 }

@@ -1,16 +1,15 @@
 // Order-independent (bitwise reproducible) accumulation of the J and K matrices.
 //
 // Every contribution v (an FP64 value produced in a fixed order by one lane or one wavefront) is
-// split into two integers, h = round(v 2^30) and l = round((v - h 2^-30) 2^61), which are added with
-// 64-bit integer atomics into two words per matrix element.  Integer addition is associative, so
-// the sums, and the value read back as H 2^-30 + L 2^-61, do not depend on the order in which the
-// blocks, waves or lanes of a launch (or of concurrent launches) arrive.  The split is exact for
-// |v| < 2^20; a single contribution's rounding is at most 2^-62 (2e-19) absolute, below an FP64
-// addition's rounding for any sum larger than 1e-3.  Element totals must stay below 2^32 in
-// magnitude (H is an int64 of 2^-30 units); J and K elements of the molecules here are O(10^2).
+// rounded to an integer multiple of 2^-S, w = round(v 2^S), and added with a 64-bit integer atomic.
+// Integer addition is associative (two's-complement wrap-around included), so the sums, and the
+// value read back as W 2^-S, do not depend on the order in which the blocks, waves or lanes of a
+// launch, or of concurrent launches, arrive.
 //
-// Layout: element e occupies words [2e] (h) and [2e + 1] (l), so both atomics of one contribution
-// touch the same 16-byte line.
+// S is chosen per build and molecule from a bound B on the accumulated magnitudes (fix_scale_kernel,
+// eri_misc.hip): the largest S with 2^S B < 2^61, separately for J and K.  For the molecules here B is
+// 2^9..2^11, so S is 50..52 and one contribution rounds by at most 2^-51..2^-53 (1e-16 absolute),
+// which is what one FP64 addition into a sum of magnitude 1 rounds by.
 #pragma once
 #include "qcg_common.h"
 
@@ -18,47 +17,36 @@ namespace qcg {
 
 using fix_t = unsigned long long;
 
-constexpr double kFixMagic = 6755399441055744.0;  // 1.5 * 2^52: adding it rounds to an integer
+// round(v * scale) as a two's-complement 64-bit word (|v * scale| < 2^62)
+QCG_HD fix_t fix_word(double v, double scale) { return (fix_t)(long long)rint(v * scale); }
 
-// h and l of v: fma(v, 2^30, 1.5 2^52) rounds v 2^30 to the nearest integer (|v 2^30| < 2^51) and the
-// integer sits in the low mantissa bits, read as the bit pattern's offset from 1.5 2^52.
-QCG_HD void fix_split(double v, fix_t& h, fix_t& l) {
-    const double m = fma(v, 0x1p30, kFixMagic);
-    const double hd = m - kFixMagic;
-    const double r = fma(-hd, 0x1p-30, v);  // exact: |r| <= 2^-31
-    const double m2 = fma(r, 0x1p61, kFixMagic);
-    h = (fix_t)(__builtin_bit_cast(long long, m) - __builtin_bit_cast(long long, kFixMagic));
-    l = (fix_t)(__builtin_bit_cast(long long, m2) - __builtin_bit_cast(long long, kFixMagic));
-}
+// the accumulated value of a word (inv_scale = 2^-S)
+QCG_HD double fix_value(fix_t w, double inv_scale) { return (double)(long long)w * inv_scale; }
 
-QCG_HD double fix_value(fix_t h, fix_t l) {
-    return (double)(long long)h * 0x1p-30 + (double)(long long)l * 0x1p-61;
+// 2^S for a bound B on the magnitudes: the largest S <= 60 with 2^S B < 2^61 (S >= 0)
+QCG_HD double fix_scale_for(double B) {
+    int e = 0;
+    (void)frexp(B > 0.0 ? B : 1.0, &e);  // B < 2^e
+    int S = 61 - e;
+    S = S > 60 ? 60 : (S < 0 ? 0 : S);
+    return ldexp(1.0, S);
 }
 
 #if defined(__HIPCC__)
-// acc[e] += v in device memory (agent scope; no return value, so no wait on the atomics).
-__device__ __forceinline__ void fix_add_global(fix_t* acc, size_t e, double v) {
-    fix_t h, l;
-    fix_split(v, h, l);
-    __hip_atomic_fetch_add(acc + 2 * e, h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(acc + 2 * e + 1, l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// acc[e] += v in device memory (agent scope; no return value, so no wait on the atomic)
+__device__ __forceinline__ void fix_add_global(fix_t* acc, size_t e, double v, double scale) {
+    __hip_atomic_fetch_add(acc + e, fix_word(v, scale), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // acc[e] += v in LDS (workgroup scope)
-__device__ __forceinline__ void fix_add_lds(fix_t* acc, int e, double v) {
-    fix_t h, l;
-    fix_split(v, h, l);
-    __hip_atomic_fetch_add(acc + 2 * e, h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    __hip_atomic_fetch_add(acc + 2 * e + 1, l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+__device__ __forceinline__ void fix_add_lds(fix_t* acc, int e, double v, double scale) {
+    __hip_atomic_fetch_add(acc + e, fix_word(v, scale), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Add an already split pair (h, l) to device memory: the flush of LDS rows, skipping zeros.
-__device__ __forceinline__ void fix_add_pair_global(fix_t* acc, size_t e, fix_t h, fix_t l) {
-    if (h) __hip_atomic_fetch_add(acc + 2 * e, h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (l) __hip_atomic_fetch_add(acc + 2 * e + 1, l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Add an already converted word to device memory (the flush of LDS rows), skipping zeros.
+__device__ __forceinline__ void fix_add_word_global(fix_t* acc, size_t e, fix_t w) {
+    if (w) __hip_atomic_fetch_add(acc + e, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-
-__device__ __forceinline__ double fix_load(const fix_t* acc, size_t e) { return fix_value(acc[2 * e], acc[2 * e + 1]); }
 #endif
 
 }  // namespace qcg

@@ -270,6 +270,71 @@ int qcg_set_basis_batch(qcg_context* c, int nmol, const int* mol_nshell, const i
             c->Qshell[(size_t)all[i].sb * ns + all[i].sa] = Q[i];
         }
     }
+    {
+        // per-molecule Schwarz matrices and function -> shell maps for the fixed-point scales
+        std::vector<int64_t> qoff(nmol);
+        std::vector<int> mns(nmol), bfoff(nmol), shell0(nmol), sfoff(nmol), bfsh, rowmol, sfirst;
+        int64_t qn = 0;
+        int gs = 0;
+        for (int m = 0; m < nmol; ++m) {
+            const HostBasis& B = c->mols[m];
+            if (B.nshell > 4096) return fail(c, QCG_INVALID_ARG, "more than 4096 shells in one molecule");
+            qoff[m] = qn;
+            mns[m] = B.nshell;
+            shell0[m] = gs;
+            bfoff[m] = (int)bfsh.size();
+            sfoff[m] = (int)sfirst.size();
+            for (int sh = 0; sh < B.nshell; ++sh) {
+                sfirst.push_back(B.bf_off[sh]);
+                for (int k = 0; k < ncart(B.l[sh]); ++k) {
+                    bfsh.push_back(sh);
+                    rowmol.push_back(m);
+                }
+            }
+            sfirst.push_back(B.nbf);
+            qn += (int64_t)B.nshell * B.nshell;
+            gs += B.nshell;
+        }
+        std::vector<double> qsh((size_t)qn, 0.0), qmax(nmol, 0.0);
+        for (int i = 0; i < npairs; ++i) {
+            const int m = all[i].mol, ns = mns[m];
+            const int a = all[i].sa - shell0[m], b = all[i].sb - shell0[m];
+            qsh[(size_t)qoff[m] + (size_t)a * ns + b] = Q[i];
+            qsh[(size_t)qoff[m] + (size_t)b * ns + a] = Q[i];
+            qmax[m] = std::max(qmax[m], Q[i]);
+        }
+        HIPCHK(c, c->alloc(&c->d_fix, 4 * (size_t)nmol));
+        HIPCHK(c, c->alloc(&c->d_qsh, qsh.size()));
+        HIPCHK(c, c->alloc(&c->d_qsh_off, nmol));
+        HIPCHK(c, c->alloc(&c->d_mol_ns, nmol));
+        HIPCHK(c, c->alloc(&c->d_bf_shell, bfsh.size()));
+        HIPCHK(c, c->alloc(&c->d_bf_off, nmol));
+        HIPCHK(c, c->alloc(&c->d_rowsum, 2 * bfsh.size()));
+        HIPCHK(c, c->alloc(&c->d_qmax, nmol));
+        HIPCHK(c, c->alloc(&c->d_row_mol, rowmol.size()));
+        HIPCHK(c, c->alloc(&c->d_shell_first, sfirst.size()));
+        HIPCHK(c, c->alloc(&c->d_shell_first_off, nmol));
+        HIPCHK(c, hipMemcpy(c->d_qsh, qsh.data(), qsh.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_qsh_off, qoff.data(), nmol * sizeof(int64_t), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_mol_ns, mns.data(), nmol * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_bf_shell, bfsh.data(), bfsh.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_bf_off, bfoff.data(), nmol * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_qmax, qmax.data(), nmol * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_row_mol, rowmol.data(), rowmol.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_shell_first, sfirst.data(), sfirst.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_shell_first_off, sfoff.data(), nmol * sizeof(int), hipMemcpyHostToDevice));
+        c->fixdata.qsh = c->d_qsh;
+        c->fixdata.qsh_off = c->d_qsh_off;
+        c->fixdata.mol_ns = c->d_mol_ns;
+        c->fixdata.bf_shell = c->d_bf_shell;
+        c->fixdata.bf_off = c->d_bf_off;
+        c->fixdata.row_mol = c->d_row_mol;
+        c->fixdata.nrows = (int)rowmol.size();
+        c->fixdata.rowsum = c->d_rowsum;
+        c->fixdata.qmax = c->d_qmax;
+        c->fixdata.shell_first = c->d_shell_first;
+        c->fixdata.shell_first_off = c->d_shell_first_off;
+    }
     // contraction buckets: every wavefront of a J/K launch then runs the same primitive loop
     // trip count; Schwarz order inside each bucket keeps the screened kets a prefix
     c->bucket_bnd.assign(kNumPairClasses * nmol, std::vector<int>());
@@ -303,8 +368,8 @@ int qcg_set_basis_batch(qcg_context* c, int nmol, const int* mol_nshell, const i
     HIPCHK(c, c->alloc(&c->d_D, n2));
     HIPCHK(c, c->alloc(&c->d_J, n2));
     HIPCHK(c, c->alloc(&c->d_K, n2));
-    HIPCHK(c, c->alloc(&c->d_Jacc, 2 * n2));
-    HIPCHK(c, c->alloc(&c->d_Kacc, 2 * n2));
+    HIPCHK(c, c->alloc(&c->d_Jacc, n2));
+    HIPCHK(c, c->alloc(&c->d_Kacc, n2));
     HIPCHK(c, c->alloc(&c->d_H, n2));
     HIPCHK(c, c->alloc(&c->d_F, n2));
     c->has_basis = true;
@@ -431,9 +496,13 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
         if (rc) return rc;
     }
     HIPCHK(c, stream_begin(c, st));
-    // the fixed-point words are zeroed as doubles: +0.0 is the all-zero bit pattern
-    if (do_j) HIPCHK(c, launch_fill(reinterpret_cast<double*>(c->d_Jacc), 2 * n2, 0.0, st));
-    if (do_k) HIPCHK(c, launch_fill(reinterpret_cast<double*>(c->d_Kacc), 2 * n2, 0.0, st));
+    // the fixed-point words are zeroed as doubles (+0.0 is the all-zero bit pattern); their scales
+    // come from a bound at this density
+    if (do_j) HIPCHK(c, launch_fill(reinterpret_cast<double*>(c->d_Jacc), n2, 0.0, st));
+    if (do_k) HIPCHK(c, launch_fill(reinterpret_cast<double*>(c->d_Kacc), n2, 0.0, st));
+    c->fixdata.mat_off = c->d_mat_off;
+    c->fixdata.mol_nbf = c->d_mol_nbf;
+    HIPCHK(c, launch_fix_scale(dD, c->fixdata, c->d_fix, c->nmol, st));
     const bool mixed = c->precision == QCG_PREC_MIXED && c->fp32_cutoff > c->schwarz_thr;
     if (c->profiling && !c->ev_made) {
         for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
@@ -486,6 +555,7 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
             a.nbf = nbf;
             a.Jacc = c->d_Jacc;
             a.Kacc = c->d_Kacc;
+            a.fix = c->d_fix;
             a.do_j = do_j;
             a.do_k = do_k;
             a.ket_off = c->d_ket_off + ck * c->nmol;
@@ -518,8 +588,10 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
             HIPCHK(c, hipEventRecord(c->ev_join[i], c->aux[i]));
             HIPCHK(c, hipStreamWaitEvent(st, c->ev_join[i], 0));
         }
-    if (c->nmol == 1) HIPCHK(c, launch_symmetrize(c->d_Jacc, c->d_Kacc, dJ, dK, nbf, st));
-    else HIPCHK(c, launch_symmetrize_batch(c->d_Jacc, c->d_Kacc, dJ, dK, c->nmol, c->d_mat_off, c->d_mol_nbf, nbf, st));
+    if (c->nmol == 1) HIPCHK(c, launch_symmetrize(c->d_Jacc, c->d_Kacc, c->d_fix, dJ, dK, nbf, st));
+    else
+        HIPCHK(c, launch_symmetrize_batch(c->d_Jacc, c->d_Kacc, c->d_fix, dJ, dK, c->nmol, c->d_mat_off, c->d_mol_nbf,
+                                          nbf, st));
     HIPCHK(c, stream_done(c, st));
     return QCG_OK;
 }
@@ -841,7 +913,7 @@ int qcg_build_fock_device(qcg_context* c, const double* dH, const double* dD, do
             if (rc) return rc;
             HIPCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             rc = run_jk(c, dD, c->d_J, c->d_K, st);
-            hipError_t e = rc ? hipSuccess : launch_fock_combine(dH, c->d_Jacc, c->d_Kacc, dF, c->basis.nbf, k_factor, st);
+            hipError_t e = rc ? hipSuccess : launch_fock_combine(dH, c->d_Jacc, c->d_Kacc, c->d_fix, dF, c->basis.nbf, k_factor, st);
             hipGraph_t g = nullptr;
             const hipError_t ec = hipStreamEndCapture(st, &g);
             if (rc || e != hipSuccess || ec != hipSuccess) {
@@ -861,7 +933,7 @@ int qcg_build_fock_device(qcg_context* c, const double* dH, const double* dD, do
     }
     int rc = run_jk(c, dD, c->d_J, c->d_K, st);
     if (rc) return rc;
-    HIPCHK(c, launch_fock_combine(dH, c->d_Jacc, c->d_Kacc, dF, c->basis.nbf, k_factor, st));
+    HIPCHK(c, launch_fock_combine(dH, c->d_Jacc, c->d_Kacc, c->d_fix, dF, c->basis.nbf, k_factor, st));
     HIPCHK(c, stream_done(c, st));
     return QCG_OK;
 }
@@ -905,7 +977,7 @@ int qcg_build_fock(qcg_context* c, const double* H, const double* D, double* F) 
     HIPCHK(c, hipMemcpyAsync(c->d_H, H, n2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
     int rc = run_jk(c, c->d_D, c->d_J, c->d_K, c->stream);
     if (rc) return rc;
-    HIPCHK(c, launch_fock_combine(c->d_H, c->d_Jacc, c->d_Kacc, c->d_F, nbf, 0.5, c->stream));
+    HIPCHK(c, launch_fock_combine(c->d_H, c->d_Jacc, c->d_Kacc, c->d_fix, c->d_F, nbf, 0.5, c->stream));
     HIPCHK(c, hipMemcpyAsync(F, c->d_F, n2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return QCG_OK;
