@@ -148,6 +148,14 @@ int qcg_eri_compressed(qcg_context* ctx, double* out);
 int qcg_build_jk(qcg_context* ctx, const double* D, double* J, double* K);
 /* build_fock (scf/mod.rs:825): F = H + J - K/2. */
 int qcg_build_fock(qcg_context* ctx, const double* H, const double* D, double* F);
+/* No counterpart in qc-core: for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same J/K build as
+ * qcg_build_jk_device, and bit-identical to it, with its accumulators in a static device array of
+ * 4.2 MB (one molecule of at most 512 basis functions; larger ones get QCG_INVALID_ARG) instead of
+ * the context's buffers.  It allocates nothing: the context's work plan must exist for the current
+ * options and tiles (one qcg_build_jk_device call after the last change), else QCG_INVALID_ARG.
+ * Every call on one GPU shares that GPU's array, so calls that overlap on one GPU race. */
+int qcg_build_jk_device_noalloc(qcg_context* ctx, const double* dD, double* dJ, double* dK, void* stream);
 /* Device-pointer variant of qcg_build_jk on the given HIP stream (NULL = context stream);
  * asynchronous with respect to the host. */
 int qcg_build_jk_device(qcg_context* ctx, const double* dD, double* dJ, double* dK, void* stream);

@@ -246,7 +246,21 @@ struct GraphHold {
 };
 
 // Integral-direct J/K into Jacc/Kacc then J = Jacc + Jacc^T, K = Kacc + Kacc^T (dJ/dK may be null).
-int run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStream_t st);
+// acc (optional) replaces the context's accumulators, scales and row-sum scratch (the no-allocation
+// entry point passes its static scratch); no_alloc: fail instead of (re)building the plan.
+struct JkAcc {
+    fix_t* Jacc;
+    fix_t* Kacc;
+    double* fix;
+    double* rowsum;
+};
+int run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStream_t st, const JkAcc* acc = nullptr,
+           bool no_alloc = false);
+// The work plan, auxiliary streams and events exist for the current options and tiles.
+bool jk_prepared(const qcg_context* c);
+// Looks up this device's address of the no-allocation entry point's static scratch (noalloc.hip);
+// once per device, with the device current.
+hipError_t noalloc_setup(int device);
 // Everything run_jk needs that allocates or creates runtime objects (work-item plan, auxiliary
 // streams, fork/join events), done ahead so that run_jk itself can be captured into a graph.
 int prepare_jk(qcg_context* c);

@@ -43,6 +43,8 @@ int qcg_create(qcg_context** out, int device) {
     if (device < 0 || device >= ndev) return QCG_INVALID_ARG;
     if (hipSetDevice(device) != hipSuccess) return QCG_DEVICE;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return QCG_DEVICE;
+    // the static scratch of qcg_build_jk_device_noalloc; a failure surfaces in that entry point
+    (void)noalloc_setup(device);
     const size_t nt = sizeof(qcg_rys_coef) / sizeof(double);
     if (c->alloc(&c->d_rys, nt) != hipSuccess || c->alloc(&c->d_rys_f, nt) != hipSuccess) return QCG_DEVICE;
     std::vector<float> tf(nt);
@@ -405,8 +407,11 @@ int qcg_screen_counts(qcg_context* c, int64_t* n_kept, int64_t* n_fp32) {
 
 static bool plan_fresh(const qcg_context* c) {
     const double key[4] = {(double)c->precision, c->fp32_cutoff, c->schwarz_thr, (double)c->tile_sig};
-    return c->plan_sig == 1 && std::memcmp(key, c->plan_key, sizeof(key)) == 0 && c->ev_fork != nullptr;
+    return c->plan_sig == 1 && std::memcmp(key, c->plan_key, sizeof(key)) == 0 && c->ev_fork != nullptr &&
+           c->ev_done != nullptr;
 }
+
+bool qcg::jk_prepared(const qcg_context* c) { return plan_fresh(c) && !(c->profiling && !c->ev_made); }
 
 static int build_plan(qcg_context* c) {
     const bool mixed = c->precision == QCG_PREC_MIXED && c->fp32_cutoff > c->schwarz_thr;
@@ -471,6 +476,7 @@ static int build_plan(qcg_context* c) {
 int qcg::prepare_jk(qcg_context* c) {
     int rc = build_plan(c);
     if (rc) return rc;
+    if (!c->ev_done) HIPCHK(c, hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming));
     if (!c->ev_fork) {
         HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
         for (int i = 0; i < qcg_context::kStreams; ++i) {
@@ -481,28 +487,35 @@ int qcg::prepare_jk(qcg_context* c) {
     return QCG_OK;
 }
 
-int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStream_t st) {
+int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStream_t st, const JkAcc* acc,
+                bool no_alloc) {
     const int nbf = c->max_nbf;
     const size_t n2 = (size_t)c->mat_total;
     const int do_j = dJ != nullptr, do_k = dK != nullptr;
-    const bool in_capture = capturing(st);
-    if (in_capture) {
-        // a stale plan would free, allocate and copy synchronously inside the capture, which ends
-        // it with an error: callers prepare the plan (prepare_jk) before they begin a capture
-        if (!plan_fresh(c) || (c->profiling && !c->ev_made))
-            return fail(c, QCG_INVALID_ARG, "run_jk: work plan not prepared before the stream capture");
+    if (no_alloc || capturing(st)) {
+        // a stale plan would free, allocate and copy synchronously, which a stream capture does not
+        // allow (it would end the capture with an error) and the no-allocation entry point must not
+        // do: those callers need the plan prepared beforehand
+        if (!jk_prepared(c))
+            return fail(c, QCG_INVALID_ARG,
+                        "run_jk: work plan not prepared (build once with qcg_build_jk_device first, outside any "
+                        "stream capture, after the last change of options or tiles)");
     } else {
         int rc = prepare_jk(c);
         if (rc) return rc;
     }
+    const JkAcc own{c->d_Jacc, c->d_Kacc, c->d_fix, c->d_rowsum};
+    const JkAcc& A = acc ? *acc : own;
     HIPCHK(c, stream_begin(c, st));
     // the fixed-point words are zeroed as doubles (+0.0 is the all-zero bit pattern); their scales
     // come from a bound at this density
-    if (do_j) HIPCHK(c, launch_fill(reinterpret_cast<double*>(c->d_Jacc), n2, 0.0, st));
-    if (do_k) HIPCHK(c, launch_fill(reinterpret_cast<double*>(c->d_Kacc), n2, 0.0, st));
-    c->fixdata.mat_off = c->d_mat_off;
-    c->fixdata.mol_nbf = c->d_mol_nbf;
-    HIPCHK(c, launch_fix_scale(dD, c->fixdata, c->d_fix, c->nmol, st));
+    if (do_j) HIPCHK(c, launch_fill(reinterpret_cast<double*>(A.Jacc), n2, 0.0, st));
+    if (do_k) HIPCHK(c, launch_fill(reinterpret_cast<double*>(A.Kacc), n2, 0.0, st));
+    FixData fd = c->fixdata;
+    fd.mat_off = c->d_mat_off;
+    fd.mol_nbf = c->d_mol_nbf;
+    fd.rowsum = A.rowsum;
+    HIPCHK(c, launch_fix_scale(dD, fd, A.fix, c->nmol, st));
     const bool mixed = c->precision == QCG_PREC_MIXED && c->fp32_cutoff > c->schwarz_thr;
     if (c->profiling && !c->ev_made) {
         for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
@@ -553,9 +566,9 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
             a.diag = cb == ck;
             a.D = dD;
             a.nbf = nbf;
-            a.Jacc = c->d_Jacc;
-            a.Kacc = c->d_Kacc;
-            a.fix = c->d_fix;
+            a.Jacc = A.Jacc;
+            a.Kacc = A.Kacc;
+            a.fix = A.fix;
             a.do_j = do_j;
             a.do_k = do_k;
             a.ket_off = c->d_ket_off + ck * c->nmol;
@@ -588,10 +601,10 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
             HIPCHK(c, hipEventRecord(c->ev_join[i], c->aux[i]));
             HIPCHK(c, hipStreamWaitEvent(st, c->ev_join[i], 0));
         }
-    if (c->nmol == 1) HIPCHK(c, launch_symmetrize(c->d_Jacc, c->d_Kacc, c->d_fix, dJ, dK, nbf, st));
+    if (c->nmol == 1) HIPCHK(c, launch_symmetrize(A.Jacc, A.Kacc, A.fix, dJ, dK, nbf, st));
     else
-        HIPCHK(c, launch_symmetrize_batch(c->d_Jacc, c->d_Kacc, c->d_fix, dJ, dK, c->nmol, c->d_mat_off, c->d_mol_nbf,
-                                          nbf, st));
+        HIPCHK(c, launch_symmetrize_batch(A.Jacc, A.Kacc, A.fix, dJ, dK, c->nmol, c->d_mat_off, c->d_mol_nbf, nbf,
+                                          st));
     HIPCHK(c, stream_done(c, st));
     return QCG_OK;
 }

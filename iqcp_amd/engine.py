@@ -40,7 +40,7 @@ EXPORTS = [
     "qcg_set_basis_batch", "qcg_batch_build_jk", "qcg_batch_rhf_scf", "qcg_autotune",
     "qcg_set_tiles", "qcg_get_tiles", "qcg_xc_gradient", "qcg_ks_gradient", "qcg_set_graph",
     "qcg_set_prim_split", "qcg_set_xc_mode", "qcg_build_fock_stored", "qcg_rhf_scf_stored",
-    "qcg_autotune_refine", "qcg_error_bound", "qcg_scf_history",
+    "qcg_autotune_refine", "qcg_error_bound", "qcg_scf_history", "qcg_build_jk_device_noalloc",
 ]
 
 
@@ -241,6 +241,12 @@ class Engine:
         """Device-pointer J/K build (e.g. torch tensor .data_ptr()) on a HIP stream."""
         self._check(self.L.qcg_build_jk_device(self.ctx, ctypes.c_void_p(dD_ptr), ctypes.c_void_p(dJ_ptr),
                                                ctypes.c_void_p(dK_ptr), ctypes.c_void_p(stream_ptr)))
+
+    def build_jk_device_noalloc(self, dD_ptr: int, dJ_ptr: int, dK_ptr: int, stream_ptr: int = 0):
+        """qcg_build_jk_device with the library's static scratch (allocates nothing; the plan must
+        exist: one build_jk_device call after the last change of options or tiles)."""
+        self._check(self.L.qcg_build_jk_device_noalloc(self.ctx, ctypes.c_void_p(dD_ptr), ctypes.c_void_p(dJ_ptr),
+                                                       ctypes.c_void_p(dK_ptr), ctypes.c_void_p(stream_ptr)))
 
     def build_fock_device(self, dH_ptr: int, dD_ptr: int, dF_ptr: int, k_factor: float = 0.5,
                           stream_ptr: int = 0):
