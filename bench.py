@@ -41,14 +41,17 @@ def kernel_name(la, lb, lc, ld, tile, f64, fused=False):
     return f"{k}<{la}, {lb}, {lc}, {ld}, {'double' if f64 else 'float'}>"
 
 
-def hbm_traffic(name):
+def hbm_traffic(name, tile):
     """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc measurement
-    (FETCH_SIZE and WRITE_SIZE, scaled by the same counters on a 1 GiB copy); None if absent."""
+    (FETCH_SIZE and WRITE_SIZE, scaled by the same counters on a 1 GiB copy), taken with the
+    committed tiles; None if absent or if this run's tile for the kernel is another one."""
     f = ROOT / "profiles" / "jk_kernels_decane_631gs_hbm_traffic.json"
     if not f.exists():
         return None
     k = json.loads(f.read_text()).get("kernels", {}).get(name)
-    return None if k is None else k["bytes_per_launch"]
+    if k is None or list(k.get("tile", [])) != list(tile):
+        return None
+    return k["bytes_per_launch"]
 
 
 def grad_model_flops(stats):
@@ -83,7 +86,11 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default tiles instead of autotuning (profiling runs)")
     ap.add_argument("--tiles", default=None,
-                    help="JSON {'fp64': tiles, 'mixed': tiles} for the decane build instead of tuning")
+                    help="JSON {'fp64': tiles, 'mixed': tiles} for the decane build (default: the committed "
+                         "profiles/decane_631gs_tiles_tuned.json, which the traffic profile was measured with)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the decane tiles in this run (autotune + whole-build refinement) instead")
+    ap.add_argument("--save-tiles", default=None, help="write the decane tiles used to this JSON file")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (5 times): the reference kernel whose "
                          "FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic figures")
@@ -126,12 +133,18 @@ def main():
     stream = torch.cuda.Stream(device=dev)
 
     tiles = {}
+    tiles_file = args.tiles
+    committed = ROOT / "profiles" / "decane_631gs_tiles_tuned.json"
+    if tiles_file is None and not args.autotune and not args.no_autotune and committed.exists():
+        tiles_file = str(committed)
+    tiles_source = (f"file {Path(tiles_file).name}" if tiles_file else
+                    "library defaults" if args.no_autotune else "autotuned in this run")
 
     def timed(prec):
         eng.set_options(prec, args.fp32_cutoff)
         # per-class tile choice for this precision mode
-        if args.tiles:
-            eng.set_tiles(json.loads(Path(args.tiles).read_text())[prec])
+        if tiles_file:
+            eng.set_tiles(json.loads(Path(tiles_file).read_text())[prec])
             tiles[prec] = eng.get_tiles()
         elif args.no_autotune:
             tiles[prec] = eng.get_tiles()
@@ -155,6 +168,8 @@ def main():
 
     el64, F64, E64 = timed("fp64")
     elmx, Fmx, Emx = timed("mixed")
+    if args.save_tiles:
+        Path(args.save_tiles).write_text(json.dumps({p: [[list(t) for t in pair] for pair in tiles[p]] for p in tiles}))
     log(f"fp64: {1e3 * el64 / args.steps:.3f} ms/build  mixed: {1e3 * elmx / args.steps:.3f} ms/build")
     # the same build through the host-buffer entry point (qcg_build_fock: D in, F out over PCIe),
     # for the PCIe-inclusive rate; not the headline value
@@ -216,8 +231,8 @@ def main():
     roofline = {"bound": "valu", "kernel": kname, "achieved": round(achieved, 3),
                 "peak": PEAK["f64" if fused else dt], "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK["f64" if fused else dt], 4),
-                "traffic": hbm_traffic(kname), "kernel_ms": round(ms, 4), "kernel_flops_model": fl,
-                "tile": list(tile)}
+                "traffic": hbm_traffic(kname, tile), "kernel_ms": round(ms, 4), "kernel_flops_model": fl,
+                "tile": list(tile), "tiles_source": tiles_source}
     tot_fl = sum(x["flops_fp64"] + x["flops_fp32"] for x in per_kernel["mixed"])
     tot_ms = sum(x["ms_fp64"] + x["ms_fp32"] for x in per_kernel["mixed"])
     kept, n32 = eng.screen_counts()
@@ -416,6 +431,7 @@ def main():
         "scf_energy_fp64": scf["energy_total"],
         "scf_iterations": scf["iterations"],
         "all_kernels_model_tflops": round(tot_fl / (tot_ms * 1e-3) / 1e12, 3) if tot_ms > 0 else None,
+        "tiles_source": tiles_source,
         "tiles_mixed": tiles.get("mixed"),
         "tiles_fp64": tiles.get("fp64"),
         "b3lyp": ks,

@@ -127,7 +127,10 @@ struct qcg_context {
     int2* d_segs = nullptr;
     double plan_key[4] = {-1.0, -1.0, -1.0, -1.0};
     int plan_sig = -1, tile_sig = 0;
-    static constexpr int kStreams = 4;
+#ifndef QCG_STREAMS
+#define QCG_STREAMS 4
+#endif
+    static constexpr int kStreams = QCG_STREAMS;
     hipStream_t aux[kStreams] = {};
     hipEvent_t ev_fork = nullptr, ev_join[kStreams] = {};
     // cached class counts / flop model, valid for (precision, fp32_cutoff, schwarz_thr) of stats_key
@@ -247,7 +250,8 @@ struct GraphHold {
 
 // Integral-direct J/K into Jacc/Kacc then J = Jacc + Jacc^T, K = Kacc + Kacc^T (dJ/dK may be null).
 // acc (optional) replaces the context's accumulators, scales and row-sum scratch (the no-allocation
-// entry point passes its static scratch); no_alloc: fail instead of (re)building the plan.
+// entry point passes its static scratch); no_alloc: fail instead of (re)building the plan;
+// symmetrize = false: leave J and K in the accumulators (a Fock build reads them there).
 struct JkAcc {
     fix_t* Jacc;
     fix_t* Kacc;
@@ -255,7 +259,7 @@ struct JkAcc {
     double* rowsum;
 };
 int run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStream_t st, const JkAcc* acc = nullptr,
-           bool no_alloc = false);
+           bool no_alloc = false, bool symmetrize = true);
 // The work plan, auxiliary streams and events exist for the current options and tiles.
 bool jk_prepared(const qcg_context* c);
 // Looks up this device's address of the no-allocation entry point's static scratch (noalloc.hip);

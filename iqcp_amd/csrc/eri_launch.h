@@ -240,8 +240,10 @@ struct FixData {
 };
 // Per molecule m the fixed-point scales fix[4m] (J) and fix[4m + 1] (K) and their inverses from a
 // bound on the accumulated magnitudes at density D: B_J = 2 Qmax sum_ls Q_ls |D_ls|,
-// B_K = 2 Qmax max_s sum_t Q_st sum_(n in t) sum_l |D_nl|.
-hipError_t launch_fix_scale(const double* D, const FixData& f, double* fix, int nmol, hipStream_t st);
+// B_K = 2 Qmax max_s sum_t Q_st sum_(n in t) sum_l |D_nl|.  Also zeroes the accumulators Jacc and
+// Kacc (either may be null).
+hipError_t launch_fix_scale(const double* D, const FixData& f, double* fix, int nmol, fix_t* Jacc, fix_t* Kacc,
+                            hipStream_t st);
 hipError_t launch_symmetrize(const fix_t* Jacc, const fix_t* Kacc, const double* fix, double* J, double* K, int nbf,
                              hipStream_t st);
 // batched J = Jacc + Jacc^T, K = Kacc + Kacc^T over block-diagonal packed matrices

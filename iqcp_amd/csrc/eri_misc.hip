@@ -67,18 +67,22 @@ __global__ __launch_bounds__(256) void fix_rows_kernel(
     const double* __restrict__ D, const int64_t* __restrict__ mat_off, const int* __restrict__ mol_nbf,
     const double* __restrict__ qsh, const int64_t* __restrict__ qsh_off, const int* __restrict__ mol_ns,
     const int* __restrict__ bf_shell, const int* __restrict__ bf_off, const int* __restrict__ row_mol, int nrows,
-    double* __restrict__ rowsum) {
+    double* __restrict__ rowsum, fix_t* __restrict__ Jacc, fix_t* __restrict__ Kacc) {
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (row >= nrows) return;
     const int m = row_mol[row], n = row - bf_off[m], nbf = mol_nbf[m], ns = mol_ns[m];
     const int* bs = bf_shell + bf_off[m];
-    const double* Drow = D + mat_off[m] + (size_t)n * nbf;
+    const size_t ro = mat_off[m] + (size_t)n * nbf;
+    const double* Drow = D + ro;
     const double* Qrow = qsh + qsh_off[m] + (size_t)bs[n] * ns;
     double r = 0.0, j = 0.0;
     for (int l = lane; l < nbf; l += 64) {
         const double a = fabs(Drow[l]);
         r += a;
         j += Qrow[bs[l]] * a;
+        // the same row of the accumulators starts at zero (null: that matrix is not built)
+        if (Jacc) Jacc[ro + l] = 0;
+        if (Kacc) Kacc[ro + l] = 0;
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
@@ -142,9 +146,10 @@ __global__ __launch_bounds__(1024) void fix_scale_kernel(FixData f, double* __re
     }
 }
 
-hipError_t launch_fix_scale(const double* D, const FixData& f, double* fix, int nmol, hipStream_t st) {
+hipError_t launch_fix_scale(const double* D, const FixData& f, double* fix, int nmol, fix_t* Jacc, fix_t* Kacc,
+                            hipStream_t st) {
     hipLaunchKernelGGL(fix_rows_kernel, dim3((f.nrows + 3) / 4), dim3(256), 0, st, D, f.mat_off, f.mol_nbf, f.qsh,
-                       f.qsh_off, f.mol_ns, f.bf_shell, f.bf_off, f.row_mol, f.nrows, f.rowsum);
+                       f.qsh_off, f.mol_ns, f.bf_shell, f.bf_off, f.row_mol, f.nrows, f.rowsum, Jacc, Kacc);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fix_scale_kernel, dim3(nmol), dim3(1024), 0, st, f, fix);

@@ -488,9 +488,8 @@ int qcg::prepare_jk(qcg_context* c) {
 }
 
 int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStream_t st, const JkAcc* acc,
-                bool no_alloc) {
+                bool no_alloc, bool symmetrize) {
     const int nbf = c->max_nbf;
-    const size_t n2 = (size_t)c->mat_total;
     const int do_j = dJ != nullptr, do_k = dK != nullptr;
     if (no_alloc || capturing(st)) {
         // a stale plan would free, allocate and copy synchronously, which a stream capture does not
@@ -507,15 +506,13 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
     const JkAcc own{c->d_Jacc, c->d_Kacc, c->d_fix, c->d_rowsum};
     const JkAcc& A = acc ? *acc : own;
     HIPCHK(c, stream_begin(c, st));
-    // the fixed-point words are zeroed as doubles (+0.0 is the all-zero bit pattern); their scales
-    // come from a bound at this density
-    if (do_j) HIPCHK(c, launch_fill(reinterpret_cast<double*>(A.Jacc), n2, 0.0, st));
-    if (do_k) HIPCHK(c, launch_fill(reinterpret_cast<double*>(A.Kacc), n2, 0.0, st));
+    // the fixed-point scales come from a bound at this density; the same pass zeroes the
+    // accumulators (a kernel, no memset: graph replays stay exact)
     FixData fd = c->fixdata;
     fd.mat_off = c->d_mat_off;
     fd.mol_nbf = c->d_mol_nbf;
     fd.rowsum = A.rowsum;
-    HIPCHK(c, launch_fix_scale(dD, fd, A.fix, c->nmol, st));
+    HIPCHK(c, launch_fix_scale(dD, fd, A.fix, c->nmol, do_j ? A.Jacc : nullptr, do_k ? A.Kacc : nullptr, st));
     const bool mixed = c->precision == QCG_PREC_MIXED && c->fp32_cutoff > c->schwarz_thr;
     if (c->profiling && !c->ev_made) {
         for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
@@ -601,7 +598,9 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
             HIPCHK(c, hipEventRecord(c->ev_join[i], c->aux[i]));
             HIPCHK(c, hipStreamWaitEvent(st, c->ev_join[i], 0));
         }
-    if (c->nmol == 1) HIPCHK(c, launch_symmetrize(A.Jacc, A.Kacc, A.fix, dJ, dK, nbf, st));
+    if (!symmetrize) {
+        // the caller reads the accumulators itself (launch_fock_combine)
+    } else if (c->nmol == 1) HIPCHK(c, launch_symmetrize(A.Jacc, A.Kacc, A.fix, dJ, dK, nbf, st));
     else
         HIPCHK(c, launch_symmetrize_batch(A.Jacc, A.Kacc, A.fix, dJ, dK, c->nmol, c->d_mat_off, c->d_mol_nbf, nbf,
                                           st));
@@ -925,7 +924,7 @@ int qcg_build_fock_device(qcg_context* c, const double* dH, const double* dD, do
             int rc = prepare_jk(c);
             if (rc) return rc;
             HIPCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            rc = run_jk(c, dD, c->d_J, c->d_K, st);
+            rc = run_jk(c, dD, c->d_J, c->d_K, st, nullptr, false, false);
             hipError_t e = rc ? hipSuccess : launch_fock_combine(dH, c->d_Jacc, c->d_Kacc, c->d_fix, dF, c->basis.nbf, k_factor, st);
             hipGraph_t g = nullptr;
             const hipError_t ec = hipStreamEndCapture(st, &g);
@@ -944,7 +943,7 @@ int qcg_build_fock_device(qcg_context* c, const double* dH, const double* dD, do
         HIPCHK(c, stream_done(c, st));
         return QCG_OK;
     }
-    int rc = run_jk(c, dD, c->d_J, c->d_K, st);
+    int rc = run_jk(c, dD, c->d_J, c->d_K, st, nullptr, false, false);
     if (rc) return rc;
     HIPCHK(c, launch_fock_combine(dH, c->d_Jacc, c->d_Kacc, c->d_fix, dF, c->basis.nbf, k_factor, st));
     HIPCHK(c, stream_done(c, st));
@@ -988,7 +987,7 @@ int qcg_build_fock(qcg_context* c, const double* H, const double* D, double* F) 
     const size_t n2 = (size_t)nbf * nbf;
     HIPCHK(c, hipMemcpyAsync(c->d_D, D, n2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_H, H, n2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    int rc = run_jk(c, c->d_D, c->d_J, c->d_K, c->stream);
+    int rc = run_jk(c, c->d_D, c->d_J, c->d_K, c->stream, nullptr, false, false);
     if (rc) return rc;
     HIPCHK(c, launch_fock_combine(c->d_H, c->d_Jacc, c->d_Kacc, c->d_fix, c->d_F, nbf, 0.5, c->stream));
     HIPCHK(c, hipMemcpyAsync(F, c->d_F, n2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
