@@ -126,10 +126,19 @@ def main():
         tiles_file = str(committed)
     tiles_source = (f"file {Path(tiles_file).name}" if tiles_file else
                     "library defaults" if args.no_autotune else "autotuned in this run")
+    tiles_cfg = json.loads(Path(tiles_file).read_text()) if tiles_file else {}
+    costs = {}
+
+    def use_tiles(prec):
+        """The tiles of a precision mode from the tiles file, with the per-class launch costs they
+        were tuned under (the class kernels' order over the streams) when the file holds them."""
+        eng.set_tiles(tiles_cfg[prec])
+        eng.set_class_cost(tiles_cfg.get(f"cost_{prec}"))
+
     if tiles_file:
         # the setup SCF's FP64 builds run the same kernels and tiles as the timed FP64 builds, so a
         # counter run over the whole command sees one tile per kernel
-        eng.set_tiles(json.loads(Path(tiles_file).read_text())["fp64"])
+        use_tiles("fp64")
     t0 = time.perf_counter()
     scf = eng.rhf_scf(ScfConfig.tight())
     t_scf = time.perf_counter() - t0
@@ -147,13 +156,16 @@ def main():
         eng.set_options(prec, args.fp32_cutoff)
         # per-class tile choice for this precision mode
         if tiles_file:
-            eng.set_tiles(json.loads(Path(tiles_file).read_text())[prec])
+            use_tiles(prec)
             tiles[prec] = eng.get_tiles()
         elif args.no_autotune:
+            eng.set_class_cost(None)
             tiles[prec] = eng.get_tiles()
         else:
             eng.autotune(dD.data_ptr(), 2)
             tiles[prec] = eng.autotune_refine(dD.data_ptr(), 30)
+        cost, measured = eng.get_class_cost()
+        costs[prec] = cost if measured else None
         eng.set_graph(not args.no_graph)  # one HIP-graph replay per Fock build
         with torch.cuda.stream(stream):
             for _ in range(args.warmup):
@@ -172,12 +184,15 @@ def main():
     el64, F64, E64 = timed("fp64")
     elmx, Fmx, Emx = timed("mixed")
     if args.save_tiles:
-        Path(args.save_tiles).write_text(json.dumps({p: [[list(t) for t in pair] for pair in tiles[p]] for p in tiles}))
+        out = {p: [[list(t) for t in pair] for pair in tiles[p]] for p in tiles}
+        out.update({f"cost_{p}": [round(x, 5) for x in costs[p]] for p in costs if costs[p]})
+        Path(args.save_tiles).write_text(json.dumps(out))
     log(f"fp64: {1e3 * el64 / args.steps:.3f} ms/build  mixed: {1e3 * elmx / args.steps:.3f} ms/build")
     # the same build through the host-buffer entry point (qcg_build_fock: D in, F out over PCIe),
     # for the PCIe-inclusive rate; not the headline value
     eng.set_options("mixed", args.fp32_cutoff)
     eng.set_tiles(tiles["mixed"])
+    eng.set_class_cost(costs["mixed"])
     eng.build_fock(H, D)
     t = time.perf_counter()
     for _ in range(args.steps):
@@ -189,6 +204,7 @@ def main():
     for prec in ("fp64", "mixed"):
         eng.set_options(prec, args.fp32_cutoff)
         eng.set_tiles(tiles[prec])
+        eng.set_class_cost(costs[prec])
         eng.set_profiling(True)
         acc = None
         with torch.cuda.stream(stream):

@@ -203,6 +203,16 @@ int qcg_autotune_refine(qcg_context* ctx, const double* dD, int builds, int* cho
  * kets for block 0, block x kets otherwise). */
 int qcg_set_tiles(qcg_context* ctx, const int* tiles);
 int qcg_get_tiles(qcg_context* ctx, int* tiles);
+/* Launch schedule of the J/K class kernels.  A build launches the 21 class kernels longest first,
+ * each onto the auxiliary stream with the least cost so far, by a cost per class (21 doubles,
+ * classes in the order of qcg_set_tiles): the per-class times qcg_autotune measured, or costs set
+ * here (e.g. saved with a set of tuned tiles), or, when neither is set (all zero or cost NULL
+ * resets), the work plan's model flops of each class's surviving quartets before the primitive
+ * screen.  qcg_get_class_cost returns the costs in use (*measured: 1 if set or measured, 0 for
+ * the plan's estimate, which exists after the first J/K build).  No counterpart in qc-core:
+ * launch configuration only; J and K do not depend on it (order-independent accumulation). */
+int qcg_set_class_cost(qcg_context* ctx, const double* cost);
+int qcg_get_class_cost(qcg_context* ctx, double* cost, int* measured);
 /* Primitive split of the J/K and gradient work items: a bra pair with more than `split` primitive
  * pairs is processed as ranges of at most `split` of them (the integral is linear in the bra
  * primitives), so heavy contracted quartets do not serialise on one lane.  split = 0 turns it off.

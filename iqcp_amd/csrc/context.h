@@ -101,9 +101,12 @@ struct qcg_context {
                          true, true, true, true, true, true, true, true, true, true};
     // bra pairs with more primitive pairs than this are cut into primitive ranges (cut_items_split)
     int prim_split = 0;
-    // launch schedule of the class kernels: measured time per class (qcg_autotune, 0 = unknown);
-    // with all known, classes launch longest first onto the least-loaded auxiliary stream
+    // launch schedule of the class kernels: classes launch longest first onto the least-loaded
+    // auxiliary stream, by the cost per class set by qcg_autotune or qcg_set_class_cost
+    // (class_cost_set), else by plan_cost, the work plan's model flops per class
     double class_ms[21] = {};
+    bool class_cost_set = false;
+    double plan_cost[21] = {};
     int launch_order[21] = {}, launch_stream[21] = {};
     // ERI gradient work items (built once per threshold and primitive split) and the device
     // gradient (natom x 3)

@@ -434,6 +434,14 @@ static int build_plan(qcg_context* c) {
     int cls = 0;
     for (int cb = kNumPairClasses - 1; cb >= 0; --cb)
         for (int ck = cb; ck >= 0; --ck, ++cls) {
+            // the class's model flops over its surviving quartets, before the primitive screen
+            // (the default launch schedule's cost): n_roots [80 + 15 (la+lb+1)(lc+ld+1)
+            // + N_q (4 + 6 (lb+ld))] per primitive quartet plus 12 N_q per quartet
+            const int la = cb == 0 ? 0 : cb <= 2 ? 1 : 2, lb = (cb == 0 || cb == 1 || cb == 3) ? 0 : (cb == 2 || cb == 4) ? 1 : 2;
+            const int lc = ck == 0 ? 0 : ck <= 2 ? 1 : 2, ld = (ck == 0 || ck == 1 || ck == 3) ? 0 : (ck == 2 || ck == 4) ? 1 : 2;
+            const double nq = (double)ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+            const double fpq = ((la + lb + lc + ld) / 2 + 1) * (80.0 + 15.0 * (la + lb + 1) * (lc + ld + 1) + nq * (4.0 + 6.0 * (lb + ld)));
+            double cost = 0.0;
             for (int part = 0; part < 2; ++part) {
                 // fused mixed launches run both parts with the FP64 part's tile
                 const qcg_context::Tile& t = c->tile[cls][(mixed && c->fuse_cls[cls]) ? 0 : part];
@@ -447,18 +455,24 @@ static int build_plan(qcg_context* c) {
                     const int pbase = c->ket_off_h[cb * nmol + m];
                     for (int p = 0; p < c->ket_cnt_h[cb * nmol + m]; ++p) {
                         const int g0 = (int)c->segs.size();
+                        const int npp = c->pairs[c->class_off[cb] + pbase + p].npp;
+                        const int kbase = c->class_off[ck] + c->ket_off_h[ck * nmol + m];
                         for_ket_ranges(c, cb, ck, m, p, c->schwarz_thr, cut, [&](int bs, int sp, int e) {
                             const int lo = part == 0 ? bs : sp, hi = part == 0 ? sp : e;
-                            if (hi > lo) c->segs.push_back(make_int2(lo, hi - lo));
+                            if (hi > lo) {
+                                c->segs.push_back(make_int2(lo, hi - lo));
+                                // a segment lies in one contraction bucket: one ket primitive count
+                                cost += (hi - lo) * (fpq * npp * c->pairs[kbase + lo].npp + 12.0 * nq);
+                            }
                         });
                         // bras above kBraPPMax primitive pairs are always cut into ranges
-                        const int npp = c->pairs[c->class_off[cb] + pbase + p].npp;
                         const int split = (c->prim_split > 0 && c->prim_split < kBraPPMax) ? c->prim_split : kBraPPMax;
                         cut_items_split(c->items, c->segs, g0, pbase + p, ch, npp, split);
                     }
                 }
                 c->item_cnt[cls][part] = (int)c->items.size() - c->item_off[cls][part];
             }
+            c->plan_cost[cls] = cost;
         }
     c->release(c->d_items);
     c->d_items = nullptr;
@@ -533,23 +547,22 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
             cls_cb[i] = cb;
             cls_ck[i] = ck;
         }
-    bool timed = true;
-    for (int i = 0; i < 21; ++i) timed = timed && c->class_ms[i] > 0.0;
-    if (timed) {
-        // longest processing time first onto the least-loaded stream
+    {
+        // longest processing time first onto the least-loaded stream, by the measured or set
+        // costs, else by the plan's model flops
+        const double* cost = c->class_cost_set ? c->class_ms : c->plan_cost;
         int* ord = c->launch_order;
         for (int i = 0; i < 21; ++i) ord[i] = i;
-        std::stable_sort(ord, ord + 21, [&](int x, int y) { return c->class_ms[x] > c->class_ms[y]; });
+        std::stable_sort(ord, ord + 21, [&](int x, int y) { return cost[x] > cost[y]; });
         double load[qcg_context::kStreams] = {};
         for (int k = 0; k < 21; ++k) {
             const int s = (int)(std::min_element(load, load + qcg_context::kStreams) - load);
             c->launch_stream[ord[k]] = s;
-            load[s] += c->class_ms[ord[k]];
+            load[s] += cost[ord[k]];
         }
     }
-    int launched = 0;
     for (int k = 0; k < 21; ++k) {
-        const int cls = timed ? c->launch_order[k] : k;
+        const int cls = c->launch_order[k];
         const int cb = cls_cb[cls], ck = cls_ck[cls];
         for (int part = 0; part < 2; ++part) {
             if (part == 1 && !mixed) continue;
@@ -585,8 +598,7 @@ int qcg::run_jk(qcg_context* c, const double* dD, double* dJ, double* dK, hipStr
                 a.ppf = c->d_ppf;
                 a.rysf = c->d_rys_f;
             }
-            hipStream_t s = !fork ? st : c->aux[timed ? c->launch_stream[cls] : launched % qcg_context::kStreams];
-            if (a.nitems > 0) ++launched;
+            hipStream_t s = !fork ? st : c->aux[c->launch_stream[cls]];
             if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4 * cls + 2 * part], s));
             HIPCHK(c, launch_jk(cb, ck, part == 1, a, s));
             if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4 * cls + 2 * part + 1], s));
@@ -679,6 +691,7 @@ int qcg_autotune(qcg_context* c, const double* dD, int iters, int* chosen) {
             if (tot[i] < best[i]) { best[i] = tot[i]; pick[i] = k; }
     }
     for (int i = 0; i < 21; ++i) c->class_ms[i] = (best[2 * i] + best[2 * i + 1]) / iters;
+    c->class_cost_set = true;
     for (int i = 0; i < 21; ++i)
         for (int part = 0; part < 2; ++part) c->tile[i][part] = tile_of(pick[2 * i + part], i);
     export_tiles(c, chosen);
@@ -955,6 +968,28 @@ int qcg_set_prim_split(qcg_context* c, int split) {
     if (split < 0) return fail(c, QCG_INVALID_ARG, "prim split must be >= 0");
     c->prim_split = split;
     ++c->tile_sig;  // the work items change: plan and captured graph are rebuilt
+    return QCG_OK;
+}
+
+int qcg_set_class_cost(qcg_context* c, const double* cost) {
+    if (!c) return QCG_INVALID_ARG;
+    bool any = false;
+    if (cost)
+        for (int i = 0; i < 21; ++i) {
+            if (!(cost[i] >= 0.0) || !std::isfinite(cost[i]))
+                return fail(c, QCG_INVALID_ARG, "class cost must be finite and >= 0");
+            any = any || cost[i] > 0.0;
+        }
+    c->class_cost_set = any;
+    for (int i = 0; i < 21; ++i) c->class_ms[i] = any ? cost[i] : 0.0;
+    ++c->tile_sig;  // the launch schedule changes: the captured graph is rebuilt
+    return QCG_OK;
+}
+
+int qcg_get_class_cost(qcg_context* c, double* cost, int* measured) {
+    if (!c || !cost) return QCG_INVALID_ARG;
+    for (int i = 0; i < 21; ++i) cost[i] = c->class_cost_set ? c->class_ms[i] : c->plan_cost[i];
+    if (measured) *measured = c->class_cost_set ? 1 : 0;
     return QCG_OK;
 }
 

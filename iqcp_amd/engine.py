@@ -41,6 +41,7 @@ EXPORTS = [
     "qcg_set_tiles", "qcg_get_tiles", "qcg_xc_gradient", "qcg_ks_gradient", "qcg_set_graph",
     "qcg_set_prim_split", "qcg_set_xc_mode", "qcg_build_fock_stored", "qcg_rhf_scf_stored",
     "qcg_autotune_refine", "qcg_error_bound", "qcg_scf_history", "qcg_build_jk_device_noalloc",
+    "qcg_set_class_cost", "qcg_get_class_cost",
 ]
 
 
@@ -287,6 +288,24 @@ class Engine:
         self._check(self.L.qcg_get_tiles(self.ctx, arr))
         return [[(arr[4 * i], arr[4 * i + 1]), (arr[4 * i + 2], arr[4 * i + 3])] for i in range(21)]
 
+    def set_class_cost(self, cost):
+        """Per-class costs (21, tile order) that order the class launches over the streams;
+        None or all zeros: the work plan's model flops."""
+        if cost is None:
+            self._check(self.L.qcg_set_class_cost(self.ctx, None))
+            return
+        c = [float(x) for x in cost]
+        if len(c) != 21:
+            raise ValueError("21 class costs expected")
+        self._check(self.L.qcg_set_class_cost(self.ctx, (ctypes.c_double * 21)(*c)))
+
+    def get_class_cost(self):
+        """(costs in use, True if measured by autotune or set, False for the plan's estimate)."""
+        arr = (ctypes.c_double * 21)()
+        m = ctypes.c_int()
+        self._check(self.L.qcg_get_class_cost(self.ctx, arr, ctypes.byref(m)))
+        return list(arr), bool(m.value)
+
     def set_graph(self, on: bool):
         """Replay device Fock builds from a captured HIP graph."""
         self._check(self.L.qcg_set_graph(self.ctx, int(bool(on))))
@@ -461,6 +480,13 @@ class BatchEngine:
 
     def get_tiles(self):
         return Engine.get_tiles(self)
+
+    def set_class_cost(self, cost):
+        """Per-class launch costs, as Engine.set_class_cost."""
+        Engine.set_class_cost(self, cost)
+
+    def get_class_cost(self):
+        return Engine.get_class_cost(self)
 
     def class_stats(self):
         """Per quartet-class counts and flop model over all molecules of the batch."""

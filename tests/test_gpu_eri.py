@@ -7,7 +7,7 @@ import pytest
 import oracle
 from helpers import GOLDEN, bench_molecule, iqcp_energy, preset, random_density
 from iqcp_amd import workloads
-from iqcp_amd.engine import Engine, ScfConfig
+from iqcp_amd.engine import Engine, QcgError, ScfConfig
 
 pytestmark = pytest.mark.gpu
 
@@ -239,6 +239,29 @@ def test_gpu_fock_bitwise_reproducible():
         eng.set_graph(False)
         for rep in range(3):
             assert np.array_equal(eng.build_fock(H, D), first), (prec, "eager", rep)
+
+
+def test_gpu_launch_schedule_does_not_change_bits():
+    """The class kernels' order over the four streams (set per-class costs, the plan's estimate,
+    reversed costs) changes which blocks meet at the accumulators, not the result: every schedule
+    gives the same J and K bits, and the plan's estimate is reported as such."""
+    b = bench_molecule("C6H6", "6-31g*")
+    eng = _eng(b)
+    D = random_density(b.nbf, seed=5)
+    eng.set_options("fp64")
+    jk0 = eng.build_jk(D)
+    cost, measured = eng.get_class_cost()
+    assert not measured and len(cost) == 21 and max(cost) > 0 and min(cost) >= 0
+    for c in (list(reversed(cost)), [1.0] * 21, [float(i + 1) for i in range(21)], None):
+        eng.set_class_cost(c)
+        got, m = eng.get_class_cost()
+        assert m == (c is not None)
+        if c is not None:
+            assert got == c
+        jk = eng.build_jk(D)
+        assert np.array_equal(jk.J, jk0.J) and np.array_equal(jk.K, jk0.K), c
+    with pytest.raises(QcgError):
+        eng.set_class_cost([1.0] * 20 + [float("nan")])
 
 
 def test_gpu_autotune_keeps_parity():
