@@ -4,6 +4,8 @@
 //                   grad rho (X never goes to HBM), then B3LYP per point (FP64), the aow rows and
 //                   E_xc -- the work of compute_density_and_gradient_matmul and the point part of
 //                   build_vxc_gga_matmul (dft/ks_scf.rs:1028-1093, :1150-1296) in one kernel;
+//   xc_rho_all_kernel : the same for nbf <= 352 with X for all columns held per wavefront, so Phi
+//                   is read once per K step instead of once per column pass;
 //   xc_v_kernel   : V = Phi^T aow as 128x128 output tiles over grid chunks (split K), one partial
 //                   matrix per chunk, summed and symmetrised afterwards (sum_parts, sym_half).
 // rho, grad rho, E_xc and the functional are accumulated in FP64 whatever T is; with T = float the
@@ -66,6 +68,99 @@ struct GridVals {
         }
     }
 };
+
+// LDS row padding that makes a fragment read conflict-free: the four rows l / 16 of one read start
+// 16 elements apart modulo the bank width (32 doubles or 64 floats), so with 16 consecutive
+// elements per row the 64 lanes cover the banks evenly.
+template <typename T>
+constexpr int lds_pad(int n) {
+    return sizeof(T) == 8 ? (16 - n % 32 + 32) % 32 : (16 - n % 64 + 64) % 64;
+}
+
+// All-columns rho kernel options.  PREFETCH_D: the next K step's D rows are loaded into registers
+// during the current step's MFMAs (FP32: on; FP64: off, the registers cost the second block per CU:
+// (H2O)16 FP64 rho 11.2 -> 9.3 ms without).  WPE64: waves per SIMD the kernel is compiled for (4
+// = two 512-thread blocks per CU, at most 128 VGPRs).
+#ifndef QCG_XC_RHO_PREFETCH_D
+#define QCG_XC_RHO_PREFETCH_D 1
+#endif
+#ifndef QCG_XC_RHO_PREFETCH_D64
+#define QCG_XC_RHO_PREFETCH_D64 0
+#endif
+#ifndef QCG_XC_RHO_WPE64
+#define QCG_XC_RHO_WPE64 4
+#endif
+
+// Most MFMA column tiles the all-columns kernel holds (nbf <= 352): D rows of one K step, all
+// columns, stay within the 64 KB of static LDS per block.
+constexpr int kRhoAllMaxNT = 22;
+
+// The point part shared by both rho kernels, once the rho and grad rho sums of the block's 64 points
+// are in red[row tile][point]: B3LYP per point (FP64), the aow rows, E_xc.  NW wavefronts per block
+// (4 or 8); wavefront wv works on row tile wv % 4, and with 8 wavefronts each takes half its rows
+// in the aow pass.
+template <typename T, int NW, int MAXC = 0>
+__device__ __forceinline__ void xc_point_tail(int G, int nbf, int wv, int l, int g0, const double* __restrict__ w,
+                                              const GridVals<T>& gv, T* __restrict__ aow, double* __restrict__ exc_sum,
+                                              double (*red)[kRhoPW][4], double (*coef)[kRhoPW][4], double* part) {
+    static_assert(NW == 4 || NW == 8, "four or eight wavefronts per block");
+    const int tid = 64 * wv + l, rt = wv & 3, gw = g0 + kRhoPW * rt;
+    double contrib = 0.0;
+    if (wv < 4 && l < kRhoPW) {
+        const int g = gw + l;
+        double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+        if (g < G) {
+            const double rho = fmax(red[rt][l][0], 0.0);
+            const double grx = 2.0 * red[rt][l][1], gry = 2.0 * red[rt][l][2], grz = 2.0 * red[rt][l][3];
+            double exc, vrho, vsig;
+            xc_b3lyp(rho, grx * grx + gry * gry + grz * grz, exc, vrho, vsig);
+            const double wg = w[g], f4 = 4.0 * wg * vsig;
+            c0 = wg * vrho;
+            c1 = f4 * grx;
+            c2 = f4 * gry;
+            c3 = f4 * grz;
+            contrib = wg * exc * rho;
+        }
+        coef[rt][l][0] = c0;
+        coef[rt][l][1] = c1;
+        coef[rt][l][2] = c2;
+        coef[rt][l][3] = c3;
+    }
+    __syncthreads();
+    // aow = w vrho Phi + 4 w vsigma (grad rho . grad Phi), row by row of the wavefront's points
+    constexpr int kRows = kRhoPW / (NW / 4);
+    const int p0 = (wv >> 2) * kRows;
+    for (int p = p0; p < p0 + kRows; ++p) {
+        const int g = gw + p;
+        if (g >= G) break;  // uniform over the wavefront
+        const double c0 = coef[rt][p][0], c1 = coef[rt][p][1], c2 = coef[rt][p][2], c3 = coef[rt][p][3];
+        const size_t row = (size_t)g * nbf;
+        if constexpr (MAXC > 0) {
+            // nbf <= 64 MAXC: the row's loads all in flight before the first use
+            double f[MAXC], dx[MAXC], dy[MAXC], dz[MAXC];
+#pragma unroll
+            for (int i = 0; i < MAXC; ++i)
+                if (l + 64 * i < nbf) gv.get(row + l + 64 * i, f[i], dx[i], dy[i], dz[i]);
+#pragma unroll
+            for (int i = 0; i < MAXC; ++i)
+                if (l + 64 * i < nbf) aow[row + l + 64 * i] = (T)(c0 * f[i] + c1 * dx[i] + c2 * dy[i] + c3 * dz[i]);
+        } else {
+            for (int col = l; col < nbf; col += 64) {
+                double f, dx, dy, dz;
+                gv.get(row + col, f, dx, dy, dz);
+                aow[row + col] = (T)(c0 * f + c1 * dx + c2 * dy + c3 * dz);
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) contrib += __shfl_xor(contrib, off, 64);
+    if (l == 0 && wv < 4) part[wv] = contrib;
+    __syncthreads();
+    if (tid == 0) {
+        const double s = part[0] + part[1] + part[2] + part[3];
+        if (s != 0.0) atomicAdd(exc_sum, s);
+    }
+}
 
 template <typename T, int NT>
 __global__ __launch_bounds__(256) void xc_rho_kernel(int G, int nbf, const double* __restrict__ w,
@@ -187,48 +282,150 @@ __global__ __launch_bounds__(256) void xc_rho_kernel(int G, int nbf, const doubl
             }
     }
     __syncthreads();
-    double contrib = 0.0;
-    if (l < kRhoPW) {
-        const int g = gw + l;
-        double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
-        if (g < G) {
-            const double rho = fmax(red[wv][l][0], 0.0);
-            const double grx = 2.0 * red[wv][l][1], gry = 2.0 * red[wv][l][2], grz = 2.0 * red[wv][l][3];
-            double exc, vrho, vsig;
-            xc_b3lyp(rho, grx * grx + gry * gry + grz * grz, exc, vrho, vsig);
-            const double wg = w[g], f4 = 4.0 * wg * vsig;
-            c0 = wg * vrho;
-            c1 = f4 * grx;
-            c2 = f4 * gry;
-            c3 = f4 * grz;
-            contrib = wg * exc * rho;
+    xc_point_tail<T, 4>(G, nbf, wv, l, g0, w, gv, aow, exc_sum, red, coef, part);
+}
+
+#if QCG_XC_RHO_WPE64
+#define QCG_XC_RHO_ATTR __attribute__((amdgpu_waves_per_eu(QCG_XC_RHO_WPE64)))
+#else
+#define QCG_XC_RHO_ATTR
+#endif
+
+// All-columns form of xc_rho_kernel for nbf <= 16 kRhoAllMaxNT, eight wavefronts per block of 64
+// points: wavefront wv keeps the X tile of the 16 points of row tile wv % 4 for column half wv / 4
+// (NH = ceil(NT / 2) MFMA column tiles, one accumulator each), so Phi is read from HBM once per K
+// step instead of once per column pass; D rows k0..k0+15 (all columns) are staged in LDS per K step
+// (D stays L2-resident: every block reads all of it).
+template <typename T, int NT>
+__global__ __launch_bounds__(512) QCG_XC_RHO_ATTR void xc_rho_all_kernel(int G, int nbf, const double* __restrict__ w,
+                                                         const T* __restrict__ phi, const GridVals<T> gv,
+                                                         const T* __restrict__ Dm, T* __restrict__ aow,
+                                                         double* __restrict__ exc_sum) {
+    using M = Mfma<T>;
+    using acc_t = typename M::acc;
+    constexpr int NH = (NT + 1) / 2, BN = 16 * NT, SA = kRhoBM + lds_pad<T>(kRhoBM);
+    // Bs row stride: padded for conflict-free fragment reads, and Bs large enough to hold red2 after
+    // the K loop (2 x 4 x 16 x 4 doubles)
+    constexpr int SB0 = BN + lds_pad<T>(BN), SBM = 4096 / (kRhoBK * (int)sizeof(T));
+    constexpr int SB = SB0 >= SBM ? SB0 : SBM;
+    constexpr int NA = kRhoBM * kRhoBK / 512;  // Phi elements staged per thread and K step
+    static_assert(kRhoMT == 1 && kRhoBM == 64 && NA * 512 == kRhoBM * kRhoBK, "one 16-point row tile per wavefront");
+    __shared__ T As[kRhoBK][SA];  // Phi block, transposed: As[k][point]
+    __shared__ T Bs[kRhoBK][SB];  // D rows k0 .. k0 + 15, all columns
+    __shared__ double red[4][kRhoPW][4];
+    __shared__ double coef[4][kRhoPW][4];
+    __shared__ double part[4];
+    const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, li = l & 15, lk = l >> 4;
+    const int rt = wv & 3, half = wv >> 2, t0 = half * NH;
+    const int g0 = blockIdx.x * kRhoBM, gw = g0 + kRhoPW * rt;
+    acc_t acc[NH];
+#pragma unroll
+    for (int t = 0; t < NH; ++t) acc[t] = acc_t{T(0), T(0), T(0), T(0)};
+    // next K step's Phi (and, with QCG_XC_RHO_PREFETCH_D, D) elements, loaded while the current step
+    // is multiplied
+    constexpr bool kPrefD = sizeof(T) == 8 ? QCG_XC_RHO_PREFETCH_D64 : QCG_XC_RHO_PREFETCH_D;
+    constexpr int NBD = kPrefD ? (kRhoBK * BN + 511) / 512 : 1;
+    T ra[NA], rb[NBD];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            const int e = tid + 512 * i, r = e / kRhoBK, c = e % kRhoBK, g = g0 + r, k = k0 + c;
+            ra[i] = (g < G && k < nbf) ? phi[(size_t)g * nbf + k] : T(0);
         }
-        coef[wv][l][0] = c0;
-        coef[wv][l][1] = c1;
-        coef[wv][l][2] = c2;
-        coef[wv][l][3] = c3;
+        if constexpr (kPrefD) {
+#pragma unroll
+            for (int i = 0; i < NBD; ++i) {
+                const int e = tid + 512 * i, r = e / BN, c = e - r * BN, k = k0 + r;
+                rb[i] = (e < kRhoBK * BN && k < nbf && c < nbf) ? Dm[(size_t)k * nbf + c] : T(0);
+            }
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < nbf; k0 += kRhoBK) {
+        __syncthreads();  // the previous step's fragment reads are done
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            const int e = tid + 512 * i;
+            As[e % kRhoBK][e / kRhoBK] = ra[i];
+        }
+        if constexpr (kPrefD) {
+#pragma unroll
+            for (int i = 0; i < NBD; ++i) {
+                const int e = tid + 512 * i, r = e / BN, c = e - r * BN;
+                if (e < kRhoBK * BN) Bs[r][c] = rb[i];
+            }
+        } else {
+            for (int e = tid; e < kRhoBK * BN; e += 512) {
+                const int r = e / BN, c = e - r * BN, k = k0 + r;
+                Bs[r][c] = (k < nbf && c < nbf) ? Dm[(size_t)k * nbf + c] : T(0);
+            }
+        }
+        __syncthreads();
+        if (k0 + kRhoBK < nbf) load(k0 + kRhoBK);
+        // two k sub-steps per unrolled body: all four hoist 4 NH fragment reads (spills at NH = 8)
+#pragma unroll 2
+        for (int ks = 0; ks < kRhoBK; ks += 4) {
+            const T a = As[ks + lk][kRhoPW * rt + li];
+#pragma unroll
+            for (int t = 0; t < NH; ++t)
+                if (t0 + t < NT) acc[t] = M::mma(a, Bs[ks + lk][16 * (t0 + t) + li], acc[t]);
+        }
     }
-    __syncthreads();
-    // aow = w vrho Phi + 4 w vsigma (grad rho . grad Phi), row by row of the wavefront's points
-    for (int p = 0; p < kRhoPW; ++p) {
-        const int g = gw + p;
-        if (g >= G) break;  // uniform over the wavefront
-        const double c0 = coef[wv][p][0], c1 = coef[wv][p][1], c2 = coef[wv][p][2], c3 = coef[wv][p][3];
-        const size_t row = (size_t)g * nbf;
-        for (int col = l; col < nbf; col += 64) {
-            double f, dx, dy, dz;
-            gv.get(row + col, f, dx, dy, dz);
-            aow[row + col] = (T)(c0 * f + c1 * dx + c2 * dy + c3 * dz);
+    // rho = sum_mu X Phi, grad rho / 2 = sum_mu X grad Phi over this half's columns
+    double rs[4] = {}, xs[4] = {}, ys[4] = {}, zs[4] = {};
+#pragma unroll
+    for (int t = 0; t < NH; ++t) {
+        const int col = 16 * (t0 + t) + li;
+        if (col < nbf) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int g = gw + M::row(lk, r);
+                if (g < G) {
+                    double f, dx, dy, dz;
+                    gv.get((size_t)g * nbf + col, f, dx, dy, dz);
+                    const double x = (double)acc[t][r];
+                    rs[r] += x * f;
+                    xs[r] += x * dx;
+                    ys[r] += x * dy;
+                    zs[r] += x * dz;
+                }
+            }
         }
+        // one tile's loads in flight at a time: hoisting all of them costs hundreds of VGPRs
+        asm volatile("" ::: "memory");
     }
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) contrib += __shfl_xor(contrib, off, 64);
-    if (l == 0) part[wv] = contrib;
-    __syncthreads();
-    if (tid == 0) {
-        const double s = part[0] + part[1] + part[2] + part[3];
-        if (s != 0.0) atomicAdd(exc_sum, s);
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) {
+            rs[r] += __shfl_xor(rs[r], off, 64);
+            xs[r] += __shfl_xor(xs[r], off, 64);
+            ys[r] += __shfl_xor(ys[r], off, 64);
+            zs[r] += __shfl_xor(zs[r], off, 64);
+        }
     }
+    // per column half, row tile and point: rho, grad rho sums, in the D staging area (free now)
+    static_assert(sizeof(Bs) >= 2 * 4 * kRhoPW * 4 * sizeof(double), "red2 fits in Bs");
+    auto red2 = reinterpret_cast<double (*)[4][kRhoPW][4]>(&Bs[0][0]);
+    __syncthreads();  // every wavefront's last fragment reads are done
+    if (li == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int p = M::row(lk, r);
+            red2[half][rt][p][0] = rs[r];
+            red2[half][rt][p][1] = xs[r];
+            red2[half][rt][p][2] = ys[r];
+            red2[half][rt][p][3] = zs[r];
+        }
+    }
+    __syncthreads();
+    // the two column halves, in a fixed order
+    if (tid < 4 * kRhoPW * 4) {
+        const int q = tid & 3, p = (tid >> 2) % kRhoPW, r4 = tid / (4 * kRhoPW);
+        red[r4][p][q] = red2[0][r4][p][q] + red2[1][r4][p][q];
+    }
+    __syncthreads();
+    xc_point_tail<T, 8, (16 * NT + 63) / 64>(G, nbf, wv, l, g0, w, gv, aow, exc_sum, red, coef, part);
 }
 
 // Vparts[chunk] = Phi[chunk rows]^T aow[chunk rows] by 128 x 128 output tiles (each wavefront a
@@ -375,7 +572,18 @@ static hipError_t launch_xc_mfma_t(int G, int nbf, int nchunk, const double* w, 
     constexpr int kMaxNT = sizeof(T) == 8 ? 4 : 5;
     const int tiles = (nbf + 15) / 16, passes = (tiles + kMaxNT - 1) / kMaxNT, nt = (tiles + passes - 1) / passes;
     const dim3 grid((G + kRhoBM - 1) / kRhoBM);
-    switch (nt) {
+    // up to kRhoAllMaxNT column tiles: the all-columns kernel (Phi read once per K step)
+    switch (tiles <= kRhoAllMaxNT ? 100 + tiles : nt) {
+#define QCG_RHO_ALL(n)                                                                                  \
+    case 100 + n:                                                                                       \
+        hipLaunchKernelGGL((xc_rho_all_kernel<T, n>), grid, dim3(512), 0, st, G, nbf, w, phi, gv, Dm, aow,  \
+                           exc_sum);                                                                    \
+        break;
+        QCG_RHO_ALL(1) QCG_RHO_ALL(2) QCG_RHO_ALL(3) QCG_RHO_ALL(4) QCG_RHO_ALL(5) QCG_RHO_ALL(6) QCG_RHO_ALL(7)
+        QCG_RHO_ALL(8) QCG_RHO_ALL(9) QCG_RHO_ALL(10) QCG_RHO_ALL(11) QCG_RHO_ALL(12) QCG_RHO_ALL(13)
+        QCG_RHO_ALL(14) QCG_RHO_ALL(15) QCG_RHO_ALL(16) QCG_RHO_ALL(17) QCG_RHO_ALL(18) QCG_RHO_ALL(19)
+        QCG_RHO_ALL(20) QCG_RHO_ALL(21) QCG_RHO_ALL(22)
+#undef QCG_RHO_ALL
 #define QCG_RHO_NT(n)                                                                                   \
     case n:                                                                                             \
         hipLaunchKernelGGL((xc_rho_kernel<T, n>), grid, dim3(256), 0, st, G, nbf, w, phi, gv, Dm, aow,      \

@@ -6,6 +6,8 @@ import pytest
 
 import oracle
 from helpers import GOLDEN, bench_molecule, iqcp_energy, random_density
+from iqcp_amd import workloads
+from iqcp_amd.basis import build_basis
 from iqcp_amd.engine import Engine, ScfConfig
 
 pytestmark = pytest.mark.gpu
@@ -117,6 +119,39 @@ def test_gpu_vxc_kernel_modes(mol, bas):
         assert abs(exc - exco) < tol_e * max(1.0, abs(exco)), (prec, mode, exc - exco)
         np.testing.assert_allclose(V, Vo, atol=tol_v, rtol=0, err_msg=f"{prec} mode {mode}")
         np.testing.assert_allclose(V, V.T, atol=1e-14)
+
+
+@pytest.mark.parametrize("nw", [4, 12, 13, 18])
+def test_gpu_vxc_all_columns_kernel(nw):
+    """The all-columns MFMA density kernel (every column tile of X held per wavefront, nbf <= 352)
+    at 5, 15, 16 and 22 column tiles ((H2O)n/6-31G*, 19 functions per water), FP64 products
+    (mode 2) and FP32 products (mixed), against the oracle's V_xc / E_xc on 8 sampled 512-point
+    blocks of the molecule's grid; the library-GEMM form (mode 1) on the same points as a second
+    reference."""
+    b = build_basis(workloads.water_cluster((nw, 1, 1), 2.9, seed=nw), "6-31g*", units="angstrom")
+    assert b.nbf == 19 * nw
+    eng = Engine(b)
+    n = eng.build_grid(75, "standard", False)
+    pts, w, _ = eng.get_grid()
+    rng = np.random.default_rng(nw)
+    blocks = np.sort(rng.choice(n // 512, size=8, replace=False))
+    idx = np.concatenate([np.arange(k * 512, (k + 1) * 512) for k in blocks])
+    eng.set_grid(pts[idx], w[idx])
+    D = random_density(b.nbf, seed=nw, nocc=b.n_occ) * 0.5
+    chi, grad = oracle.basis_on_grid(b, pts[idx])
+    Vo, exco, _ = oracle.vxc(chi, grad, w[idx], D)
+    got = {}
+    for prec, mode in (("fp64", 1), ("fp64", 2), ("mixed", 0)):
+        eng.set_options(prec)
+        eng.set_xc_mode(mode)
+        got[(prec, mode)] = eng.xc(D)
+    for (prec, mode), (V, exc) in got.items():
+        dv, de = np.abs(V - Vo).max(), abs(exc - exco) / max(1.0, abs(exco))
+        print(f"nbf {b.nbf} {prec} mode {mode}: max |dV| {dv:.2e}, rel dE {de:.2e}")
+        tol_v, tol_e = (2e-9, 1e-9) if prec == "fp64" else (1e-6, 1e-8)  # measured: 1.1e-7, 1.6e-9
+        assert dv < tol_v and de < tol_e, (prec, mode, dv, de)
+        np.testing.assert_allclose(V, V.T, atol=1e-14)
+    np.testing.assert_allclose(got[("fp64", 2)][0], got[("fp64", 1)][0], atol=1e-11, rtol=0)
 
 
 @pytest.mark.parametrize("mol,bas", [("H2O", "6-31g*"), ("HF", "6-31+g*")])
