@@ -117,7 +117,7 @@ def cmd_counters(a):
     states = {"SQ_WAVE_CYCLES", "SQ_WAIT_ANY", "SQ_WAIT_INST_ANY", "SQ_ACTIVE_INST_VALU"} <= have
     f64 = {"SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_ADD_F64"} <= have
     s = header(a.title, a.command)
-    cols = ["kernel", "dispatches", "waves / dispatch", "VGPRs", "LDS bytes", "avg µs"]
+    cols = ["kernel", "dispatches", "waves / dispatch", "VGPR_Count", "LDS_Block_Size", "avg µs"]
     if states:
         s += ("Wave states are fractions of SQ_WAVE_CYCLES (wave lifetime) summed over each dispatch: waiting on a "
               "counter (SQ_WAIT_ANY: s_waitcnt on vector memory, LDS or scalar loads), waiting to issue "
@@ -129,6 +129,8 @@ def cmd_counters(a):
               "(SQ_INSTS_VALU_*_F64; transcendental instructions not counted) / avg duration; FP64 peak "
               "78.6 TFLOP/s.\n\n")
         cols += ["FP64 instr / dispatch", "counted FP64 TFLOP/s", "of FP64 peak"]
+    s += ("VGPR_Count and LDS_Block_Size are the fields of rocprofv3's dispatch records as written (LDS_Block_Size "
+          "without the dynamic LDS of the launch).\n\n")
     if st:
         s += "Average durations come from the kernel-trace run (last command).\n\n"
     s += "| " + " | ".join(cols) + " |\n|---|" + "---:|" * (len(cols) - 1) + "\n"
