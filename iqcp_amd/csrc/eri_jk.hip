@@ -29,13 +29,15 @@
 #ifndef QCG_CB
 #error "compile with -DQCG_CB=<bra pair class>"
 #endif
-// Per-lane kernel options: the bra's primitive pairs staged in LDS, the D rows of the bra
-// functions staged in LDS next to the K rows.
+// Per-lane kernel options: the bra's primitive pairs staged in LDS; the D rows of the bra
+// functions staged in LDS next to the K rows (off: the K digestion reads D through the caches,
+// and the smaller LDS footprint keeps more blocks resident; decane with tiles retuned for each,
+// 643 -> 652 builds/s FP64 and 631 -> 643 mixed without them).
 #ifndef QCG_JK_BRA_LDS
 #define QCG_JK_BRA_LDS 1
 #endif
 #ifndef QCG_JK_DROWS
-#define QCG_JK_DROWS 1
+#define QCG_JK_DROWS 0
 #endif
 
 namespace qcg {
