@@ -29,12 +29,13 @@
 #ifndef QCG_CB
 #error "compile with -DQCG_CB=<bra pair class>"
 #endif
-// Per-lane kernel options: the bra's primitive pairs staged in LDS; the D rows of the bra
-// functions staged in LDS next to the K rows (off: the K digestion reads D through the caches,
-// and the smaller LDS footprint keeps more blocks resident; decane with tiles retuned for each,
-// 643 -> 652 builds/s FP64 and 631 -> 643 mixed without them).
+// Per-lane kernel options, both off by default: the bra's primitive pairs staged in LDS (off: the
+// block-uniform reads go through the scalar cache), the D rows of the bra functions staged in LDS
+// next to the K rows (off: the K digestion reads D through the caches).  The smaller LDS footprint
+// keeps more blocks resident: decane, with tiles retuned for each, 643 -> 652 builds/s FP64 and
+// 631 -> 643 mixed without the D rows, then 652 -> 677 and 643 -> 659 without the bra staging.
 #ifndef QCG_JK_BRA_LDS
-#define QCG_JK_BRA_LDS 1
+#define QCG_JK_BRA_LDS 0
 #endif
 #ifndef QCG_JK_DROWS
 #define QCG_JK_DROWS 0
