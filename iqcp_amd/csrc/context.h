@@ -146,7 +146,7 @@ struct qcg_context {
     double *d_pts = nullptr, *d_w = nullptr, *d_phi = nullptr, *d_dphx = nullptr, *d_dphy = nullptr,
            *d_dphz = nullptr, *d_X = nullptr, *d_aow = nullptr, *d_V = nullptr, *d_exc = nullptr,
            *d_Vparts = nullptr;  // split-K partial products of V = Phi^T aow
-    int v_chunks = 0, v_chunks_mfma = 0;
+    int v_chunks = 0, v_chunks_mfma = 0, v_chunks_mfma32 = 0;
     // XC build (qcg_set_xc_mode): 0 = FP32 MFMA kernels for the mixed option and rocBLAS DGEMMs
     // around point_kernel for FP64; 1 = rocBLAS DGEMMs for both; 2 = MFMA kernels for both
     int xc_mode = 0;

@@ -6,6 +6,7 @@
 //   point_kernel: rho, grad rho, sigma, B3LYP, aow row, E_xc
 //   V = Phi^T aow, V_xc = (V + V^T)/2   (GEMM, nbf x nbf x G)
 // Phi and grad Phi are evaluated once per grid and kept resident in HBM.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -49,8 +50,15 @@ int upload_grid(qcg_context* c) {
     c->v_chunks = (int)std::max<int64_t>(1, G / kVChunk);
     // the MFMA V kernel: enough grid chunks for about 2048 blocks of (chunk, output tile), at
     // least 1024 points per chunk
-    c->v_chunks_mfma = (int)std::max<int64_t>(1, std::min<int64_t>(G / 1024, 2048 / xc_mfma_vtiles(B.nbf)));
-    HIPCHK(c, c->alloc(&c->d_Vparts, (size_t)std::max(c->v_chunks, c->v_chunks_mfma) * B.nbf * B.nbf));
+    c->v_chunks_mfma = (int)std::max<int64_t>(1, std::min<int64_t>(G / 1024, 2048 / xc_mfma_vtiles(B.nbf, false)));
+    // its FP32 form: about 1024 blocks ((H2O)16: 160-wide tiles at two blocks per CU, i.e. two full
+    // rounds on 256 CUs, 256 chunks of 4 100 points), and never fewer chunks than the FP64 form, so
+    // each FP32 accumulator sums no more points than that; one round (512 blocks) was 1 % faster
+    // with twice the FP32 summation length
+    c->v_chunks_mfma32 = (int)std::max<int64_t>(1, std::min<int64_t>(G / 1024, 1024 / xc_mfma_vtiles(B.nbf, true)));
+    c->v_chunks_mfma32 = std::max(c->v_chunks_mfma32, std::min(c->v_chunks_mfma, (int)std::max<int64_t>(1, G / 1024)));
+    HIPCHK(c, c->alloc(&c->d_Vparts, (size_t)std::max({c->v_chunks, c->v_chunks_mfma, c->v_chunks_mfma32}) * B.nbf *
+                                         B.nbf));
     HIPCHK(c, c->alloc(&c->d_exc, 1));
     HIPCHK(c, hipMemcpy(c->d_pts, c->grid_xyz.data(), 3 * (size_t)G * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_w, c->grid_w.data(), (size_t)G * sizeof(double), hipMemcpyHostToDevice));
@@ -105,13 +113,14 @@ int ensure_xc32(qcg_context* c, hipStream_t st) {
 // V_xc and E_xc with the MFMA kernels (xc_mfma.hip): FP32 products in mixed precision, FP64 otherwise
 int xc_build_mfma(qcg_context* c, const double* dD, double* dV, double* dExc, hipStream_t st) {
     const int nbf = c->basis.nbf, G = (int)c->G;
-    const int nch = xc_mfma_chunks(G, c->v_chunks_mfma);
+    const bool mixed = c->precision == QCG_PREC_MIXED;
+    const int nch = xc_mfma_chunks(G, mixed ? c->v_chunks_mfma32 : c->v_chunks_mfma);
     HIPCHK(c, launch_fill(dExc, 1, 0.0, st));
     if (c->precision == QCG_PREC_MIXED) {
         int rc = ensure_xc32(c, st);
         if (rc) return rc;
         HIPCHK(c, launch_to_f32(dD, c->d_D32, (size_t)nbf * nbf, st));
-        HIPCHK(c, launch_xc_mfma(G, nbf, c->v_chunks_mfma, c->d_w, c->d_phi32, c->d_grid4, c->d_D32, c->d_aow32, dExc,
+        HIPCHK(c, launch_xc_mfma(G, nbf, c->v_chunks_mfma32, c->d_w, c->d_phi32, c->d_grid4, c->d_D32, c->d_aow32, dExc,
                                  c->d_Vparts, st));
     } else {
         HIPCHK(c, launch_xc_mfma(G, nbf, c->v_chunks_mfma, c->d_w, c->d_phi, c->d_dphx, c->d_dphy, c->d_dphz, dD, c->d_aow,

@@ -31,9 +31,27 @@ hipError_t launch_xc_mfma(int G, int nbf, int nchunk, const double* w, const flo
 hipError_t launch_xc_mfma(int G, int nbf, int nchunk, const double* w, const double* phi, const double* dpx,
                           const double* dpy, const double* dpz, const double* Dm, double* aow, double* exc_sum,
                           double* Vparts, hipStream_t st);
-// output tiles of the MFMA V kernel (128 x 128 tiles of the nbf x nbf matrix)
-inline int xc_mfma_vtiles(int nbf) {
-    const int t = (nbf + 127) / 128;
+// Output tile width of the MFMA V kernel in 16-wide MFMA tiles per wavefront (VW; a block is
+// 2 x 2 wavefronts, a 32 VW square): the width whose tiles pad the nbf x nbf matrix least, the
+// larger one on ties; at most 4 in FP64, whose accumulators take twice the registers.  nbf = 304
+// (19 MFMA tiles) in FP32: VW = 5, 2 x 2 tiles of 160 covering 320^2 instead of 3 x 3 tiles of 128
+// covering 384^2.
+inline int xc_v_width(int nbf, bool fp32) {
+    const int n16 = (nbf + 15) / 16;
+    int best = 4, bpad = -1;
+    for (int w = fp32 ? 5 : 4; w >= 3; --w) {
+        const int t = (n16 + 2 * w - 1) / (2 * w) * 2 * w;
+        if (bpad < 0 || t * t < bpad) {
+            bpad = t * t;
+            best = w;
+        }
+    }
+    return best;
+}
+// output tiles of the MFMA V kernel over the nbf x nbf matrix
+inline int xc_mfma_vtiles(int nbf, bool fp32) {
+    const int b = 32 * xc_v_width(nbf, fp32);
+    const int t = (nbf + b - 1) / b;
     return t * t;
 }
 inline int xc_mfma_chunks(int G, int nchunk) {

@@ -45,10 +45,9 @@ constexpr int kRhoBM = 4 * 16 * kRhoMT;   // grid points per block
 constexpr int kRhoPW = 16 * kRhoMT;       // grid points per wavefront
 constexpr int kRhoBK = 16;                // basis functions per K step
 constexpr int kPad = 16;                  // LDS row padding: the lanes of a fragment row hit distinct banks
-constexpr int kVW = 4;                    // V: MFMA tiles per wavefront and dimension (64 x 64)
-constexpr int kVBM = 2 * 16 * kVW;        // V output tile per block (2 x 2 wavefronts): 128 x 128
+// V: VW MFMA tiles per wavefront and dimension (xc_v_width, 3 to 5), so the block's output tile
+// (2 x 2 wavefronts) is 32 VW square
 constexpr int kVBK = 16;                  // grid points per K step
-static_assert(kVBM == 128, "xc_mfma_vtiles (xc_launch.h) counts 128 x 128 output tiles");
 
 // NT: MFMA column tiles (16 columns each) of X per pass over the columns; a pass covers 16 NT
 // columns, all passes together the nbf columns of X.  Each wavefront owns kRhoPW points (kRhoMT
@@ -428,16 +427,18 @@ __global__ __launch_bounds__(512) QCG_XC_RHO_ATTR void xc_rho_all_kernel(int G, 
     xc_point_tail<T, 8, (16 * NT + 63) / 64>(G, nbf, wv, l, g0, w, gv, aow, exc_sum, red, coef, part);
 }
 
-// Vparts[chunk] = Phi[chunk rows]^T aow[chunk rows] by 128 x 128 output tiles (each wavefront a
-// 64 x 64 quarter: every fragment read feeds kVW MFMAs); block -> (chunk, tile) so that the tiles of
-// one chunk share an XCD (blocks are dealt round-robin over the 8 XCDs) and reuse its Phi and aow
-// rows from that XCD's L2.
-template <typename T>
+// Vparts[chunk] = Phi[chunk rows]^T aow[chunk rows] by 32 VW square output tiles (each wavefront a
+// 16 VW square quarter: every fragment read feeds VW MFMAs); block -> (chunk, tile) so that the
+// tiles of one chunk share an XCD (blocks are dealt round-robin over the 8 XCDs) and reuse its Phi
+// and aow rows from that XCD's L2.
+template <typename T, int VW>
 __global__ __launch_bounds__(256) void xc_v_kernel(int G, int nbf, int ntile, int nchunk, int chunk,
                                                    const T* __restrict__ phi, const T* __restrict__ aow,
                                                    double* __restrict__ Vparts) {
     using M = Mfma<T>;
     using acc_t = typename M::acc;
+    constexpr int kVW = VW, kVBM = 32 * VW;
+    static_assert(kVBK * kVBM % 256 == 0, "whole elements per thread in the staging");
     __shared__ T As[kVBK][kVBM + kPad];  // Phi[g][m0 + i]
     __shared__ T Bs[kVBK][kVBM + kPad];  // aow[g][n0 + j]
     const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, li = l & 15, lk = l >> 4;
@@ -479,8 +480,8 @@ __global__ __launch_bounds__(256) void xc_v_kernel(int G, int nbf, int ntile, in
             T a[kVW], b[kVW];
 #pragma unroll
             for (int i = 0; i < kVW; ++i) {
-                a[i] = As[ks + lk][64 * wr + 16 * i + li];
-                b[i] = Bs[ks + lk][64 * wc + 16 * i + li];
+                a[i] = As[ks + lk][16 * kVW * wr + 16 * i + li];
+                b[i] = Bs[ks + lk][16 * kVW * wc + 16 * i + li];
             }
             // 16 x 16 sub-tiles wholly past nbf (the edge tiles of the 128 x 128 grid) are skipped:
             // for nbf = 304 that is 37 % of the products; the condition is uniform per wavefront
@@ -488,7 +489,7 @@ __global__ __launch_bounds__(256) void xc_v_kernel(int G, int nbf, int ntile, in
             for (int i = 0; i < kVW; ++i)
 #pragma unroll
                 for (int j = 0; j < kVW; ++j)
-                    if (m0 + 64 * wr + 16 * i < nbf && n0 + 64 * wc + 16 * j < nbf)
+                    if (m0 + 16 * kVW * wr + 16 * i < nbf && n0 + 16 * kVW * wc + 16 * j < nbf)
                         acc[i][j] = M::mma(a[i], b[j], acc[i][j]);
         }
     }
@@ -497,10 +498,10 @@ __global__ __launch_bounds__(256) void xc_v_kernel(int G, int nbf, int ntile, in
     for (int i = 0; i < kVW; ++i)
 #pragma unroll
         for (int j = 0; j < kVW; ++j) {
-            const int col = n0 + 64 * wc + 16 * j + li;
+            const int col = n0 + 16 * kVW * wc + 16 * j + li;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = m0 + 64 * wr + 16 * i + M::row(lk, r);
+                const int row = m0 + 16 * kVW * wr + 16 * i + M::row(lk, r);
                 if (row < nbf && col < nbf) out[(size_t)row * nbf + col] = (double)acc[i][j][r];
             }
         }
@@ -597,9 +598,27 @@ static hipError_t launch_xc_mfma_t(int G, int nbf, int nchunk, const double* w, 
     if (e != hipSuccess) return e;
     const int chunk = (G + nchunk - 1) / nchunk;
     const int nch = (G + chunk - 1) / chunk;  // <= nchunk partial matrices written
-    const int ntile = (nbf + kVBM - 1) / kVBM;
+    const int vw = xc_v_width(nbf, sizeof(T) == 4);
+    const int ntile = (nbf + 32 * vw - 1) / (32 * vw);
     const int nblk = 8 * ((nch + 7) / 8) * ntile * ntile;
-    hipLaunchKernelGGL((xc_v_kernel<T>), dim3(nblk), dim3(256), 0, st, G, nbf, ntile, nch, chunk, phi, aow, Vparts);
+    switch (vw) {
+        case 3:
+            hipLaunchKernelGGL((xc_v_kernel<T, 3>), dim3(nblk), dim3(256), 0, st, G, nbf, ntile, nch, chunk, phi, aow,
+                               Vparts);
+            break;
+        case 4:
+            hipLaunchKernelGGL((xc_v_kernel<T, 4>), dim3(nblk), dim3(256), 0, st, G, nbf, ntile, nch, chunk, phi, aow,
+                               Vparts);
+            break;
+        case 5:
+            if constexpr (sizeof(T) == 4) {
+                hipLaunchKernelGGL((xc_v_kernel<T, 5>), dim3(nblk), dim3(256), 0, st, G, nbf, ntile, nch, chunk, phi,
+                                   aow, Vparts);
+                break;
+            }
+            return hipErrorInvalidValue;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -121,13 +121,14 @@ def test_gpu_vxc_kernel_modes(mol, bas):
         np.testing.assert_allclose(V, V.T, atol=1e-14)
 
 
-@pytest.mark.parametrize("nw", [4, 12, 13, 18])
+@pytest.mark.parametrize("nw", [4, 12, 13, 16, 18])
 def test_gpu_vxc_all_columns_kernel(nw):
     """The all-columns MFMA density kernel (every column tile of X held per wavefront, nbf <= 352)
-    at 5, 15, 16 and 22 column tiles ((H2O)n/6-31G*, 19 functions per water), FP64 products
+    at 5, 15, 16, 19 and 22 column tiles ((H2O)n/6-31G*, 19 functions per water), FP64 products
     (mode 2) and FP32 products (mixed), against the oracle's V_xc / E_xc on 8 sampled 512-point
     blocks of the molecule's grid; the library-GEMM form (mode 1) on the same points as a second
-    reference."""
+    reference.  The V kernel's output tiles are 96, 128 and 160 wide over these sizes (FP32: 96 at
+    5 tiles, 128 at 15, 16 and 22, 160 at 19; FP64: 96 at 5, 128 otherwise)."""
     b = build_basis(workloads.water_cluster((nw, 1, 1), 2.9, seed=nw), "6-31g*", units="angstrom")
     assert b.nbf == 19 * nw
     eng = Engine(b)
