@@ -241,9 +241,9 @@ int qcg_xc_device(qcg_context* ctx, const double* dD, double* dVxc, double* dExc
  * rho on the fly, then V = Phi^T aow by tiles; the library form runs DGEMMs around a point kernel.
  * mode 0 (default): MFMA kernels with FP32 products (FP32 accumulation, rho, the functional and
  * E_xc in FP64, plus the first-order FP64 correction of E_xc for rounding D to FP32) for the mixed
- * precision option, library form in FP64 otherwise; mode 1: library form for both options;
- * mode 2: MFMA kernels for both (FP64 products for FP64).  No counterpart in qc-core: kernel
- * selection only. */
+ * precision option, and MFMA kernels with FP64 products for FP64 up to 352 basis functions (the
+ * library form above that); mode 1: library form for both options; mode 2: MFMA kernels for both
+ * (FP64 products for FP64).  No counterpart in qc-core: kernel selection only. */
 int qcg_set_xc_mode(qcg_context* ctx, int mode);
 /* ks_scf (dft/ks_scf.rs:1567) for B3LYP on the current grid: F = H + J - (a/2) K + V_xc. */
 int qcg_ks_scf(qcg_context* ctx, const qcg_scf_config* cfg, const double* D_init, qcg_scf_result* res, double* D,

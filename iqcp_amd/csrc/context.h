@@ -147,8 +147,9 @@ struct qcg_context {
            *d_dphz = nullptr, *d_X = nullptr, *d_aow = nullptr, *d_V = nullptr, *d_exc = nullptr,
            *d_Vparts = nullptr;  // split-K partial products of V = Phi^T aow
     int v_chunks = 0, v_chunks_mfma = 0, v_chunks_mfma32 = 0;
-    // XC build (qcg_set_xc_mode): 0 = FP32 MFMA kernels for the mixed option and rocBLAS DGEMMs
-    // around point_kernel for FP64; 1 = rocBLAS DGEMMs for both; 2 = MFMA kernels for both
+    // XC build (qcg_set_xc_mode): 0 = FP32 MFMA kernels for the mixed option, FP64 MFMA kernels up
+    // to 352 functions and rocBLAS DGEMMs around point_kernel above; 1 = rocBLAS DGEMMs for both;
+    // 2 = MFMA kernels for both
     int xc_mode = 0;
     // FP32 copies of Phi and grad Phi plus FP32 work arrays for the mixed-precision XC build,
     // made on its first use for the current grid

@@ -27,6 +27,10 @@ int ensure_blas(qcg_context* c) {
 }
 
 constexpr int64_t kVChunk = 16384;
+// Largest basis whose default FP64 XC build runs on the MFMA kernels: up to here the density kernel
+// holds all of X's columns per wavefront (16 x 22 MFMA column tiles); (H2O)16 (304 functions):
+// 11.5 ms per FP64 XC build against 12.1 ms for the library DGEMMs around point_kernel
+constexpr int kXcMfmaFp64MaxNbf = 352;
 
 int upload_grid(qcg_context* c) {
     const HostBasis& B = c->basis;
@@ -48,15 +52,25 @@ int upload_grid(qcg_context* c) {
     // kVChunk points, one strided-batched GEMM over the chunks plus a sum, so the GEMM has
     // chunks x tiles workgroups instead of a handful of tiles with a 10^6-long K loop
     c->v_chunks = (int)std::max<int64_t>(1, G / kVChunk);
-    // the MFMA V kernel: enough grid chunks for about 2048 blocks of (chunk, output tile), at
-    // least 1024 points per chunk
-    c->v_chunks_mfma = (int)std::max<int64_t>(1, std::min<int64_t>(G / 1024, 2048 / xc_mfma_vtiles(B.nbf, false)));
-    // its FP32 form: about 1024 blocks ((H2O)16: 160-wide tiles at two blocks per CU, i.e. two full
-    // rounds on 256 CUs, 256 chunks of 4 100 points), and never fewer chunks than the FP64 form, so
-    // each FP32 accumulator sums no more points than that; one round (512 blocks) was 1 % faster
-    // with twice the FP32 summation length
-    c->v_chunks_mfma32 = (int)std::max<int64_t>(1, std::min<int64_t>(G / 1024, 1024 / xc_mfma_vtiles(B.nbf, true)));
-    c->v_chunks_mfma32 = std::max(c->v_chunks_mfma32, std::min(c->v_chunks_mfma, (int)std::max<int64_t>(1, G / 1024)));
+    // the MFMA V kernel: grid chunks for two full rounds of (chunk, output tile) blocks at the
+    // kernel's occupancy, at least 1024 points per chunk, and in FP32 (whose accumulators sum a
+    // chunk) at most 4096 points per chunk.  (H2O)16: FP64 160-wide tiles at one block per CU,
+    // 128 chunks of 8 200 points (2048 / 1024 / 512 blocks: 11.70 / 11.57 / 11.50 ms per FP64
+    // XC build); FP32 at two blocks per CU, 256 chunks of 4 100 points (512 blocks were 1 % faster
+    // with twice the FP32 summation length).
+    {
+        int cus = 0;
+        HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+        const int64_t most = std::max<int64_t>(1, G / 1024);
+        auto chunks = [&](bool fp32) {
+            const int64_t blocks = 2LL * std::max(cus, 1) * xc_v_blocks_per_cu(B.nbf, fp32);
+            int64_t n = std::max<int64_t>(1, blocks / xc_mfma_vtiles(B.nbf, fp32));
+            if (fp32) n = std::max<int64_t>(n, (G + 4095) / 4096);
+            return (int)std::min(n, most);
+        };
+        c->v_chunks_mfma = chunks(false);
+        c->v_chunks_mfma32 = chunks(true);
+    }
     HIPCHK(c, c->alloc(&c->d_Vparts, (size_t)std::max({c->v_chunks, c->v_chunks_mfma, c->v_chunks_mfma32}) * B.nbf *
                                          B.nbf));
     HIPCHK(c, c->alloc(&c->d_exc, 1));
@@ -135,9 +149,12 @@ int xc_build_mfma(qcg_context* c, const double* dD, double* dV, double* dExc, hi
 
 int xc_build(qcg_context* c, const double* dD, double* dV, double* dExc, hipStream_t st) {
     if (c->G <= 0) return fail(c, QCG_NO_GRID, "no grid: call qcg_build_grid or qcg_set_grid");
-    // mode 0: FP32 MFMA kernels for the mixed option, library DGEMMs for FP64 (faster than the
-    // FP64 MFMA kernels, which mode 2 selects); mode 1: library DGEMMs for both
-    if ((c->xc_mode == 0 && c->precision == QCG_PREC_MIXED) || c->xc_mode == 2) return xc_build_mfma(c, dD, dV, dExc, st);
+    // mode 0: the MFMA kernels (FP32 products for the mixed option, FP64 otherwise), except FP64
+    // above kXcMfmaFp64MaxNbf functions, where the density kernel makes column passes and the
+    // library DGEMMs win; mode 1: library DGEMMs for both; mode 2: MFMA kernels for both
+    const bool mfma = c->xc_mode == 2 ||
+                      (c->xc_mode == 0 && (c->precision == QCG_PREC_MIXED || c->basis.nbf <= kXcMfmaFp64MaxNbf));
+    if (mfma) return xc_build_mfma(c, dD, dV, dExc, st);
     int rc = ensure_blas(c);
     if (rc) return rc;
     const int nbf = c->basis.nbf;

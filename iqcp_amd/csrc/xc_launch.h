@@ -33,13 +33,13 @@ hipError_t launch_xc_mfma(int G, int nbf, int nchunk, const double* w, const dou
                           double* Vparts, hipStream_t st);
 // Output tile width of the MFMA V kernel in 16-wide MFMA tiles per wavefront (VW; a block is
 // 2 x 2 wavefronts, a 32 VW square): the width whose tiles pad the nbf x nbf matrix least, the
-// larger one on ties; at most 4 in FP64, whose accumulators take twice the registers.  nbf = 304
-// (19 MFMA tiles) in FP32: VW = 5, 2 x 2 tiles of 160 covering 320^2 instead of 3 x 3 tiles of 128
-// covering 384^2.
-inline int xc_v_width(int nbf, bool fp32) {
+// larger one on ties.  nbf = 304 (19 MFMA tiles): VW = 5, 2 x 2 tiles of 160 covering 320^2
+// instead of 3 x 3 tiles of 128 covering 384^2 (in FP64 the 25 accumulators of a wavefront take
+// 200 AGPRs, one wavefront per SIMD, and it still wins).
+inline int xc_v_width(int nbf, bool /*fp32*/) {
     const int n16 = (nbf + 15) / 16;
     int best = 4, bpad = -1;
-    for (int w = fp32 ? 5 : 4; w >= 3; --w) {
+    for (int w = 5; w >= 3; --w) {
         const int t = (n16 + 2 * w - 1) / (2 * w) * 2 * w;
         if (bpad < 0 || t * t < bpad) {
             bpad = t * t;
@@ -54,6 +54,9 @@ inline int xc_mfma_vtiles(int nbf, bool fp32) {
     const int t = (nbf + b - 1) / b;
     return t * t;
 }
+// Blocks of the MFMA V kernel (FP32 or FP64 products) resident per CU at this basis size's tile
+// width, from the runtime's occupancy query
+int xc_v_blocks_per_cu(int nbf, bool fp32);
 inline int xc_mfma_chunks(int G, int nchunk) {
     const int chunk = (G + nchunk - 1) / nchunk;
     return (G + chunk - 1) / chunk;

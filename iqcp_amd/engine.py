@@ -311,8 +311,9 @@ class Engine:
         self._check(self.L.qcg_set_graph(self.ctx, int(bool(on))))
 
     def set_xc_mode(self, mode: int):
-        """XC kernels: 0 = FP32 MFMA kernels for mixed, library DGEMMs for FP64 (default);
-        1 = library DGEMMs for both; 2 = MFMA kernels for both."""
+        """XC kernels: 0 = MFMA kernels, FP32 products for mixed and FP64 products for FP64 up to
+        352 basis functions, library DGEMMs for FP64 above (default); 1 = library DGEMMs for both;
+        2 = MFMA kernels for both."""
         self._check(self.L.qcg_set_xc_mode(self.ctx, int(mode)))
 
     def set_prim_split(self, split: int):
