@@ -4,14 +4,18 @@
     python -m iqcp_amd.rocprof_summary counters --pmc DIR [--pmc DIR ...] --stats KERNEL_STATS.csv --md OUT.md
     python -m iqcp_amd.rocprof_summary traffic  --fetch DIR --write DIR --copy-fetch DIR --copy-write DIR
                                                 --tiles TILES.json --json OUT.json --md OUT.md
+    python -m iqcp_amd.rocprof_summary mfma     --pmc DIR [--pmc DIR ...] --stats KERNEL_STATS.csv
+                                                --match SUBSTRING [--match ...] --md OUT.md
 
 `stats` tabulates a `rocprofv3 --kernel-trace --stats` kernel_stats.csv.  `counters` averages the
 counters of one or more `rocprofv3 --pmc` runs per dispatch of each J/K kernel and derives wave
 states (fractions of SQ_WAVE_CYCLES), VALU issue rate and counted FP64 FLOP/s against the average
 durations of a kernel-trace run.  `traffic` turns FETCH_SIZE / WRITE_SIZE runs into HBM bytes per
 launch, calibrated by the same counters on `bench.py --copy-probe` (a 1 GiB device copy), and writes
-the JSON bench.py reads for its roofline entry.  DIR is a rocprofv3 output directory (searched for
-*counter_collection.csv); each --command is copied into the summary verbatim.
+the JSON bench.py reads for its roofline entry.  `mfma` does the same per dispatch for the kernels
+whose names contain a --match string (the XC grid products): wave states, MFMA instructions, MFMA busy
+fraction and the MFMA FLOP/s those instructions amount to.  DIR is a rocprofv3 output directory
+(searched for *counter_collection.csv); each --command is copied into the summary verbatim.
 """
 from __future__ import annotations
 
@@ -26,6 +30,9 @@ from pathlib import Path
 # MI355X: 256 CUs x 4 SIMDs, 2.4 GHz peak engine clock; a wave64 VALU instruction occupies a SIMD for
 # 4 cycles; FP64 vector peak 78.6 TFLOP/s (= 1024 SIMDs x 16 lanes x 2 flops x 2.4 GHz)
 SIMDS, CLOCK_HZ, CYCLES_PER_VALU, PEAK_F64 = 1024, 2.4e9, 4, 78.6e12
+# dense MFMA peaks (FP32 157.3, FP64 78.6 TFLOP/s); a 16x16x4 MFMA instruction is 2048 flops
+PEAK_MFMA = {"F32": 157.3e12, "F64": 78.6e12}
+MFMA_FLOPS = 2 * 16 * 16 * 4
 
 KERNEL_RE = re.compile(r"(jk_wave_kernel_mp|jk_kernel_mp|jk_wave_kernel|jk_kernel)<(\d), (\d), (\d), (\d)"
                        r"(?:, (double|float))?(?:, (\d))?>")
@@ -197,10 +204,47 @@ def cmd_traffic(a):
     Path(a.md).write_text(s + a.note)
 
 
+def cmd_mfma(a):
+    cnt, ndisp, res = {}, {}, {}
+    for d in a.pmc:
+        c, n, r = load_counters(d)
+        for k, v in c.items():
+            cnt.setdefault(k, {}).update(v)
+            ndisp[k] = n[k]
+            res[k] = r[k]
+    st = load_stats(a.stats) if a.stats else {}
+    keys = [k for k in cnt if any(m in k for m in a.match)]
+    keys.sort(key=lambda k: -(st.get(k, (0, 0))[0] * st.get(k, (0, 0))[1]))
+    s = header(a.title, a.command)
+    s += ("Per dispatch: wave states as fractions of SQ_WAVE_CYCLES (waiting on a counter, waiting to issue, "
+          "issuing VALU or LDS instructions); MFMA instructions (SQ_INSTS_VALU_MFMA_F32 / _F64, "
+          f"{MFMA_FLOPS} flops each for the 16x16x4 shapes these kernels use); MFMA busy = "
+          f"SQ_VALU_MFMA_BUSY_CYCLES / ({SIMDS} SIMDs x 2.4 GHz x avg duration); MFMA TFLOP/s against the dense "
+          "peak of the product type (FP32 157.3, FP64 78.6).  Average durations come from the kernel-trace run.\n\n")
+    cols = ["kernel", "dispatches", "VGPR_Count", "avg µs", "wait on counter", "wait to issue", "VALU", "LDS",
+            "MFMA instr / dispatch", "MFMA busy", "MFMA TFLOP/s", "of peak"]
+    s += "| " + " | ".join(cols) + " |\n|---|" + "---:|" * (len(cols) - 1) + "\n"
+    for k in keys:
+        c = cnt[k]
+        dur = st.get(k, (0, 0.0))[1] * 1e-6
+        wc = c.get("SQ_WAVE_CYCLES", 0)
+        states = [f"{c.get(x, 0) / wc:.2f}" if wc else "-" for x in
+                  ("SQ_WAIT_ANY", "SQ_WAIT_INST_ANY", "SQ_ACTIVE_INST_VALU", "SQ_ACTIVE_INST_LDS")]
+        kind = "F64" if c.get("SQ_INSTS_VALU_MFMA_F64", 0) > c.get("SQ_INSTS_VALU_MFMA_F32", 0) else "F32"
+        n = c.get(f"SQ_INSTS_VALU_MFMA_{kind}", 0)
+        busy = c.get("SQ_VALU_MFMA_BUSY_CYCLES", 0) / (SIMDS * CLOCK_HZ * dur) if dur else 0.0
+        fl = n * MFMA_FLOPS / dur if dur else 0.0
+        row = [f"`{k}`", str(ndisp[k]), str(res[k][0]), f"{dur * 1e6:.1f}" if dur else "-"] + states + [
+            f"{n / 1e6:.2f} M ({kind})", f"{busy:.2f}" if dur else "-", f"{fl / 1e12:.1f}" if dur else "-",
+            f"{fl / PEAK_MFMA[kind]:.3f}" if dur else "-"]
+        s += "| " + " | ".join(row) + " |\n"
+    Path(a.md).write_text(s + a.note)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest="what", required=True)
-    for name in ("stats", "counters", "traffic"):
+    for name in ("stats", "counters", "traffic", "mfma"):
         p = sub.add_parser(name)
         p.add_argument("--md", required=True)
         p.add_argument("--title", default="rocprofv3 summary")
@@ -209,14 +253,16 @@ def main(argv=None):
         if name == "stats":
             p.add_argument("--csv", required=True)
             p.add_argument("--top", type=int, default=40)
-        elif name == "counters":
+        elif name in ("counters", "mfma"):
             p.add_argument("--pmc", action="append", required=True)
             p.add_argument("--stats", default=None)
+            if name == "mfma":
+                p.add_argument("--match", action="append", required=True, help="substring of the kernel names")
         else:
             for o in ("--fetch", "--write", "--copy-fetch", "--copy-write", "--tiles", "--json"):
                 p.add_argument(o, required=True)
     a = ap.parse_args(argv)
-    {"stats": cmd_stats, "counters": cmd_counters, "traffic": cmd_traffic}[a.what](a)
+    {"stats": cmd_stats, "counters": cmd_counters, "traffic": cmd_traffic, "mfma": cmd_mfma}[a.what](a)
 
 
 if __name__ == "__main__":

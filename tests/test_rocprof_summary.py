@@ -80,3 +80,28 @@ def test_counters_states_and_f64(tmp_path):
     assert cells[11] == f"{6.144e6 * 4 / (1024 * 2.4e9 * 1e-4):.2f}"
     fl = 64 * (2e6 + 5e5 + 5e5) / 1e-4
     assert cells[12] == "2.01 M" and cells[13] == f"{fl / 1e12:.2f}" and cells[14] == f"{fl / 78.6e12:.3f}"
+
+
+def test_mfma_states_and_rate(tmp_path):
+    k = "void qcg::xc_v_kernel<float, 5>(int, int)"
+    other = "void qcg::jk_kernel<0, 0, 0, 0, double, 1>(int)"
+    write_counters(tmp_path / "s", [
+        [1, k, 120, 0, "SQ_WAVES", 4096.0], [1, k, 120, 0, "SQ_WAVE_CYCLES", 1e6],
+        [1, k, 120, 0, "SQ_WAIT_ANY", 3e5], [1, k, 120, 0, "SQ_WAIT_INST_ANY", 2e5],
+        [1, k, 120, 0, "SQ_ACTIVE_INST_VALU", 1e5], [1, k, 120, 0, "SQ_ACTIVE_INST_LDS", 5e4],
+        [1, other, 64, 0, "SQ_WAVES", 10.0]])
+    write_counters(tmp_path / "m", [
+        [1, k, 120, 0, "SQ_INSTS_VALU_MFMA_F32", 9.6e7], [1, k, 120, 0, "SQ_INSTS_VALU_MFMA_F64", 0.0],
+        [1, k, 120, 0, "SQ_VALU_MFMA_BUSY_CYCLES", 3.072e9]])
+    write_stats(tmp_path / "ks.csv", [[k, 10, 2e7, 2000000.0, 40.0, 1900000, 2100000, 1.0]])
+    rs.main(["mfma", "--pmc", str(tmp_path / "s"), "--pmc", str(tmp_path / "m"), "--stats", str(tmp_path / "ks.csv"),
+             "--match", "xc_", "--md", str(tmp_path / "x.md")])
+    rows = [l for l in (tmp_path / "x.md").read_text().splitlines() if l.startswith("| `")]
+    assert len(rows) == 1 and "jk_kernel" not in rows[0]
+    cells = [c.strip() for c in rows[0].strip("|").split("|")]
+    # dispatches, VGPRs, avg µs, four wave states, MFMA instructions, busy, TFLOP/s, fraction of peak
+    assert cells[1:8] == ["1", "120", "2000.0", "0.30", "0.20", "0.10", "0.05"]
+    assert cells[8] == "96.00 M (F32)"
+    assert cells[9] == f"{3.072e9 / (1024 * 2.4e9 * 2e-3):.2f}"
+    fl = 9.6e7 * 2048 / 2e-3
+    assert cells[10] == f"{fl / 1e12:.1f}" and cells[11] == f"{fl / 157.3e12:.3f}"
