@@ -130,7 +130,7 @@ int xc_build_mfma(qcg_context* c, const double* dD, double* dV, double* dExc, hi
     const bool mixed = c->precision == QCG_PREC_MIXED;
     const int nch = xc_mfma_chunks(G, mixed ? c->v_chunks_mfma32 : c->v_chunks_mfma);
     HIPCHK(c, launch_fill(dExc, 1, 0.0, st));
-    if (c->precision == QCG_PREC_MIXED) {
+    if (mixed) {
         int rc = ensure_xc32(c, st);
         if (rc) return rc;
         HIPCHK(c, launch_to_f32(dD, c->d_D32, (size_t)nbf * nbf, st));
@@ -142,7 +142,7 @@ int xc_build_mfma(qcg_context* c, const double* dD, double* dV, double* dExc, hi
     }
     HIPCHK(c, launch_sum_parts(c->d_Vparts, nch, (size_t)nbf * nbf, c->d_V, st));
     HIPCHK(c, launch_sym_half(c->d_V, dV, nbf, st));
-    if (c->precision == QCG_PREC_MIXED)
+    if (mixed)
         HIPCHK(c, launch_exc_d32_correction(dV, dD, c->d_D32, (size_t)nbf * nbf, dExc, st));
     return QCG_OK;
 }
