@@ -235,7 +235,10 @@ int qcg_xc_eval(qcg_context* ctx, int64_t n, const double* rho, const double* si
                 double* vsigma);
 /* V_xc (symmetrised) and E_xc for density D (build_vxc_gga_matmul, dft/ks_scf.rs:1150). */
 int qcg_xc(qcg_context* ctx, const double* D, double* Vxc, double* Exc);
-/* Device-pointer variant; *dExc is written on the device. */
+/* Device-pointer variant; *dExc is written on the device.  It uses none of the J/K entry points'
+ * buffers, so it may run on another stream concurrently with qcg_build_jk_device on the same
+ * context and density (qcg_ks_scf's Fock build does so); two XC builds of one context must not
+ * overlap. */
 int qcg_xc_device(qcg_context* ctx, const double* dD, double* dVxc, double* dExc, void* stream);
 /* XC build kernels.  The hand-written MFMA kernels form X = Phi D block by block and reduce it into
  * rho on the fly, then V = Phi^T aow by tiles; the library form runs DGEMMs around a point kernel.

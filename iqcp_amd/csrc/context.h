@@ -136,6 +136,9 @@ struct qcg_context {
     static constexpr int kStreams = QCG_STREAMS;
     hipStream_t aux[kStreams] = {};
     hipEvent_t ev_fork = nullptr, ev_join[kStreams] = {};
+    // the KS Fock build's XC stream, concurrent with the J/K class kernels (made on first use)
+    hipStream_t xc_stream = nullptr;
+    hipEvent_t ev_xc_fork = nullptr, ev_xc_join = nullptr;
     // cached class counts / flop model, valid for (precision, fp32_cutoff, schwarz_thr) of stats_key
     std::vector<qcg_class_stat> stats;
     // exchange-correlation grid and per-point buffers

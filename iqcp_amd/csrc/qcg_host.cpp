@@ -77,6 +77,7 @@ void qcg_destroy(qcg_context* c) {
     if (c->ev_done) (void)hipEventSynchronize(c->ev_done);  // the last entry point's work on any stream
     for (int i = 0; i < qcg_context::kStreams; ++i)
         if (c->aux[i]) (void)hipStreamSynchronize(c->aux[i]);
+    if (c->xc_stream) (void)hipStreamSynchronize(c->xc_stream);
     for (void* p : c->owned) (void)hipFree(p);
     c->owned.clear();
     if (c->ev_made)
@@ -89,6 +90,11 @@ void qcg_destroy(qcg_context* c) {
         }
     }
     if (c->ev_done) (void)hipEventDestroy(c->ev_done);
+    if (c->xc_stream) {
+        (void)hipEventDestroy(c->ev_xc_fork);
+        (void)hipEventDestroy(c->ev_xc_join);
+        (void)hipStreamDestroy(c->xc_stream);
+    }
     if (c->blas) (void)rocblas_destroy_handle(c->blas);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;

@@ -166,3 +166,37 @@ def test_gpu_b3lyp_energy_mixed_vs_iqcp(mol, bas):
     out = eng.ks_scf(ScfConfig.tight())
     assert out["converged"] == 1
     assert abs(out["energy_total"] - iqcp_energy(mol, bas, "B3LYP")) < 1e-6, out["energy_total"]
+
+
+def test_gpu_xc_concurrent_with_jk():
+    """qcg_xc_device on a second stream, concurrent with qcg_build_jk_device on the first (the KS
+    Fock build's overlap), gives the same J, K and V_xc bit for bit as the two calls one after the
+    other on one stream, and the same E_xc (a sum of per-block partials, to 1e-13 relative); FP64
+    and mixed, (H2O)4/6-31G* on its 75 x 302 grid."""
+    import torch
+    b = build_basis(workloads.water_cluster((4, 1, 1), 2.9, seed=4), "6-31g*", units="angstrom")
+    eng = Engine(b)
+    eng.build_grid(75, "standard", False)
+    dev = torch.device("cuda:0")
+    dD = torch.tensor(random_density(b.nbf, seed=4, nocc=b.n_occ) * 0.5, device=dev)
+    s1, s2 = torch.cuda.Stream(device=dev), torch.cuda.Stream(device=dev)
+    for prec in ("fp64", "mixed"):
+        eng.set_options(prec)
+        out = []
+        for concurrent in (False, True):
+            dJ, dK, dV = torch.empty_like(dD), torch.empty_like(dD), torch.empty_like(dD)
+            dE = torch.zeros(1, dtype=torch.float64, device=dev)
+            xs = s2 if concurrent else s1
+            fork = torch.cuda.Event()
+            fork.record(s1)
+            xs.wait_event(fork)
+            eng.build_jk_device(dD.data_ptr(), dJ.data_ptr(), dK.data_ptr(), s1.cuda_stream)
+            eng.xc_device(dD.data_ptr(), dV.data_ptr(), dE.data_ptr(), xs.cuda_stream)
+            join = torch.cuda.Event()
+            join.record(xs)
+            s1.wait_event(join)
+            torch.cuda.synchronize()
+            out.append((dJ.cpu().numpy(), dK.cpu().numpy(), dV.cpu().numpy(), float(dE.cpu()[0])))
+        (J0, K0, V0, E0), (J1, K1, V1, E1) = out
+        assert np.array_equal(J0, J1) and np.array_equal(K0, K1) and np.array_equal(V0, V1), prec
+        assert abs(E1 - E0) <= 1e-13 * abs(E0), (prec, E0, E1)
