@@ -310,9 +310,7 @@ def main():
         dDw = torch.tensor(sw["D"], device=dev)
         dJw, dKw, dVw = torch.empty_like(dDw), torch.empty_like(dDw), torch.empty_like(dDw)
         dEw = torch.zeros(1, dtype=torch.float64, device=dev)
-        stream_xc = torch.cuda.Stream(device=dev)
-        ks = {"workload": "B3LYP/6-31G* (H2O)16, J + K + V_xc build at the converged KS density "
-                          "(J/K and V_xc on two streams, joined per build)",
+        ks = {"workload": "B3LYP/6-31G* (H2O)16, J + K + V_xc build at the converged KS density",
               "nbf": bw.nbf, "grid_points": npts, "scf_energy": sw["energy_total"], "scf_iterations": sw["iterations"]}
         for prec in ("fp64", "mixed"):
             ew.set_options(prec, args.fp32_cutoff)
@@ -330,27 +328,16 @@ def main():
                 e1.record(stream)
                 torch.cuda.synchronize()
             ks[f"{prec}_xc_ms"] = round(e0.elapsed_time(e1) / nx, 3)
-
-            # one J+K+XC build: J/K on `stream`, V_xc on `stream_xc` concurrently (they share no
-            # buffer), joined before the next build starts, as qcg_ks_scf's Fock build does
-            def ks_build():
-                fork = torch.cuda.Event()
-                fork.record(stream)
-                stream_xc.wait_event(fork)
-                ew.build_jk_device(dDw.data_ptr(), dJw.data_ptr(), dKw.data_ptr(), stream.cuda_stream)
-                ew.xc_device(dDw.data_ptr(), dVw.data_ptr(), dEw.data_ptr(), stream_xc.cuda_stream)
-                join = torch.cuda.Event()
-                join.record(stream_xc)
-                stream.wait_event(join)
-
             with torch.cuda.stream(stream):
                 for _ in range(2):
-                    ks_build()
+                    ew.build_jk_device(dDw.data_ptr(), dJw.data_ptr(), dKw.data_ptr(), stream.cuda_stream)
+                    ew.xc_device(dDw.data_ptr(), dVw.data_ptr(), dEw.data_ptr(), stream.cuda_stream)
                 torch.cuda.synchronize()
                 nk = max(1, args.steps // 2)
                 t = time.perf_counter()
                 for _ in range(nk):
-                    ks_build()
+                    ew.build_jk_device(dDw.data_ptr(), dJw.data_ptr(), dKw.data_ptr(), stream.cuda_stream)
+                    ew.xc_device(dDw.data_ptr(), dVw.data_ptr(), dEw.data_ptr(), stream.cuda_stream)
                 torch.cuda.synchronize()
                 el_k = time.perf_counter() - t
             Dn = sw["D"]
